@@ -1,0 +1,10 @@
+"""deco_amd: MI355X-native CPR-preconditioned Krylov solve path of deco.
+
+The numerical work lives in libdeco_hip.so (hand-written HIP kernels for gfx950, C ABI in
+include/deco_hip.h). Importing this package does not touch the GPU; the library is loaded on
+first use and its absence is an error.
+"""
+from .ptree import PTree, remove_json_comments  # noqa: F401
+from .core import (  # noqa: F401
+    CSR, DeviceArray, as_dev, init, dot, nrm2, axpby, generate, cpr_params, sol_params, lib,
+)

@@ -1,0 +1,181 @@
+// Shared infrastructure of libdeco_hip.so: error state, the library stream, device buffers,
+// launch helpers and wave64 reduction primitives for gfx950.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <cstdarg>
+#include <string>
+#include <vector>
+#include <stdexcept>
+#include <type_traits>
+#include "deco_hip.h"
+
+namespace deco {
+
+// ------------------------------------------------------------------ errors -------------
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+[[noreturn]] void fail(int code, const char* fmt, ...);
+void set_error(const char* msg);
+
+#define DECO_HIP(expr)                                                                  \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      ::deco::fail(e_ == hipErrorOutOfMemory ? DECO_ERR_NOMEM : DECO_ERR_DEVICE,        \
+                   "%s:%d %s: %s", __FILE__, __LINE__, #expr, hipGetErrorString(e_));   \
+  } while (0)
+
+#define DECO_CHECK_LAUNCH() DECO_HIP(hipGetLastError())
+
+// Wraps a C-ABI body: converts exceptions into status codes.
+template <class F>
+int guard(F&& f) {
+  try {
+    f();
+    return DECO_OK;
+  } catch (const Error& e) {
+    set_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_error("host allocation failed");
+    return DECO_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return DECO_ERR_ARG;
+  }
+}
+
+// ------------------------------------------------------------------ context ------------
+struct Context {
+  int device = -1;
+  hipStream_t stream = nullptr;
+  int num_cu = 0;
+  size_t lds_per_block = 0;
+  // pinned host staging for scalar readbacks
+  double* pinned = nullptr;
+};
+Context& ctx();
+void ensure_init();
+inline hipStream_t stream() { ensure_init(); return ctx().stream; }
+void sync();
+
+// ------------------------------------------------------------------ device buffer ------
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  explicit DBuf(size_t count) { alloc(count); }
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DBuf& operator=(DBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    ensure_init();
+    if (count) DECO_HIP(hipMalloc(&p, count * sizeof(T)));
+    n = count;
+  }
+  void ensure(size_t count) { if (count > n) alloc(count); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  T* release_ownership() { T* q = p; p = nullptr; n = 0; return q; }
+  void upload(const T* h, size_t count) {
+    ensure(count);
+    if (count) DECO_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, stream()));
+  }
+  void download(T* h, size_t count) const {
+    if (count) DECO_HIP(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, stream()));
+    sync();
+  }
+  void zero() { if (n) DECO_HIP(hipMemsetAsync(p, 0, n * sizeof(T), stream())); }
+  std::vector<T> to_host() const { std::vector<T> h(n); download(h.data(), n); return h; }
+};
+
+template <class T>
+void copy_d2d(T* dst, const T* src, size_t count) {
+  if (count) DECO_HIP(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDeviceToDevice, stream()));
+}
+template <class T>
+void copy_h2d(T* dst, const T* src, size_t count) {
+  if (count) DECO_HIP(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice, stream()));
+}
+template <class T>
+void copy_d2h(T* dst, const T* src, size_t count) {
+  if (count) DECO_HIP(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDeviceToHost, stream()));
+  sync();
+}
+
+inline unsigned grid_for(int64_t work, int block) {
+  int64_t g = (work + block - 1) / block;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+// ------------------------------------------------------------------ wave64 reductions ---
+// Sum over the G lanes of a lane group (G a power of two <= 64, groups aligned in the wave).
+template <int G, class T>
+__device__ __forceinline__ T group_sum(T v) {
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, G);
+  return v;
+}
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) { return group_sum<64>(v); }
+
+// Block-wide sum of one value per thread, fixed order (deterministic). BLOCK multiple of 64.
+template <int BLOCK, class T>
+__device__ __forceinline__ T block_sum(T v, T* smem /* BLOCK/64 entries */) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) smem[wid] = v;
+  __syncthreads();
+  T s = T(0);
+  if (threadIdx.x < 64) {
+    s = (lane < BLOCK / 64) ? smem[lane] : T(0);
+    s = wave_sum(s);
+  }
+  __syncthreads();
+  return s;  // valid in wave 0
+}
+
+// Fixed-grid size for reductions over n elements: depends only on n, so results are
+// bit-reproducible run to run.
+constexpr int kRedBlock = 256;
+constexpr int kRedMaxGrid = 1024;
+inline int red_grid(int64_t n) {
+  int64_t g = (n + kRedBlock * 4 - 1) / (kRedBlock * 4);
+  if (g < 1) g = 1;
+  if (g > kRedMaxGrid) g = kRedMaxGrid;
+  return (int)g;
+}
+
+// Sum `count` partials in a fixed order with the first wave of the block; every block that
+// calls it gets the bitwise-same value.
+__device__ __forceinline__ double reduce_partials(const double* __restrict__ part, int count) {
+  __shared__ double sh_rp[1];
+  if (threadIdx.x < 64) {
+    double s = 0.0;
+    for (int k = threadIdx.x; k < count; k += 64) s += part[k];
+    s = wave_sum(s);
+    if (threadIdx.x == 0) sh_rp[0] = s;
+  }
+  __syncthreads();
+  double r = sh_rp[0];
+  __syncthreads();
+  return r;
+}
+
+}  // namespace deco

@@ -1,0 +1,713 @@
+// Runtime (errors, stream, memory) and the device CSR container with its kernels:
+// create/copy/download, filters, pressure extraction, transpose, SpMV.
+// Reference: core/CSR.hpp (container and methods), core/alg.hpp:51-64 (spmv).
+#include "csr.hpp"
+#include <climits>
+#include <cstring>
+#include <mutex>
+
+namespace deco {
+
+static thread_local std::string t_last_error;
+static Context g_ctx;
+static std::mutex g_init_mu;
+
+void set_error(const char* msg) { t_last_error = msg ? msg : ""; }
+
+void fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  throw Error(code, buf);
+}
+
+Context& ctx() { return g_ctx; }
+
+static void init_device(int device) {
+  std::lock_guard<std::mutex> lk(g_init_mu);
+  if (g_ctx.stream && (device < 0 || device == g_ctx.device)) return;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0) fail(DECO_ERR_DEVICE, "no HIP device available (%s)", hipGetErrorString(e));
+  if (device < 0) device = 0;
+  if (device >= count) fail(DECO_ERR_ARG, "device %d out of range (%d devices)", device, count);
+  DECO_HIP(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  DECO_HIP(hipGetDeviceProperties(&prop, device));
+  if (g_ctx.stream) { (void)hipStreamDestroy(g_ctx.stream); g_ctx.stream = nullptr; }
+  // default-priority, non-blocking stream
+  DECO_HIP(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
+  if (!g_ctx.pinned) DECO_HIP(hipHostMalloc(&g_ctx.pinned, 4096 * sizeof(double), hipHostMallocDefault));
+  g_ctx.device = device;
+  g_ctx.num_cu = prop.multiProcessorCount;
+  g_ctx.lds_per_block = prop.sharedMemPerBlock;
+}
+
+void ensure_init() {
+  if (!g_ctx.stream) init_device(-1);
+}
+
+void sync() { DECO_HIP(hipStreamSynchronize(stream())); }
+
+// ------------------------------------------------------------------ scan ---------------
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+// inclusive scan of one int per thread over the block
+__device__ __forceinline__ int block_inclusive_scan(int v, int* smem /* 4 */) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    int t = __shfl_up(v, off, 64);
+    if (lane >= off) v += t;
+  }
+  if (lane == 63) smem[wid] = v;
+  __syncthreads();
+  int add = 0;
+  for (int w = 0; w < wid; ++w) add += smem[w];
+  __syncthreads();
+  return v + add;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_tiles(const int* __restrict__ in, int* __restrict__ out,
+                                                          int n, int* __restrict__ tile_sums) {
+  __shared__ int sm[4];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  int vals[kScanItems];
+  int tsum = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    int64_t i = base + k;
+    vals[k] = (i < n) ? in[i] : 0;
+    tsum += vals[k];
+  }
+  int incl = block_inclusive_scan(tsum, sm);
+  int run = incl - tsum;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    int64_t i = base + k;
+    if (i < n) out[i] = run;
+    run += vals[k];
+  }
+  if (threadIdx.x == kScanBlock - 1) tile_sums[blockIdx.x] = incl;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_add(int* __restrict__ out, int n,
+                                                        const int* __restrict__ tile_off) {
+  const int64_t base = (int64_t)blockIdx.x * kScanTile;
+  const int add = tile_off[blockIdx.x];
+  for (int k = threadIdx.x; k < kScanTile; k += kScanBlock) {
+    int64_t i = base + k;
+    if (i < n) out[i] += add;
+  }
+}
+
+__global__ void k_set_total(const int* __restrict__ in, int* __restrict__ out, int n) {
+  // out[n] = out[n-1] + in[n-1]  (in already overwritten when aliasing is handled by caller)
+  if (threadIdx.x == 0 && blockIdx.x == 0) out[n] = (n > 0) ? out[n - 1] + in[0] : 0;
+}
+
+// Exclusive scan of in[0..n) into out[0..n) (out may alias in), recursive over tile sums.
+static void exclusive_scan(const int* in, int* out, int n) {
+  if (n <= 0) return;
+  const int tiles = (n + kScanTile - 1) / kScanTile;
+  DBuf<int> sums(tiles + 1);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(kScanBlock), 0, stream(), in, out, n, sums.p);
+  DECO_CHECK_LAUNCH();
+  if (tiles > 1) {
+    exclusive_scan(sums.p, sums.p, tiles);
+    hipLaunchKernelGGL(k_scan_add, dim3(tiles), dim3(kScanBlock), 0, stream(), out, n, sums.p);
+    DECO_CHECK_LAUNCH();
+  }
+}
+
+__global__ void k_last_plus(const int* __restrict__ last_in, int* __restrict__ out, int n) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) out[n] = out[n - 1] + last_in[0];
+}
+
+void scan_to_ptr_async(const int* in, int* out, int n) {
+  if (n <= 0) {
+    DECO_HIP(hipMemsetAsync(out, 0, sizeof(int), stream()));
+    return;
+  }
+  // keep the last count before a possibly aliasing scan overwrites it
+  DBuf<int> last(1);
+  copy_d2d(last.p, in + (n - 1), 1);
+  exclusive_scan(in, out, n);
+  hipLaunchKernelGGL(k_last_plus, dim3(1), dim3(1), 0, stream(), last.p, out, n);
+  DECO_CHECK_LAUNCH();
+}
+
+int64_t scan_to_ptr(const int* in, int* out, int n) {
+  scan_to_ptr_async(in, out, n);
+  int tot = 0;
+  copy_d2h(&tot, out + n, 1);
+  return tot;
+}
+
+deco_csr* csr_new(int n, int m) {
+  auto* A = new deco_csr();
+  A->n = n;
+  A->m = m;
+  return A;
+}
+
+int build_row_ptr(deco_csr* dst, const int* counts, int n) {
+  dst->r.alloc(n + 1);
+  int64_t nnz = scan_to_ptr(counts, dst->r.p, n);
+  if (nnz > INT_MAX) fail(DECO_ERR_UNSUPPORTED, "nnz %lld exceeds int32", (long long)nnz);
+  dst->nnz = (int)nnz;
+  dst->c.alloc(nnz);
+  dst->v.alloc(nnz);
+  dst->invalidate();
+  return (int)nnz;
+}
+
+// ------------------------------------------------------------------ row sort -----------
+__global__ void k_row_len(int n, const int* __restrict__ r, int* __restrict__ len) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) len[i] = r[i + 1] - r[i];
+}
+
+__global__ void k_max_reduce(const int* __restrict__ a, int n, int* __restrict__ out) {
+  __shared__ int sm[4];
+  int m = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) m = max(m, a[i]);
+  for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int mm = sm[0];
+    for (int w = 1; w < (int)(blockDim.x / 64); ++w) mm = max(mm, sm[w]);
+    atomicMax(out, mm);
+  }
+}
+
+int max_row_length(const deco_csr* A) {
+  if (A->n == 0) return 0;
+  DBuf<int> len(A->n), mx(1);
+  mx.zero();
+  hipLaunchKernelGGL(k_row_len, dim3(grid_for(A->n, 256)), dim3(256), 0, stream(), A->n, A->r.p, len.p);
+  DECO_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_max_reduce, dim3(std::min<unsigned>(1024, grid_for(A->n, 256))), dim3(256), 0, stream(),
+                     len.p, A->n, mx.p);
+  DECO_CHECK_LAUNCH();
+  int h = 0;
+  copy_d2h(&h, mx.p, 1);
+  return h;
+}
+
+// insertion sort per row for short rows (one thread per row)
+__global__ void k_sort_rows_small(int n, const int* __restrict__ r, int* __restrict__ c, double* __restrict__ v,
+                                  int limit) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = r[i], e = r[i + 1];
+  if (e - b > limit) return;
+  for (int j = b + 1; j < e; ++j) {
+    int key = c[j];
+    double val = v ? v[j] : 0.0;
+    int k = j - 1;
+    while (k >= b && c[k] > key) {
+      c[k + 1] = c[k];
+      if (v) v[k + 1] = v[k];
+      --k;
+    }
+    c[k + 1] = key;
+    if (v) v[k + 1] = val;
+  }
+}
+
+// bitonic sort in LDS for rows longer than the small limit (one block per row, <= 4096)
+constexpr int kSortMax = 4096;
+__global__ __launch_bounds__(256) void k_sort_rows_large(int n, const int* __restrict__ r, int* __restrict__ c,
+                                                         double* __restrict__ v, int limit) {
+  __shared__ int sk[kSortMax];
+  __shared__ double sv[kSortMax];
+  const int i = blockIdx.x;
+  const int b = r[i], e = r[i + 1], len = e - b;
+  if (len <= limit) return;
+  int P = 1;
+  while (P < len) P <<= 1;
+  for (int k = threadIdx.x; k < P; k += 256) {
+    sk[k] = (k < len) ? c[b + k] : INT_MAX;
+    sv[k] = (k < len && v) ? v[b + k] : 0.0;
+  }
+  __syncthreads();
+  for (int size = 2; size <= P; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int k = threadIdx.x; k < P; k += 256) {
+        int partner = k ^ stride;
+        if (partner > k) {
+          bool up = ((k & size) == 0);
+          if ((sk[k] > sk[partner]) == up) {
+            int tk = sk[k]; sk[k] = sk[partner]; sk[partner] = tk;
+            double tv = sv[k]; sv[k] = sv[partner]; sv[partner] = tv;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int k = threadIdx.x; k < len; k += 256) {
+    c[b + k] = sk[k];
+    if (v) v[b + k] = sv[k];
+  }
+}
+
+void sort_rows(int n, const int* r, int* c, double* v, int max_row) {
+  if (n <= 0) return;
+  const int small = 32;
+  hipLaunchKernelGGL(k_sort_rows_small, dim3(grid_for(n, 128)), dim3(128), 0, stream(), n, r, c, v, small);
+  DECO_CHECK_LAUNCH();
+  if (max_row > small) {
+    if (max_row > kSortMax) fail(DECO_ERR_UNSUPPORTED, "row of %d entries exceeds the sort limit %d", max_row, kSortMax);
+    hipLaunchKernelGGL(k_sort_rows_large, dim3(n), dim3(256), 0, stream(), n, r, c, v, small);
+    DECO_CHECK_LAUNCH();
+  }
+}
+
+// ------------------------------------------------------------------ transpose ----------
+__global__ void k_col_count(int nnz, const int* __restrict__ c, int* __restrict__ cnt) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < nnz) atomicAdd(&cnt[c[j]], 1);
+}
+
+__global__ void k_transpose_scatter(int n, const int* __restrict__ r, const int* __restrict__ c,
+                                    const double* __restrict__ v, int* __restrict__ cursor,
+                                    int* __restrict__ ct, double* __restrict__ vt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int j = r[i]; j < r[i + 1]; ++j) {
+    int pos = atomicAdd(&cursor[c[j]], 1);
+    ct[pos] = i;
+    vt[pos] = v[j];
+  }
+}
+
+deco_csr* csr_transpose(const deco_csr* A) {
+  deco_csr* T = csr_new(A->m, A->n);
+  DBuf<int> cnt(A->m + 1);
+  cnt.zero();
+  if (A->nnz > 0) {
+    hipLaunchKernelGGL(k_col_count, dim3(grid_for(A->nnz, 256)), dim3(256), 0, stream(), A->nnz, A->c.p, cnt.p);
+    DECO_CHECK_LAUNCH();
+  }
+  build_row_ptr(T, cnt.p, A->m);
+  if (T->nnz > 0) {
+    DBuf<int> cursor(A->m + 1);
+    copy_d2d(cursor.p, T->r.p, A->m + 1);
+    hipLaunchKernelGGL(k_transpose_scatter, dim3(grid_for(A->n, 128)), dim3(128), 0, stream(), A->n, A->r.p,
+                       A->c.p, A->v.p, cursor.p, T->c.p, T->v.p);
+    DECO_CHECK_LAUNCH();
+    sort_rows(T->n, T->r.p, T->c.p, T->v.p, max_row_length(T));
+  }
+  return T;
+}
+
+deco_csr* csr_clone(const deco_csr* A) {
+  deco_csr* B = csr_new(A->n, A->m);
+  B->nnz = A->nnz;
+  B->r.alloc(A->n + 1);
+  B->c.alloc(A->nnz);
+  B->v.alloc(A->nnz);
+  copy_d2d(B->r.p, A->r.p, A->n + 1);
+  copy_d2d(B->c.p, A->c.p, A->nnz);
+  copy_d2d(B->v.p, A->v.p, A->nnz);
+  return B;
+}
+
+// ------------------------------------------------------------------ filters ------------
+// Diagonal position search: core::binarysearch (core/alg.hpp:16-25) returns the last index
+// whose column is <= target inside [lower, upper). Reproduced exactly (used for the diagonal
+// lookup of filter_proper, CSR.hpp:486).
+__device__ __forceinline__ int dev_binarysearch(const int* __restrict__ arr, int lower, int upper, int target) {
+  int mn = lower, mx = upper, mid = (lower + upper) >> 1;
+  while (mx - mn > 1) {
+    if (arr[mid] > target) mx = mid; else mn = mid;
+    mid = (mn + mx) >> 1;
+  }
+  return mid;
+}
+
+template <int MODE>  // 0: |a| > tol ; 1: |a| > |a_ii * tol|
+__global__ void k_filter_count(int n, const int* __restrict__ r, const int* __restrict__ c,
+                               const double* __restrict__ v, double tol, int* __restrict__ cnt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = r[i], e = r[i + 1];
+  double D = tol;
+  if (MODE == 1) D = fabs(v[dev_binarysearch(c, b, e, i)] * tol);
+  int k = 0;
+  for (int j = b; j < e; ++j)
+    if (fabs(v[j]) > D || c[j] == i) ++k;
+  cnt[i] = k;
+}
+
+template <int MODE>
+__global__ void k_filter_fill(int n, const int* __restrict__ r, const int* __restrict__ c,
+                              const double* __restrict__ v, double tol, const int* __restrict__ rn,
+                              int* __restrict__ cn, double* __restrict__ vn) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = r[i], e = r[i + 1];
+  double D = tol;
+  if (MODE == 1) D = fabs(v[dev_binarysearch(c, b, e, i)] * tol);
+  int o = rn[i];
+  for (int j = b; j < e; ++j)
+    if (fabs(v[j]) > D || c[j] == i) {
+      cn[o] = c[j];
+      vn[o] = v[j];
+      ++o;
+    }
+}
+
+template <int MODE>
+static void filter_impl(deco_csr* A, double tol) {
+  const int n = A->n;
+  if (n == 0) return;
+  DBuf<int> cnt(n + 1);
+  hipLaunchKernelGGL((k_filter_count<MODE>), dim3(grid_for(n, 256)), dim3(256), 0, stream(), n, A->r.p, A->c.p,
+                     A->v.p, tol, cnt.p);
+  DECO_CHECK_LAUNCH();
+  deco_csr tmp;
+  tmp.n = n;
+  tmp.m = A->m;
+  build_row_ptr(&tmp, cnt.p, n);
+  hipLaunchKernelGGL((k_filter_fill<MODE>), dim3(grid_for(n, 256)), dim3(256), 0, stream(), n, A->r.p, A->c.p,
+                     A->v.p, tol, tmp.r.p, tmp.c.p, tmp.v.p);
+  DECO_CHECK_LAUNCH();
+  A->r = std::move(tmp.r);
+  A->c = std::move(tmp.c);
+  A->v = std::move(tmp.v);
+  A->nnz = tmp.nnz;
+  A->invalidate();
+  if (A->At) { delete A->At; A->At = nullptr; }
+}
+
+// ------------------------------------------------------------------ local_matrix -------
+__global__ void k_local_count(int nl, int bs, int p, const int* __restrict__ r, const int* __restrict__ c,
+                              int* __restrict__ cnt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nl) return;
+  const int row = bs * i + p;
+  int k = 0;
+  for (int j = r[row]; j < r[row + 1]; ++j)
+    if (c[j] % bs == p) ++k;
+  cnt[i] = k;
+}
+
+__global__ void k_local_fill(int nl, int bs, int p, const int* __restrict__ r, const int* __restrict__ c,
+                             const double* __restrict__ v, const double* __restrict__ w,
+                             const int* __restrict__ rl, int* __restrict__ cl, double* __restrict__ vl) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nl) return;
+  const int row = bs * i + p;
+  int o = rl[i];
+  const double scale = w ? w[row] : 1.0;
+  for (int j = r[row]; j < r[row + 1]; ++j)
+    if (c[j] % bs == p) {
+      cl[o] = c[j] / bs;
+      vl[o] = w ? v[j] / scale : v[j];
+      ++o;
+    }
+}
+
+deco_csr* local_matrix(const deco_csr* A, int bs, int p, const double* w) {
+  if (bs <= 0 || A->n % bs != 0 || A->m % bs != 0)
+    fail(DECO_ERR_ARG, "n or m are not divisible by bs: n=%d m=%d bs=%d", A->n, A->m, bs);
+  if (p < 0 || p >= bs) fail(DECO_ERR_ARG, "pressure id %d out of range for bs %d", p, bs);
+  const int nl = A->n / bs;
+  deco_csr* L = csr_new(nl, A->m / bs);
+  DBuf<int> cnt(nl + 1);
+  if (nl > 0) {
+    hipLaunchKernelGGL(k_local_count, dim3(grid_for(nl, 256)), dim3(256), 0, stream(), nl, bs, p, A->r.p, A->c.p,
+                       cnt.p);
+    DECO_CHECK_LAUNCH();
+  }
+  build_row_ptr(L, cnt.p, nl);
+  if (nl > 0) {
+    hipLaunchKernelGGL(k_local_fill, dim3(grid_for(nl, 256)), dim3(256), 0, stream(), nl, bs, p, A->r.p, A->c.p,
+                       A->v.p, w, L->r.p, L->c.p, L->v.p);
+    DECO_CHECK_LAUNCH();
+  }
+  return L;
+}
+
+void csr_filter(deco_csr* A, double tol) { filter_impl<0>(A, tol); }
+void csr_filter_proper(deco_csr* A, double tol) { filter_impl<1>(A, tol); }
+
+// ------------------------------------------------------------------ conversions --------
+__global__ void k_d2f(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
+void to_f32(const double* a, float* b, int64_t n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_d2f, dim3(grid_for(n, 256)), dim3(256), 0, stream(), n, a, b);
+  DECO_CHECK_LAUNCH();
+}
+
+}  // namespace deco
+
+deco_csr::~deco_csr() { delete At; }
+
+const float* deco_csr::values_f32() {
+  if (!vf_valid) {
+    vf.ensure(nnz);
+    deco::to_f32(v.p, vf.p, nnz);
+    vf_valid = true;
+  }
+  return vf.p;
+}
+
+using namespace deco;
+
+// ====================================================================== C ABI ==========
+extern "C" {
+
+int deco_init(int device) { return guard([&] { init_device(device); }); }
+
+const char* deco_last_error(void) { return t_last_error.c_str(); }
+
+int deco_sync(void) { return guard([&] { sync(); }); }
+
+void* deco_stream(void) {
+  try { return (void*)stream(); } catch (...) { return nullptr; }
+}
+
+int deco_malloc(size_t bytes, void** out) {
+  return guard([&] {
+    if (!out) fail(DECO_ERR_ARG, "null output");
+    ensure_init();
+    *out = nullptr;
+    if (bytes) DECO_HIP(hipMalloc(out, bytes));
+  });
+}
+
+int deco_free(void* p) {
+  return guard([&] { if (p) DECO_HIP(hipFree(p)); });
+}
+
+int deco_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+  return guard([&] {
+    if (bytes) DECO_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream()));
+    sync();
+  });
+}
+
+int deco_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+  return guard([&] {
+    if (bytes) DECO_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream()));
+    sync();
+  });
+}
+
+int deco_memcpy_d2d(void* dst, const void* src, size_t bytes) {
+  return guard([&] {
+    if (bytes) DECO_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream()));
+    sync();
+  });
+}
+
+int deco_memset(void* dst, int value, size_t bytes) {
+  return guard([&] {
+    if (bytes) DECO_HIP(hipMemsetAsync(dst, value, bytes, stream()));
+    sync();
+  });
+}
+
+static void check_csr_args(int n, int m, int nnz) {
+  if (n < 0 || m < 0 || nnz < 0) fail(DECO_ERR_ARG, "negative CSR size n=%d m=%d nnz=%d", n, m, nnz);
+}
+
+int deco_csr_create(int n, int m, int nnz, const int* r, const int* c, const double* v, deco_csr** out) {
+  return guard([&] {
+    check_csr_args(n, m, nnz);
+    if (!out || !r || (nnz && (!c || !v))) fail(DECO_ERR_ARG, "null CSR array");
+    if (r[0] != 0 || r[n] != nnz) fail(DECO_ERR_ARG, "row pointer does not span [0, nnz]");
+    auto* A = csr_new(n, m);
+    A->nnz = nnz;
+    A->r.upload(r, n + 1);
+    A->c.upload(c, nnz);
+    A->v.upload(v, nnz);
+    sync();
+    *out = A;
+  });
+}
+
+int deco_csr_create_dev(int n, int m, int nnz, const int* r, const int* c, const double* v, deco_csr** out) {
+  return guard([&] {
+    check_csr_args(n, m, nnz);
+    if (!out) fail(DECO_ERR_ARG, "null output");
+    auto* A = csr_new(n, m);
+    A->nnz = nnz;
+    A->r.alloc(n + 1);
+    A->c.alloc(nnz);
+    A->v.alloc(nnz);
+    copy_d2d(A->r.p, r, n + 1);
+    copy_d2d(A->c.p, c, nnz);
+    copy_d2d(A->v.p, v, nnz);
+    sync();
+    *out = A;
+  });
+}
+
+int deco_csr_destroy(deco_csr* A) {
+  return guard([&] { delete A; });
+}
+
+int deco_csr_info(const deco_csr* A, int* n, int* m, int* nnz) {
+  return guard([&] {
+    if (!A) fail(DECO_ERR_ARG, "null CSR");
+    if (n) *n = A->n;
+    if (m) *m = A->m;
+    if (nnz) *nnz = A->nnz;
+  });
+}
+
+int deco_csr_download(const deco_csr* A, int* r, int* c, double* v) {
+  return guard([&] {
+    if (!A) fail(DECO_ERR_ARG, "null CSR");
+    if (r) A->r.download(r, A->n + 1);
+    if (c) A->c.download(c, A->nnz);
+    if (v) A->v.download(v, A->nnz);
+  });
+}
+
+int deco_csr_device_ptrs(const deco_csr* A, int** r, int** c, double** v) {
+  return guard([&] {
+    if (!A) fail(DECO_ERR_ARG, "null CSR");
+    if (r) *r = A->r.p;
+    if (c) *c = A->c.p;
+    if (v) *v = A->v.p;
+  });
+}
+
+int deco_csr_update_values(deco_csr* A, const double* v) {
+  return guard([&] {
+    if (!A || (A->nnz && !v)) fail(DECO_ERR_ARG, "null argument");
+    A->v.upload(v, A->nnz);
+    A->invalidate();
+    if (A->At) { delete A->At; A->At = nullptr; }
+    sync();
+  });
+}
+
+int deco_csr_copy(const deco_csr* A, deco_csr** out) {
+  return guard([&] {
+    if (!A || !out) fail(DECO_ERR_ARG, "null argument");
+    *out = csr_clone(A);
+    sync();
+  });
+}
+
+int deco_csr_filter(deco_csr* A, double tol) {
+  return guard([&] {
+    if (!A) fail(DECO_ERR_ARG, "null CSR");
+    csr_filter(A, tol);
+    sync();
+  });
+}
+
+int deco_csr_filter_proper(deco_csr* A, double tol) {
+  return guard([&] {
+    if (!A) fail(DECO_ERR_ARG, "null CSR");
+    csr_filter_proper(A, tol);
+    sync();
+  });
+}
+
+int deco_csr_local_matrix(const deco_csr* A, int bs, int p, const double* w, deco_csr** out) {
+  return guard([&] {
+    if (!A || !out) fail(DECO_ERR_ARG, "null argument");
+    *out = local_matrix(A, bs, p, w);
+    sync();
+  });
+}
+
+int deco_csr_transpose(const deco_csr* A, deco_csr** out) {
+  return guard([&] {
+    if (!A || !out) fail(DECO_ERR_ARG, "null argument");
+    *out = csr_transpose(A);
+    sync();
+  });
+}
+
+int deco_spmv(const deco_csr* A, const double* x, double* y, int bs) {
+  return guard([&] {
+    if (!A) fail(DECO_ERR_ARG, "null CSR");
+    if (bs < 1 || A->n % bs != 0) fail(DECO_ERR_ARG, "n=%d not divisible by bs=%d", A->n, bs);
+    launch_spmv<EPI_SET, double, double, double, double>(csr_group(A), A->n / bs, bs, A->r.p, A->c.p, A->v.p, x, y,
+                                                         nullptr, 0, 0);
+    sync();
+  });
+}
+
+int deco_spmv_residual(const deco_csr* A, const double* x, const double* z, double* y) {
+  return guard([&] {
+    if (!A) fail(DECO_ERR_ARG, "null CSR");
+    spmv_res(A, x, z, y);
+    sync();
+  });
+}
+
+int deco_spmv_transpose(deco_csr* A, const double* x, double* y) {
+  return guard([&] {
+    if (!A) fail(DECO_ERR_ARG, "null CSR");
+    if (!A->At) A->At = csr_transpose(A);
+    spmv(A->At, x, y);
+    sync();
+  });
+}
+
+int deco_spmv_f32(deco_csr* A, const float* x, float* y) {
+  return guard([&] {
+    if (!A) fail(DECO_ERR_ARG, "null CSR");
+    const float* vf = A->values_f32();
+    launch_spmv<EPI_SET, float, float, float, float>(csr_group(A), A->n, 1, A->r.p, A->c.p, vf, x, y, nullptr, 0, 0);
+    sync();
+  });
+}
+
+int deco_spmv_time(deco_csr* A, int precision, int iters, double* avg_ms) {
+  return guard([&] {
+    if (!A || !avg_ms || iters <= 0) fail(DECO_ERR_ARG, "bad argument");
+    hipEvent_t e0, e1;
+    DECO_HIP(hipEventCreate(&e0));
+    DECO_HIP(hipEventCreate(&e1));
+    const int G = csr_group(A);
+    if (precision == DECO_PREC_FP32) {
+      const float* vf = A->values_f32();
+      DBuf<float> x(A->m), y(A->n);
+      x.zero();
+      launch_spmv<EPI_SET, float, float, float, float>(G, A->n, 1, A->r.p, A->c.p, vf, x.p, y.p, nullptr, 0, 0);
+      DECO_HIP(hipEventRecord(e0, stream()));
+      for (int k = 0; k < iters; ++k)
+        launch_spmv<EPI_SET, float, float, float, float>(G, A->n, 1, A->r.p, A->c.p, vf, x.p, y.p, nullptr, 0, 0);
+      DECO_HIP(hipEventRecord(e1, stream()));
+    } else {
+      DBuf<double> x(A->m), y(A->n);
+      x.zero();
+      launch_spmv<EPI_SET, double, double, double, double>(G, A->n, 1, A->r.p, A->c.p, A->v.p, x.p, y.p, nullptr, 0,
+                                                           0);
+      DECO_HIP(hipEventRecord(e0, stream()));
+      for (int k = 0; k < iters; ++k)
+        launch_spmv<EPI_SET, double, double, double, double>(G, A->n, 1, A->r.p, A->c.p, A->v.p, x.p, y.p, nullptr,
+                                                             0, 0);
+      DECO_HIP(hipEventRecord(e1, stream()));
+    }
+    DECO_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    DECO_HIP(hipEventElapsedTime(&ms, e0, e1));
+    *avg_ms = (double)ms / iters;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  });
+}
+
+}  // extern "C"

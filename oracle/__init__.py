@@ -1,0 +1,370 @@
+"""CPU oracle (test infrastructure only): ctypes wrapper of oracle/liboracle.so.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this package, to
+check the HIP path or to time the CPU baseline. The product package deco_amd never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+from deco_amd._lib import CprParams, SolParams, SolInfo  # noqa: E402  (shared struct layouts)
+
+
+class OrcCSR(C.Structure):
+    _fields_ = [("n", C.c_int), ("m", C.c_int), ("nnz", C.c_int),
+                ("r", C.POINTER(C.c_int)), ("c", C.POINTER(C.c_int)), ("v", C.POINTER(C.c_double))]
+
+
+def build():
+    subprocess.run(["make", "-C", HERE], check=True, capture_output=True)
+    return LIB
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        vp, d, i = C.c_void_p, C.c_double, C.c_int
+        P = C.POINTER
+        sigs = {
+            "orc_last_error": (C.c_char_p, []),
+            "orc_csr_free": (None, [P(OrcCSR)]),
+            "orc_num_threads": (i, []),
+            "orc_set_threads": (None, [i]),
+            "orc_spmv": (i, [P(OrcCSR), vp, vp, i]),
+            "orc_dot": (d, [i, vp, vp]),
+            "orc_nrm2": (d, [i, vp]),
+            "orc_filter": (i, [P(OrcCSR), d]),
+            "orc_filter_proper": (i, [P(OrcCSR), d]),
+            "orc_local_matrix": (i, [P(OrcCSR), i, i, vp, P(OrcCSR)]),
+            "orc_trans_mat_full": (i, [P(OrcCSR), P(OrcCSR)]),
+            "orc_spgemm": (i, [P(OrcCSR), P(OrcCSR), P(OrcCSR)]),
+            "orc_fsai_pattern": (i, [P(OrcCSR), i, P(OrcCSR)]),
+            "orc_fix_diag": (i, [P(OrcCSR), vp, i, vp]),
+            "orc_quasi_impes_ge": (i, [P(OrcCSR), i, vp]),
+            "orc_quasi_impes": (i, [P(OrcCSR), i, vp]),
+            "orc_drs_weights": (i, [P(OrcCSR), i, d, d, vp]),
+            "orc_fsai_create": (i, [P(CprParams), P(OrcCSR), P(vp)]),
+            "orc_jacobi_create": (i, [i, P(OrcCSR), P(vp)]),
+            "orc_dense_create": (i, [P(OrcCSR), P(vp)]),
+            "orc_identity_create": (i, [i, P(vp)]),
+            "orc_amg_create": (i, [i, P(OrcCSR), P(OrcCSR), P(OrcCSR), P(vp), P(vp), i, i, vp, P(vp)]),
+            "orc_cpr_create": (i, [P(CprParams), P(OrcCSR), vp, P(vp)]),
+            "orc_cpr_pressure_matrix": (i, [vp, P(OrcCSR)]),
+            "orc_cpr_set_local": (i, [vp, vp]),
+            "orc_cpr_left_transform": (i, [vp, vp]),
+            "orc_fsai_factors": (i, [vp, P(OrcCSR), P(OrcCSR)]),
+            "orc_prec_apply": (i, [vp, vp, vp]),
+            "orc_prec_apply_transpose": (i, [vp, vp, vp]),
+            "orc_prec_destroy": (None, [vp]),
+            "orc_solve": (i, [P(SolParams), P(OrcCSR), vp, vp, vp, P(SolInfo), vp]),
+            "orc_solve_transpose": (i, [P(SolParams), P(OrcCSR), vp, vp, vp, P(SolInfo)]),
+            "orc_lu_factor": (i, [i, vp, vp]),
+            "orc_lu_solve": (i, [i, vp, vp, i, vp]),
+        }
+        for name, (res, args) in sigs.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(code):
+    if code != 0:
+        raise RuntimeError(f"oracle error {code}: {lib().orc_last_error().decode()}")
+
+
+def _vp(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Mat:
+    """Host CSR owned by numpy arrays, passed to the oracle as an orc_csr view."""
+
+    def __init__(self, n, m, r, c, v):
+        self.n, self.m = int(n), int(m)
+        self.r = np.ascontiguousarray(r, np.int32)
+        self.c = np.ascontiguousarray(c, np.int32)
+        self.v = np.ascontiguousarray(v, np.float64)
+
+    @classmethod
+    def from_scipy(cls, S):
+        S = S.tocsr()
+        S.sort_indices()
+        return cls(S.shape[0], S.shape[1], S.indptr, S.indices, S.data)
+
+    def view(self):
+        s = OrcCSR()
+        s.n, s.m, s.nnz = self.n, self.m, int(self.r[self.n])
+        s.r = self.r.ctypes.data_as(C.POINTER(C.c_int))
+        s.c = self.c.ctypes.data_as(C.POINTER(C.c_int))
+        s.v = self.v.ctypes.data_as(C.POINTER(C.c_double))
+        return s
+
+    @classmethod
+    def take(cls, s):
+        """Copy an oracle-allocated orc_csr into numpy and free it."""
+        n, nnz = s.n, s.nnz
+        r = np.ctypeslib.as_array(s.r, shape=(n + 1,)).copy()
+        c = np.ctypeslib.as_array(s.c, shape=(max(nnz, 1),))[:nnz].copy()
+        v = np.ctypeslib.as_array(s.v, shape=(max(nnz, 1),))[:nnz].copy()
+        m = s.m
+        lib().orc_csr_free(C.byref(s))
+        return cls(n, m, r, c, v)
+
+    @property
+    def nnz(self):
+        return int(self.r[self.n])
+
+    def sync_from(self, s):
+        """After an in-place oracle call that may shrink nnz."""
+        nnz = s.nnz
+        self.c = self.c[:nnz].copy()
+        self.v = self.v[:nnz].copy()
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        return sp.csr_matrix((self.v[:self.nnz], self.c[:self.nnz], self.r), shape=(self.n, self.m))
+
+    def copy(self):
+        return Mat(self.n, self.m, self.r.copy(), self.c.copy(), self.v.copy())
+
+
+def spmv(A, x, bs=1):
+    y = np.empty(A.n // bs)
+    x = np.ascontiguousarray(x, np.float64)
+    _check(lib().orc_spmv(C.byref(A.view()), _vp(x), _vp(y), bs))
+    return y
+
+
+def dot(x, y):
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    return lib().orc_dot(x.size, _vp(x), _vp(y))
+
+
+def nrm2(x):
+    x = np.ascontiguousarray(x, np.float64)
+    return lib().orc_nrm2(x.size, _vp(x))
+
+
+def _inplace(fn, A, *args):
+    s = A.view()
+    _check(fn(C.byref(s), *args))
+    A.sync_from(s)
+    return A
+
+
+def filter(A, tol):
+    return _inplace(lib().orc_filter, A, tol)
+
+
+def filter_proper(A, tol):
+    return _inplace(lib().orc_filter_proper, A, tol)
+
+
+def local_matrix(A, bs, p=0, w=None):
+    out = OrcCSR()
+    wp = _vp(np.ascontiguousarray(w, np.float64)) if w is not None else None
+    _check(lib().orc_local_matrix(C.byref(A.view()), bs, p, wp, C.byref(out)))
+    return Mat.take(out)
+
+
+def transpose(A):
+    out = OrcCSR()
+    _check(lib().orc_trans_mat_full(C.byref(A.view()), C.byref(out)))
+    return Mat.take(out)
+
+
+def spgemm(A, B):
+    out = OrcCSR()
+    _check(lib().orc_spgemm(C.byref(A.view()), C.byref(B.view()), C.byref(out)))
+    return Mat.take(out)
+
+
+def fsai_pattern(A, power):
+    out = OrcCSR()
+    _check(lib().orc_fsai_pattern(C.byref(A.view()), power, C.byref(out)))
+    return Mat.take(out)
+
+
+def quasi_impes_ge(A, bs):
+    w = np.empty(A.n)
+    _check(lib().orc_quasi_impes_ge(C.byref(A.view()), bs, _vp(w)))
+    return w
+
+
+def quasi_impes(A, bs):
+    w = np.empty(A.n)
+    _check(lib().orc_quasi_impes(C.byref(A.view()), bs, _vp(w)))
+    return w
+
+
+def drs_weights(A, bs, e_dd, e_ps):
+    w = np.empty(A.n, np.uint8)
+    _check(lib().orc_drs_weights(C.byref(A.view()), bs, e_dd, e_ps, _vp(w)))
+    return w.astype(bool)
+
+
+def fix_diag(A, b, bs):
+    flags = np.zeros(A.n, np.uint8)
+    bp = _vp(b) if b is not None else None
+    _inplace(lib().orc_fix_diag, A, bp, bs, _vp(flags))
+    return flags.astype(bool)
+
+
+def lu_factor(M):
+    """M: k x k array; returns (column-major LU, ipiv, info)."""
+    k = M.shape[0]
+    a = np.asfortranarray(M, np.float64).ravel(order="F").copy()
+    ipiv = np.zeros(k, np.int32)
+    info = lib().orc_lu_factor(k, _vp(a), _vp(ipiv))
+    return a, ipiv, info
+
+
+def lu_solve(lu, ipiv, b, trans=0):
+    b = np.ascontiguousarray(b, np.float64).copy()
+    lib().orc_lu_solve(ipiv.size, _vp(lu), _vp(ipiv), trans, _vp(b))
+    return b
+
+
+class Prec:
+    def __init__(self, h, n, keep=()):
+        self.h = h
+        self.n = n
+        self._keep = keep
+
+    def apply(self, r):
+        r = np.ascontiguousarray(r, np.float64)
+        x = np.empty(self.n)
+        _check(lib().orc_prec_apply(self.h, _vp(r), _vp(x)))
+        return x
+
+    def apply_t(self, r):
+        r = np.ascontiguousarray(r, np.float64)
+        x = np.empty(self.n)
+        _check(lib().orc_prec_apply_transpose(self.h, _vp(r), _vp(x)))
+        return x
+
+    def release(self):
+        h, self.h = self.h, None
+        return h
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().orc_prec_destroy(self.h)
+        except Exception:
+            pass
+
+
+def fsai(params, A):
+    h = C.c_void_p()
+    _check(lib().orc_fsai_create(C.byref(params), C.byref(A.view()), C.byref(h)))
+    return Prec(h.value, A.n)
+
+
+def fsai_factors(M):
+    FL, FUt = OrcCSR(), OrcCSR()
+    _check(lib().orc_fsai_factors(M.h, C.byref(FL), C.byref(FUt)))
+    return Mat.take(FL), Mat.take(FUt)
+
+
+def jacobi(bs, A):
+    h = C.c_void_p()
+    _check(lib().orc_jacobi_create(bs, C.byref(A.view()), C.byref(h)))
+    return Prec(h.value, A.n)
+
+
+def dense(A):
+    h = C.c_void_p()
+    _check(lib().orc_dense_create(C.byref(A.view()), C.byref(h)))
+    return Prec(h.value, A.n)
+
+
+def identity(n):
+    h = C.c_void_p()
+    _check(lib().orc_identity_create(n, C.byref(h)))
+    return Prec(h.value, n)
+
+
+def amg(levels, relax_type, coarse_inv):
+    """levels: list of dicts with A, P, R (Mat), d (float64), cf (int32); coarse_inv (nc x nc)."""
+    L = len(levels)
+    As = (OrcCSR * max(L, 1))(*[lv["A"].view() for lv in levels])
+    Ps = (OrcCSR * max(L, 1))(*[lv["P"].view() for lv in levels])
+    Rs = (OrcCSR * max(L, 1))(*[lv["R"].view() for lv in levels])
+    ds = [np.ascontiguousarray(lv["d"], np.float64) for lv in levels]
+    cfs = [np.ascontiguousarray(lv["cf"], np.int32) for lv in levels]
+    dp = (C.c_void_p * max(L, 1))(*[_vp(a) for a in ds])
+    cp = (C.c_void_p * max(L, 1))(*[_vp(a) for a in cfs])
+    ci = np.ascontiguousarray(coarse_inv, np.float64)
+    nc = ci.shape[0]
+    h = C.c_void_p()
+    _check(lib().orc_amg_create(L, As, Ps, Rs, dp, cp, relax_type, nc, _vp(ci), C.byref(h)))
+    n = levels[0]["A"].n if L else nc
+    return Prec(h.value, n)
+
+
+class CPR(Prec):
+    """CPR setup on the CPU; decouples A (a Mat) and b in place (precond/CPR.hpp:42-172)."""
+
+    def __init__(self, params, A, b):
+        h = C.c_void_p()
+        s = A.view()
+        bp = _vp(b) if b is not None else None
+        _check(lib().orc_cpr_create(C.byref(params), C.byref(s), bp, C.byref(h)))
+        A.sync_from(s)
+        super().__init__(h.value, A.n, keep=(A,))
+        self.A = A
+
+    def pressure_matrix(self):
+        out = OrcCSR()
+        _check(lib().orc_cpr_pressure_matrix(self.h, C.byref(out)))
+        return Mat.take(out)
+
+    def set_local(self, prec):
+        _check(lib().orc_cpr_set_local(self.h, prec.release()))
+
+    def left_transform(self, bs):
+        W = np.empty(self.n * bs)
+        _check(lib().orc_cpr_left_transform(self.h, _vp(W)))
+        return W
+
+
+def solve(sp, A, M, b, resvec=False):
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.zeros(A.n)
+    info = SolInfo()
+    rv = np.zeros(sp.itmax + 2) if resvec else None
+    _check(lib().orc_solve(C.byref(sp), C.byref(A.view()), M.h if M is not None else None, _vp(b), _vp(x),
+                           C.byref(info), _vp(rv) if rv is not None else None))
+    return (x, info, rv) if resvec else (x, info)
+
+
+def solve_transpose(sp, A, M, g):
+    g = np.ascontiguousarray(g, np.float64)
+    y = np.zeros(A.n)
+    info = SolInfo()
+    _check(lib().orc_solve_transpose(C.byref(sp), C.byref(A.view()), M.h if M is not None else None, _vp(g),
+                                     _vp(y), C.byref(info)))
+    return y, info
+
+
+def set_threads(t):
+    lib().orc_set_threads(int(t))
+
+
+def num_threads():
+    return lib().orc_num_threads()

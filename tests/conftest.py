@@ -1,0 +1,57 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP path)")
+
+
+_gen_cache = {}
+
+
+def gen_system(cfg, seed=0, custom=None):
+    """Generated system (host arrays), cached per session."""
+    import deco_amd
+    key = (cfg, seed, custom)
+    if key not in _gen_cache:
+        _gen_cache[key] = deco_amd.generate(cfg, seed, custom=custom)
+    return _gen_cache[key]
+
+
+def to_mat(g):
+    import oracle
+    return oracle.Mat(g["n"], g["n"], g["r"], g["c"], g["v"])
+
+
+@pytest.fixture(scope="session")
+def gpu():
+    import deco_amd
+    deco_amd.init(0)
+    return True
+
+
+def small_params(bs, **kw):
+    """Reference-style "prec" tree (param_norne.json-like) for tests."""
+    import deco_amd
+    prec = {
+        "block_size": str(bs), "pressure_id": "0", "seq": "3", "weights": "quasiimpes_GE",
+        "use_drs": "1", "drs": {"e_dd": "0.2", "e_ps": "0.0"}, "CPD": "1", "tol_zero": "0.0",
+        "prec_glo": {"type": "FSAI", "tol_rel": "0.2", "tol_abs": "1e-16", "power": "1", "block_size": "1"},
+        "prec_loc": {"tol_rel": "1e-4", "amg_coarse_type": "10", "amg_relax_type": "18", "amg_agg_levs": "0",
+                     "amg_agg_path": "1", "amg_A_drop_tol": "1e-4", "amg_prol_trunc_fac": "1e-3",
+                     "amg_SoC_alpha": "0.5"},
+    }
+    for k, v in kw.items():
+        node = prec
+        keys = k.split(".")
+        for kk in keys[:-1]:
+            node = node[kk]
+        node[keys[-1]] = str(v)
+    return deco_amd.PTree(prec)

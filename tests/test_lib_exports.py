@@ -1,0 +1,44 @@
+"""The C-ABI library loads on a machine without a GPU and exports every declared symbol."""
+import ctypes
+
+from deco_amd import _lib
+
+
+def test_library_exports_every_header_symbol():
+    lib = _lib.load()
+    names = _lib.header_functions()
+    assert len(names) > 50
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # every declared function has a ctypes signature in the binding
+    assert sorted(_lib.SIGNATURES) == names
+
+
+def test_error_string_is_available_without_gpu():
+    lib = _lib.load()
+    assert isinstance(lib.deco_last_error(), (bytes, type(None)))
+
+
+def test_struct_layouts_match_c(tmp_path):
+    """ctypes mirrors of the parameter/result structs have the C compiler's layout."""
+    import os
+    import subprocess
+    src = tmp_path / "layout.c"
+    structs = {"deco_cpr_params": _lib.CprParams, "deco_sol_params": _lib.SolParams,
+               "deco_sol_info": _lib.SolInfo}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "deco_hip.h"', "int main(void){"]
+    for cname, py in structs.items():
+        lines.append(f'printf("%zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("%zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0;}")
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    inc = os.path.join(os.path.dirname(_lib.HEADER))
+    subprocess.run(["gcc", "-I", inc, str(src), "-o", str(exe)], check=True)
+    out = [int(t) for t in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    want = []
+    for _, py in structs.items():
+        want.append(ctypes.sizeof(py))
+        want += [getattr(py, f).offset for f, _ in py._fields_]
+    assert out == want
