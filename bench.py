@@ -33,6 +33,48 @@ def spmv_bytes(n_rows, n_cols, nnz, sv, sx, sy):
     return nnz * (sv + 4) + 4 * (n_rows + 1) + sx * n_cols + sy * n_rows
 
 
+# CPR parameters of the bench: the reference's param_norne.json "prec" set with block size 2 and
+# the l1-Jacobi smoother (hypre relax type 18).
+PREC_TREE = {
+    "block_size": "2", "pressure_id": "0", "seq": "3", "weights": "quasiimpes_GE", "use_drs": "1",
+    "drs": {"e_dd": "0.2", "e_ps": "0.0"}, "CPD": "1", "tol_zero": "0.0",
+    "prec_glo": {"type": "FSAI", "tol_rel": "0.2", "tol_abs": "1e-16", "power": "1", "block_size": "1"},
+    "prec_loc": {"tol_rel": "1e-4", "amg_coarse_type": "10", "amg_relax_type": "18", "amg_agg_levs": "0",
+                 "amg_agg_path": "1", "amg_A_drop_tol": "1e-4", "amg_prol_trunc_fac": "1e-3",
+                 "amg_SoC_alpha": "0.5"},
+}
+
+
+def cpu_solve_baseline(g, M, prec_tree, sol, gpu_iters, args, log):
+    """The oracle's OpenMP CPR-GMRES on the host cores, on the same system and the same AMG
+    hierarchy, timed over a bounded number of iterations and scaled to the GPU's iteration
+    count."""
+    import deco_amd
+    import oracle
+    n = g["n"]
+    Ao = oracle.Mat(n, n, g["r"], g["c"], g["v"])
+    bo = g["b"].copy()
+    t0 = time.perf_counter()
+    Mo = oracle.CPR(deco_amd.cpr_params(deco_amd.PTree(prec_tree)), Ao, bo)
+    levels, cinv = M.amg().export()
+    lv = [dict(A=oracle.Mat(l["n"], l["n"], *l["A"]), P=oracle.Mat(l["n"], l["nc"], *l["P"]),
+               R=oracle.Mat(l["nc"], l["n"], *l["R"]), d=l["d"], cf=l["cf"]) for l in levels]
+    Mo.set_local(oracle.amg(lv, 18, cinv))
+    setup = time.perf_counter() - t0
+    k = args.cpu_iters
+    sp = deco_amd.sol_params(dict(sol, itmax=str(k)))
+    t1 = time.perf_counter()
+    _, info = oracle.solve(sp, Ao, Mo, bo)
+    dt = time.perf_counter() - t1
+    per_it = dt / max(info.iter, 1)
+    threads = oracle.num_threads()
+    log(f"CPU oracle: {info.iter} iterations in {dt:.2f}s ({threads} threads), setup+import {setup:.1f}s")
+    return {"value": per_it * gpu_iters, "unit": "s", "cores": threads, "kind": "port",
+            "sample": f"oracle CPR-rgmres(50) on the same decoupled system and AMG hierarchy, {info.iter} "
+                      f"iterations timed ({dt:.2f}s, OpenMP {threads} threads), scaled to the {gpu_iters} "
+                      f"iterations of the GPU FP64 solve; per-iteration {per_it * 1e3:.1f} ms"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -42,6 +84,9 @@ def main():
     ap.add_argument("--spmv-iters", type=int, default=50)
     ap.add_argument("--cpu-sample", type=int, default=10, help="CPU SpMV repetitions in the baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu-solve", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=5, help="GMRES iterations in the CPU baseline sample")
+    ap.add_argument("--export-levels", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (every GPU box here has one GPU); "
@@ -76,21 +121,53 @@ def main():
                  "bytes_fp64": b64, "bytes_fp32": b32},
         "roofline": {"bound": "hbm", "achieved": gbs64, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": gbs64 / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_spmv<16,EPI_SET,double,double,double> (FP64 CSR SpMV with A)",
+                     "kernel": "k_spmv_stream<double,double,EpiStd<EPI_SET>> (FP64 CSR SpMV with A)",
                      "bytes_per_launch": b64},
     }
 
-    solve_done = False
-    try:
-        from deco_amd import solver as _solver  # noqa: F401
-        have_solver = True
-    except ImportError:
-        have_solver = False
-    if have_solver:
-        solve_done = _solver.bench_solve(g, A, args, result, log)
-    if not solve_done:
-        result.update({"value": gbs64, "unit": "GB/s", "higher_is_better": True, "dtype": "f64",
-                       "ms_per_step": ms64})
+    # ---------------------------------------------------------------- CPR-GMRES solves
+    prec_tree = deco_amd.PTree(PREC_TREE)
+    sol = {"type": "rgmres", "reltol": "1e-8", "restart": "50", "itmax": "500", "verbosity": "0"}
+    b_dev = deco_amd.DeviceArray.from_numpy(g["b"])
+    deco_amd.lib().deco_sync()
+    t0 = time.perf_counter()
+    M = deco_amd.CPR(prec_tree, A, b_dev)            # decouples A and b in place (CPR.hpp:42-172)
+    deco_amd.lib().deco_sync()
+    setup_s = time.perf_counter() - t0
+    levels, _ = M.amg().export() if args.export_levels else ([], None)
+    log(f"CPR setup {setup_s:.2f}s")
+    x_ref = deco_amd.SOL(dict(sol, reltol="1e-12", itmax="2000"), A, M, b_dev)   # FP64 oracle solution
+    xr = x_ref.x.numpy()
+    log(f"FP64 reference solve: iter {x_ref.iter} res {x_ref.res_rel:.2e}")
+
+    def run(precision):
+        M.set_precision(precision)
+        for _ in range(args.warmup):
+            deco_amd.SOL(sol, A, M, b_dev)
+        deco_amd.lib().deco_sync()
+        times, last = [], None
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            last = deco_amd.SOL(sol, A, M, b_dev)
+            times.append(last.info.time_ms)
+        deco_amd.lib().deco_sync()
+        wall = (time.perf_counter() - t1) / args.steps
+        x = last.x.numpy()
+        return dict(ms=float(np.median(times)), wall_ms=wall * 1e3, iter=last.iter, res_rel=last.res_rel,
+                    flag=last.flag, err_vs_fp64_oracle=float(np.linalg.norm(x - xr) / np.linalg.norm(xr)),
+                    err_vs_xstar=float(np.linalg.norm(x - g["xstar"]) / np.linalg.norm(g["xstar"])))
+
+    r64 = run(deco_amd.PREC_FP64)
+    log(f"FP64-only: {r64}")
+    r32 = run(deco_amd.PREC_FP32)
+    log(f"FP32-precond: {r32}")
+    M.set_precision(deco_amd.PREC_FP64)
+    result.update({
+        "value": r32["wall_ms"] / 1e3, "unit": "s", "higher_is_better": False, "dtype": "f64",
+        "ms_per_step": r32["wall_ms"],
+        "solve": {"fp32_precond": r32, "fp64_only": r64, "speedup_fp32_vs_fp64": r64["wall_ms"] / r32["wall_ms"],
+                  "setup_s": setup_s, "reference_solve_iter": x_ref.iter},
+    })
 
     # ---------------------------------------------------------------- CPU baseline
     if not args.no_cpu:
@@ -108,8 +185,8 @@ def main():
             "sample": f"{args.cpu_sample} FP64 CSR SpMVs with A of the same system (oracle/oracle.cpp "
                       f"alg::spmv restatement, OpenMP {threads} threads); {dt * 1e3:.1f} ms each",
         }
-        if solve_done and "cpu_solve" in result:
-            result["cpu_baseline"] = result.pop("cpu_solve")
+        if not args.no_cpu_solve:
+            result["cpu_baseline"] = cpu_solve_baseline(g, M, PREC_TREE, sol, r64["iter"], args, log)
     print(json.dumps(result), flush=True)
     return 0
 

@@ -117,6 +117,8 @@ SIGNATURES = {
     "deco_amg_num_levels": (i, [vp, ip]),
     "deco_amg_level": (i, [vp, i, pp, pp, pp, pp, pp]),
     "deco_amg_coarse_inverse": (i, [vp, ip, pp]),
+    "deco_amg_interp_host": (i, [C.POINTER(CprParams), i, vp, vp, vp, vp, ip, ip, pp, pp, pp]),
+    "deco_host_free": (None, [vp]),
     "deco_solve": (i, [C.POINTER(SolParams), vp, vp, vp, vp, C.POINTER(SolInfo)]),
     "deco_solve_transpose": (i, [C.POINTER(SolParams), vp, vp, vp, vp, C.POINTER(SolInfo)]),
     "deco_grad_A_pattern": (i, [vp, vp, vp, vp]),

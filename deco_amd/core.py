@@ -320,3 +320,232 @@ def sol_params(sol):
     s.verbosity = sol.get("verbosity", int, 0)
     s.use_x0 = sol.get("use_x0", int, 0)
     return s
+
+
+# ---------------------------------------------------------------------------- preconditioners
+def _view(h, parent):
+    v = CSR(handle=h, owner=False)
+    v._parent = parent
+    return v
+
+
+class Prec:
+    """A device preconditioner handle: apply(r) = M^{-1} r (CPRBase::apply, precond/CPR.hpp:3-8)."""
+
+    def __init__(self, h, owner=True, parent=None):
+        self.h = h
+        self.owner = owner
+        self._parent = parent
+
+    @property
+    def n(self):
+        v = C.c_int()
+        check(lib().deco_prec_size(self.h, C.byref(v)))
+        return v.value
+
+    def apply(self, r, out=None):
+        rd = as_dev(r)
+        x = out if out is not None else DeviceArray(self.n)
+        check(lib().deco_prec_apply(self.h, rd.ptr, x.ptr))
+        return x
+
+    def apply_t(self, r, out=None):
+        rd = as_dev(r)
+        x = out if out is not None else DeviceArray(self.n)
+        check(lib().deco_prec_apply_transpose(self.h, rd.ptr, x.ptr))
+        return x
+
+    def set_precision(self, precision):
+        check(lib().deco_prec_set_precision(self.h, precision))
+
+    def __del__(self):
+        try:
+            if self.owner and getattr(self, "h", None):
+                lib().deco_prec_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class Identity(Prec):
+    def __init__(self, n):
+        h = C.c_void_p()
+        check(lib().deco_identity_create(int(n), C.byref(h)))
+        super().__init__(h.value)
+
+
+class Jacobi(Prec):
+    """Jacobi(param, A) (precond/Jacobi/Jacobi.hpp:15-25); param is a prec_glo-like subtree."""
+
+    def __init__(self, param, A):
+        if isinstance(param, dict):
+            param = PTree(param)
+        p = CprParams()
+        p.glo_block_size = param.get("block_size", int, 1)
+        h = C.c_void_p()
+        check(lib().deco_jacobi_create(C.byref(p), A.h, C.byref(h)))
+        super().__init__(h.value, parent=A)
+
+
+class FSAI(Prec):
+    """FSAI(param, A) (precond/FSAI/FSAI.hpp:14-37); param is the whole "prec" tree or the
+    "prec_glo" subtree."""
+
+    def __init__(self, param, A, handle=None, parent=None):
+        if handle is not None:
+            super().__init__(handle, owner=False, parent=parent)
+            return
+        if isinstance(param, dict):
+            param = PTree(param)
+        tree = param if param.has("prec_glo") else PTree({"block_size": "1", "prec_glo": param.to_dict()})
+        p = cpr_params(tree)
+        h = C.c_void_p()
+        check(lib().deco_fsai_create(C.byref(p), A.h, C.byref(h)))
+        super().__init__(h.value, parent=A)
+
+    def factors(self):
+        fl, fu = C.c_void_p(), C.c_void_p()
+        check(lib().deco_fsai_factors(self.h, C.byref(fl), C.byref(fu)))
+        return _view(fl.value, self), _view(fu.value, self)
+
+
+class AMG(Prec):
+    """BoomerAMG-like pressure preconditioner in the role of hypre_AMG(param, Ap)
+    (precond/hypre_AMG/hypre_AMG.hpp:16-21)."""
+
+    def __init__(self, param, Ap, handle=None, parent=None):
+        if handle is not None:
+            super().__init__(handle, owner=False, parent=parent)
+            return
+        if isinstance(param, dict):
+            param = PTree(param)
+        tree = param if param.has("prec_loc") else PTree({"block_size": "1", "prec_loc": param.to_dict()})
+        p = cpr_params(tree)
+        h = C.c_void_p()
+        check(lib().deco_amg_create(C.byref(p), Ap.h, C.byref(h)))
+        super().__init__(h.value, parent=Ap)
+
+    def export(self):
+        """Host copy of the hierarchy: (levels, coarse_inverse) for the CPU V-cycle."""
+        L = lib()
+        nl = C.c_int()
+        check(L.deco_amg_num_levels(self.h, C.byref(nl)))
+        levels = []
+        for lvl in range(nl.value):
+            a, p, r, d, cf = (C.c_void_p() for _ in range(5))
+            check(L.deco_amg_level(self.h, lvl, C.byref(a), C.byref(p), C.byref(r), C.byref(d), C.byref(cf)))
+            A = _view(a.value, self)
+            n = A.n
+            dd = np.empty(n)
+            cc = np.empty(n, np.int32)
+            check(L.deco_memcpy_d2h(_vp(dd), d.value, 8 * n))
+            check(L.deco_memcpy_d2h(_vp(cc), cf.value, 4 * n))
+            levels.append(dict(A=A.download(), P=_view(p.value, self).download(), R=_view(r.value, self).download(),
+                               n=n, nc=_view(p.value, self).m, d=dd, cf=cc))
+        nc = C.c_int()
+        inv = C.c_void_p()
+        check(L.deco_amg_coarse_inverse(self.h, C.byref(nc), C.byref(inv)))
+        ci = np.empty(nc.value * nc.value)
+        if nc.value:
+            check(L.deco_memcpy_d2h(_vp(ci), inv.value, 8 * ci.size))
+        return levels, ci.reshape(nc.value, nc.value)
+
+
+class CPR(Prec):
+    """CPR<int,int,double,FSAI|Jacobi,AMG>(param, A, b) (precond/CPR.hpp:11-172).
+
+    As in the reference the setup decouples A and b in place (b may be None)."""
+
+    def __init__(self, param, A, b=None, precision=_lib.PREC_FP64):
+        if isinstance(param, dict):
+            param = PTree(param)
+        p = cpr_params(param, precision)
+        h = C.c_void_p()
+        check(lib().deco_cpr_create(C.byref(p), A.h, b.ptr if b is not None else None, C.byref(h)))
+        super().__init__(h.value, parent=A)
+        self.bs = p.block_size
+        self.A = A
+
+    def pressure_matrix(self):
+        h = C.c_void_p()
+        check(lib().deco_cpr_pressure_matrix(self.h, C.byref(h)))
+        return _view(h.value, self)
+
+    def fsai(self):
+        h = C.c_void_p()
+        check(lib().deco_cpr_fsai(self.h, C.byref(h)))
+        return FSAI(None, None, handle=h.value, parent=self)
+
+    def amg(self):
+        h = C.c_void_p()
+        check(lib().deco_cpr_amg(self.h, C.byref(h)))
+        return AMG(None, None, handle=h.value, parent=self)
+
+    def left_transform(self):
+        W = np.empty(self.A.n * self.bs)
+        check(lib().deco_cpr_left_transform(self.h, _vp(W)))
+        return W
+
+    def transform_transpose(self, mu):
+        md = as_dev(mu)
+        lam = DeviceArray(md.n)
+        check(lib().deco_cpr_transform_transpose(self.h, md.ptr, lam.ptr))
+        return lam
+
+
+# ---------------------------------------------------------------------------- solver -----
+class SOL:
+    """SOL(param, A, precond, b, x) (solver/SOL.hpp:19-39): the constructor runs the solve on the
+    device. Results: iter, res_abs, res_rel, flag, x (DeviceArray), info (timings, counts)."""
+
+    def __init__(self, param, A, precond, b, x=None, transpose=False):
+        sp = sol_params(param)
+        bd = as_dev(b)
+        if x is not None:
+            xd = as_dev(x)
+            if not isinstance(x, DeviceArray):
+                sp.use_x0 = 1
+        else:
+            xd = DeviceArray(A.n)
+            sp.use_x0 = 0
+        info = SolInfo()
+        fn = lib().deco_solve_transpose if transpose else lib().deco_solve
+        check(fn(C.byref(sp), A.h, precond.h if precond is not None else None, bd.ptr, xd.ptr, C.byref(info)))
+        self.x = xd
+        self.info = info
+        self.iter, self.res_abs, self.res_rel, self.flag = info.iter, info.res_abs, info.res_rel, info.flag
+
+    def get_result(self):
+        return self.x
+
+
+def grad_A_pattern(A, lam, x):
+    """dL/dA_ij = -lambda_i x_j on A's pattern (device)."""
+    ld, xd = as_dev(lam), as_dev(x)
+    dA = DeviceArray(A.nz)
+    check(lib().deco_grad_A_pattern(A.h, ld.ptr, xd.ptr, dA.ptr))
+    return dA
+
+
+def amg_interp_host(param, n, r, c, v):
+    """One AMG setup level on the host (strength, C/F split, ext+i interpolation)."""
+    if isinstance(param, dict):
+        param = PTree(param)
+    tree = param if param.has("prec_loc") else PTree({"block_size": "1", "prec_loc": param.to_dict()})
+    p = cpr_params(tree)
+    r = np.ascontiguousarray(r, np.int32)
+    c = np.ascontiguousarray(c, np.int32)
+    v = np.ascontiguousarray(v, np.float64)
+    cf = np.empty(n, np.int32)
+    nc, pnnz = C.c_int(), C.c_int()
+    pr, pc, pv = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    L = lib()
+    check(L.deco_amg_interp_host(C.byref(p), n, _vp(r), _vp(c), _vp(v), _vp(cf), C.byref(nc), C.byref(pnnz),
+                                 C.byref(pr), C.byref(pc), C.byref(pv)))
+    k = pnnz.value
+    Pr = np.ctypeslib.as_array(C.cast(pr, C.POINTER(C.c_int)), shape=(n + 1,)).copy()
+    Pc = np.ctypeslib.as_array(C.cast(pc, C.POINTER(C.c_int)), shape=(max(k, 1),))[:k].copy()
+    Pv = np.ctypeslib.as_array(C.cast(pv, C.POINTER(C.c_double)), shape=(max(k, 1),))[:k].copy()
+    for q in (pr, pc, pv):
+        L.deco_host_free(q)
+    return cf, nc.value, (Pr, Pc, Pv)

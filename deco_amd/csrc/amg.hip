@@ -1,0 +1,429 @@
+// AMG preconditioner for the CPR pressure stage (the role of hypre_AMG,
+// precond/hypre_AMG/hypre_AMG.hpp:16-37, hypre_AMG.tpp:3-80).
+//
+// Setup, level by level: the host computes strength, C/F splitting and the extended+i
+// interpolation P (amg_setup.cpp); the device forms R = P^T, the Galerkin product
+// A_c = R (A P) with the device SpGEMM, drops entries below amg_A_drop_tol times the row
+// maximum, and computes the smoother diagonal. The coarsest matrix (<= amg_max_coarse rows) is
+// inverted densely in FP64 on the host and uploaded.
+//
+// Apply: one V-cycle from a zero initial guess (hypre_AMG.hpp:28-33), one sweep down and up.
+// Smoothers: relax 18 and -1 = l1-Jacobi (diagonal sum_j |a_ij|), 0/7 = Jacobi, 17 = F-C-F
+// Jacobi half-sweeps. Relaxation is fused into the stream SpMV epilogue. FP32 mode runs the
+// levels on f32 shadows and the coarsest solve in FP64.
+#include "prec.hpp"
+#include "amg_setup.hpp"
+#include <vector>
+#include <cstdlib>
+#include <cstring>
+
+namespace deco {
+
+deco_csr* spgemm(const deco_csr* A, const deco_csr* B);
+
+// keep a_ij if i == j or |a_ij| >= tol * max_k |a_ik|
+__global__ void k_drop_count(int n, const int* __restrict__ r, const int* __restrict__ c, const double* __restrict__ v,
+                             double tol, int* __restrict__ cnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double mx = 0.0;
+  for (int j = r[i]; j < r[i + 1]; ++j) mx = fmax(mx, fabs(v[j]));
+  const double thr = tol * mx;
+  int k = 0;
+  for (int j = r[i]; j < r[i + 1]; ++j)
+    if (c[j] == i || fabs(v[j]) >= thr) ++k;
+  cnt[i] = k;
+}
+__global__ void k_drop_fill(int n, const int* __restrict__ r, const int* __restrict__ c, const double* __restrict__ v,
+                            double tol, const int* __restrict__ rn, int* __restrict__ cn, double* __restrict__ vn) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double mx = 0.0;
+  for (int j = r[i]; j < r[i + 1]; ++j) mx = fmax(mx, fabs(v[j]));
+  const double thr = tol * mx;
+  int o = rn[i];
+  for (int j = r[i]; j < r[i + 1]; ++j)
+    if (c[j] == i || fabs(v[j]) >= thr) { cn[o] = c[j]; vn[o] = v[j]; ++o; }
+}
+
+static deco_csr* drop_small(const deco_csr* A, double tol) {
+  deco_csr* B = csr_new(A->n, A->m);
+  DBuf<int> cnt(A->n + 1);
+  if (A->n) {
+    hipLaunchKernelGGL(k_drop_count, dim3(grid_for(A->n, 256)), dim3(256), 0, stream(), A->n, A->r.p, A->c.p, A->v.p,
+                       tol, cnt.p);
+    DECO_CHECK_LAUNCH();
+  }
+  build_row_ptr(B, cnt.p, A->n);
+  if (A->n) {
+    hipLaunchKernelGGL(k_drop_fill, dim3(grid_for(A->n, 256)), dim3(256), 0, stream(), A->n, A->r.p, A->c.p, A->v.p,
+                       tol, (const int*)B->r.p, B->c.p, B->v.p);
+    DECO_CHECK_LAUNCH();
+  }
+  return B;
+}
+
+// smoother diagonal: l1 (sum |a_ij|) or the diagonal entry
+__global__ void k_smoother_diag(int n, const int* __restrict__ r, const int* __restrict__ c,
+                                const double* __restrict__ v, int l1, double* __restrict__ d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int j = r[i]; j < r[i + 1]; ++j) {
+    if (l1) s += fabs(v[j]);
+    else if (c[j] == i) s += v[j];
+  }
+  d[i] = s;
+}
+
+// x = b/d on the rows of the subset (others 0): first half-sweep from a zero guess
+template <class T>
+__global__ void k_relax_zero(int n, const T* __restrict__ b, const T* __restrict__ d, const int* __restrict__ cf,
+                             int subset, T* __restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const bool on = subset < 0 || (subset == 1 ? cf[i] == 1 : cf[i] != 1);
+  x[i] = on ? (T)0 + (b[i] - (T)0) / d[i] : (T)0;
+}
+
+// y = x + (b - A x)/d on the subset rows, y = x elsewhere
+template <class T>
+struct EpiRelax {
+  const T* __restrict__ x;
+  const T* __restrict__ b;
+  const T* __restrict__ d;
+  const int* __restrict__ cf;
+  int subset;
+  T* __restrict__ y;
+  template <class AT>
+  __device__ __forceinline__ void operator()(int o, AT acc) const {
+    const bool on = subset < 0 || (subset == 1 ? cf[o] == 1 : cf[o] != 1);
+    y[o] = on ? (T)(x[o] + (b[o] - (T)acc) / d[o]) : x[o];
+  }
+};
+
+// coarse solve x = Cinv b with CT[k*nc + i] = Cinv(i,k); input/output in T, FP64 arithmetic
+template <class T>
+__global__ void k_coarse(int nc, const double* __restrict__ CT, const T* __restrict__ b, T* __restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nc) return;
+  double s = 0.0;
+  for (int k = 0; k < nc; ++k) s += CT[(size_t)k * nc + i] * (double)b[k];
+  x[i] = (T)s;
+}
+
+struct AMG : deco_prec {
+  struct Level {
+    deco_csr* A = nullptr;
+    deco_csr* At = nullptr;   // transposed level matrix (transposed apply)
+    deco_csr* P = nullptr;
+    deco_csr* R = nullptr;
+    DBuf<double> d;
+    DBuf<float> df;
+    DBuf<int> cf;
+    DBuf<double> x, b, t, r;
+    DBuf<float> xf, bf, tf, rf;
+  };
+  std::vector<Level> L;       // levels 0..nl-1 with interpolation; L.back() may be the coarsest
+  deco_csr* Ac = nullptr;     // coarsest matrix
+  int nc = 0;
+  DBuf<double> CT, Cinv;      // coarse inverse: transposed (forward) and row-major (transposed)
+  DBuf<double> xc, bc;
+  DBuf<float> xcf, bcf;
+  int relax = 18;
+  bool have_t = false;
+
+  AMG(const deco_cpr_params& p, const deco_csr* A0) {
+    kind = deco_prec::AMG;
+    n = A0->n;
+    relax = p.amg_relax_type;
+    if (relax == -1) relax = 18;
+    if (relax != 0 && relax != 7 && relax != 17 && relax != 18)
+      fail(DECO_ERR_UNSUPPORTED, "amg_relax_type %d is not supported (use -1, 0, 7, 17 or 18)", p.amg_relax_type);
+    if (p.amg_agg_levs > 0) fail(DECO_ERR_UNSUPPORTED, "aggressive coarsening (amg_agg_levs > 0) is not supported");
+    if (A0->n != A0->m) fail(DECO_ERR_ARG, "AMG matrix is not square");
+    const int max_coarse = p.amg_max_coarse > 0 ? p.amg_max_coarse : 300;
+    const int max_levels = p.amg_max_levels > 0 ? p.amg_max_levels : 25;
+    const double theta = p.amg_SoC_alpha;
+    const double mrs = p.amg_max_row_sum > 0 ? p.amg_max_row_sum : 0.9;
+    deco_csr* Al = csr_clone(A0);
+    try {
+      for (int lev = 0;; ++lev) {
+        if (Al->n <= max_coarse || lev >= max_levels - 1) break;
+        amg::HostCSR H;
+        H.n = Al->n;
+        H.m = Al->m;
+        H.r.resize(Al->n + 1);
+        H.c.resize(Al->nnz);
+        H.v.resize(Al->nnz);
+        Al->r.download(H.r.data(), Al->n + 1);
+        Al->c.download(H.c.data(), Al->nnz);
+        Al->v.download(H.v.data(), Al->nnz);
+        std::vector<char> strong;
+        amg::strength(H, theta, mrs, strong);
+        std::vector<int> cf;
+        amg::coarsen(H, strong, p.amg_coarse_type, 0x5eed0000ull + (uint64_t)lev, cf);
+        int ncl = 0;
+        amg::HostCSR Ph = amg::interp_extpi(H, strong, cf, p.amg_prol_trunc_fac, p.amg_P_max_elmts, ncl);
+        if (ncl == 0 || ncl == Al->n) break;
+        Level lv;
+        lv.A = Al;
+        lv.cf.upload(cf.data(), cf.size());
+        lv.P = csr_new(Ph.n, Ph.m);
+        lv.P->nnz = (int)Ph.c.size();
+        lv.P->r.upload(Ph.r.data(), Ph.r.size());
+        lv.P->c.upload(Ph.c.data(), Ph.c.size());
+        lv.P->v.upload(Ph.v.data(), Ph.v.size());
+        lv.R = csr_transpose(lv.P);
+        deco_csr* AP = spgemm(Al, lv.P);
+        deco_csr* RAP = spgemm(lv.R, AP);
+        delete AP;
+        if (p.amg_A_drop_tol > 0.0) {
+          Al = drop_small(RAP, p.amg_A_drop_tol);
+          delete RAP;
+        } else {
+          Al = RAP;
+        }
+        L.push_back(std::move(lv));
+      }
+      Ac = Al;
+      Al = nullptr;
+    } catch (...) {
+      delete Al;
+      release();
+      throw;
+    }
+    // smoothers and work vectors
+    const int l1 = (relax == 18) ? 1 : 0;
+    for (auto& lv : L) {
+      const int nl = lv.A->n;
+      lv.d.alloc(nl);
+      hipLaunchKernelGGL(k_smoother_diag, dim3(grid_for(nl, 256)), dim3(256), 0, stream(), nl, lv.A->r.p, lv.A->c.p,
+                         lv.A->v.p, l1, lv.d.p);
+      DECO_CHECK_LAUNCH();
+      lv.df.alloc(nl);
+      convert(lv.d.p, lv.df.p, nl);
+      lv.x.alloc(nl); lv.b.alloc(nl); lv.t.alloc(nl); lv.r.alloc(nl);
+      lv.xf.alloc(nl); lv.bf.alloc(nl); lv.tf.alloc(nl); lv.rf.alloc(nl);
+    }
+    // coarsest: dense FP64 inverse
+    nc = Ac->n;
+    amg::HostCSR H;
+    H.n = nc;
+    H.m = nc;
+    H.r.resize(nc + 1);
+    H.c.resize(Ac->nnz);
+    H.v.resize(Ac->nnz);
+    Ac->r.download(H.r.data(), nc + 1);
+    Ac->c.download(H.c.data(), Ac->nnz);
+    Ac->v.download(H.v.data(), Ac->nnz);
+    std::vector<double> inv;
+    if (nc > 0 && !amg::dense_inverse(H, inv)) {
+      release();
+      fail(DECO_ERR_SINGULAR, "coarsest AMG matrix (%d rows) is singular", nc);
+    }
+    std::vector<double> invT((size_t)nc * nc);
+    for (int i = 0; i < nc; ++i)
+      for (int k = 0; k < nc; ++k) invT[(size_t)k * nc + i] = inv[(size_t)i * nc + k];
+    Cinv.upload(inv.data(), inv.size());
+    CT.upload(invT.data(), invT.size());
+    xc.alloc(nc); bc.alloc(nc); xcf.alloc(nc); bcf.alloc(nc);
+    sync();
+  }
+
+  void release() {
+    for (auto& lv : L) { delete lv.A; delete lv.At; delete lv.P; delete lv.R; lv.A = lv.At = lv.P = lv.R = nullptr; }
+    L.clear();
+    delete Ac;
+    Ac = nullptr;
+  }
+  ~AMG() override { release(); }
+
+  // level data accessors by precision
+  template <class T> static T* X(Level& l);
+  template <class T> static T* B(Level& l);
+  template <class T> static T* Tm(Level& l);
+  template <class T> static T* Rr(Level& l);
+  template <class T> static const T* D(Level& l);
+  template <class T> static const T* vals(deco_csr* A);
+  template <class T> T* XC();
+  template <class T> T* BC();
+
+  template <class T>
+  void relax_sweep(Level& lv, const deco_csr* A, const T* x, const T* b, T* y, int subset) {
+    spmv_stream_f(A, vals<T>(const_cast<deco_csr*>(A)), x, EpiRelax<T>{x, b, D<T>(lv), lv.cf.p, subset, y});
+  }
+
+  // one V-cycle on level l with right-hand side b, result in x (x != b)
+  template <class T>
+  void cycle(size_t l, const T* b, T* x, bool tr) {
+    if (l == L.size()) {
+      if (nc == 0) return;
+      hipLaunchKernelGGL(k_coarse<T>, dim3(grid_for(nc, 64)), dim3(64), 0, stream(), nc,
+                         (const double*)(tr ? Cinv.p : CT.p), b, x);
+      DECO_CHECK_LAUNCH();
+      return;
+    }
+    Level& lv = L[l];
+    const deco_csr* A = tr ? lv.At : lv.A;
+    const int nl = A->n;
+    T* t = Tm<T>(lv);
+    T* r = Rr<T>(lv);
+    // down: smoother from a zero guess
+    if (relax == 17) {
+      hipLaunchKernelGGL(k_relax_zero<T>, dim3(grid_for(nl, 256)), dim3(256), 0, stream(), nl, b, D<T>(lv), lv.cf.p, 0, x);
+      DECO_CHECK_LAUNCH();
+      relax_sweep<T>(lv, A, x, b, t, 1);
+      relax_sweep<T>(lv, A, t, b, x, 0);
+    } else {
+      hipLaunchKernelGGL(k_relax_zero<T>, dim3(grid_for(nl, 256)), dim3(256), 0, stream(), nl, b, D<T>(lv), lv.cf.p, -1, x);
+      DECO_CHECK_LAUNCH();
+    }
+    // residual and restriction
+    spmv_stream<EPI_RES, T, T, T, T>(A, vals<T>(const_cast<deco_csr*>(A)), x, r, b);
+    const bool last = (l + 1 == L.size());
+    T* bn = last ? BC<T>() : B<T>(L[l + 1]);
+    T* xn = last ? XC<T>() : X<T>(L[l + 1]);
+    spmv_stream<EPI_SET, T, T, T, T>(lv.R, vals<T>(lv.R), r, bn, (const T*)nullptr);
+    cycle<T>(l + 1, bn, xn, tr);
+    // prolongation t = x + P xn, then the up smoother back into x
+    spmv_stream<EPI_ADDZ, T, T, T, T>(lv.P, vals<T>(lv.P), xn, t, x);
+    if (relax == 17) {
+      relax_sweep<T>(lv, A, t, b, x, 0);
+      relax_sweep<T>(lv, A, x, b, t, 1);
+      relax_sweep<T>(lv, A, t, b, x, 0);
+    } else {
+      relax_sweep<T>(lv, A, t, b, x, -1);
+    }
+  }
+
+  void ensure_t() {
+    if (have_t) return;
+    for (auto& lv : L) lv.At = csr_transpose(lv.A);
+    have_t = true;
+  }
+
+  template <class T>
+  void run(const T* r, T* x, bool tr) {
+    if (tr) ensure_t();
+    if (L.empty()) {
+      hipLaunchKernelGGL(k_coarse<T>, dim3(grid_for(nc, 64)), dim3(64), 0, stream(), nc,
+                         (const double*)(tr ? Cinv.p : CT.p), r, x);
+      DECO_CHECK_LAUNCH();
+      return;
+    }
+    cycle<T>(0, r, x, tr);
+  }
+  void apply(const double* r, double* x) override { run<double>(r, x, false); }
+  void apply_t(const double* r, double* x) override { run<double>(r, x, true); }
+  void apply32(const float* r, float* x) override { run<float>(r, x, false); }
+  void apply32_t(const float* r, float* x) override { run<float>(r, x, true); }
+};
+
+template <> double* AMG::X<double>(Level& l) { return l.x.p; }
+template <> float* AMG::X<float>(Level& l) { return l.xf.p; }
+template <> double* AMG::B<double>(Level& l) { return l.b.p; }
+template <> float* AMG::B<float>(Level& l) { return l.bf.p; }
+template <> double* AMG::Tm<double>(Level& l) { return l.t.p; }
+template <> float* AMG::Tm<float>(Level& l) { return l.tf.p; }
+template <> double* AMG::Rr<double>(Level& l) { return l.r.p; }
+template <> float* AMG::Rr<float>(Level& l) { return l.rf.p; }
+template <> const double* AMG::D<double>(Level& l) { return l.d.p; }
+template <> const float* AMG::D<float>(Level& l) { return l.df.p; }
+template <> const double* AMG::vals<double>(deco_csr* A) { return A->v.p; }
+template <> const float* AMG::vals<float>(deco_csr* A) { return A->values_f32(); }
+template <> double* AMG::XC<double>() { return xc.p; }
+template <> float* AMG::XC<float>() { return xcf.p; }
+template <> double* AMG::BC<double>() { return bc.p; }
+template <> float* AMG::BC<float>() { return bcf.p; }
+
+deco_prec* make_amg(const deco_cpr_params& p, const deco_csr* Ap) { return new AMG(p, Ap); }
+
+static AMG* as_amg(const deco_prec* M) {
+  if (!M || M->kind != deco_prec::AMG) fail(DECO_ERR_ARG, "not an AMG preconditioner");
+  return static_cast<AMG*>(const_cast<deco_prec*>(M));
+}
+
+}  // namespace deco
+
+using namespace deco;
+
+extern "C" {
+
+int deco_amg_create(const deco_cpr_params* p, const deco_csr* Ap, deco_prec** out) {
+  return guard([&] {
+    if (!p || !Ap || !out) fail(DECO_ERR_ARG, "null argument");
+    *out = make_amg(*p, Ap);
+  });
+}
+
+int deco_amg_num_levels(const deco_prec* M, int* levels) {
+  return guard([&] {
+    if (!levels) fail(DECO_ERR_ARG, "null argument");
+    *levels = (int)as_amg(M)->L.size();
+  });
+}
+
+int deco_amg_level(const deco_prec* M, int level, const deco_csr** A, const deco_csr** P, const deco_csr** R,
+                   const double** d, const int** cf) {
+  return guard([&] {
+    AMG* G = as_amg(M);
+    if (level < 0 || level > (int)G->L.size()) fail(DECO_ERR_ARG, "level %d out of range", level);
+    if (level == (int)G->L.size()) {
+      if (A) *A = G->Ac;
+      if (P) *P = nullptr;
+      if (R) *R = nullptr;
+      if (d) *d = nullptr;
+      if (cf) *cf = nullptr;
+      return;
+    }
+    auto& lv = G->L[level];
+    if (A) *A = lv.A;
+    if (P) *P = lv.P;
+    if (R) *R = lv.R;
+    if (d) *d = lv.d.p;
+    if (cf) *cf = lv.cf.p;
+  });
+}
+
+int deco_amg_coarse_inverse(const deco_prec* M, int* nc, const double** inv) {
+  return guard([&] {
+    AMG* G = as_amg(M);
+    if (nc) *nc = G->nc;
+    if (inv) *inv = G->Cinv.p;
+  });
+}
+
+int deco_amg_interp_host(const deco_cpr_params* p, int n, const int* r, const int* c, const double* v, int* cf,
+                         int* nc, int* p_nnz, int** P_r, int** P_c, double** P_v) {
+  return guard([&] {
+    if (!p || n < 0 || !r || !cf || !nc || !p_nnz || !P_r || !P_c || !P_v) fail(DECO_ERR_ARG, "null argument");
+    amg::HostCSR H;
+    H.n = n;
+    H.m = n;
+    H.r.assign(r, r + n + 1);
+    H.c.assign(c, c + r[n]);
+    H.v.assign(v, v + r[n]);
+    std::vector<char> strong;
+    amg::strength(H, p->amg_SoC_alpha, p->amg_max_row_sum > 0 ? p->amg_max_row_sum : 0.9, strong);
+    std::vector<int> cfv;
+    amg::coarsen(H, strong, p->amg_coarse_type, 0x5eed0000ull, cfv);
+    int ncl = 0;
+    amg::HostCSR P = amg::interp_extpi(H, strong, cfv, p->amg_prol_trunc_fac, p->amg_P_max_elmts, ncl);
+    std::memcpy(cf, cfv.data(), sizeof(int) * n);
+    *nc = ncl;
+    *p_nnz = (int)P.c.size();
+    *P_r = (int*)malloc(sizeof(int) * (n + 1));
+    *P_c = (int*)malloc(sizeof(int) * std::max<size_t>(1, P.c.size()));
+    *P_v = (double*)malloc(sizeof(double) * std::max<size_t>(1, P.v.size()));
+    std::memcpy(*P_r, P.r.data(), sizeof(int) * (n + 1));
+    if (!P.c.empty()) {
+      std::memcpy(*P_c, P.c.data(), sizeof(int) * P.c.size());
+      std::memcpy(*P_v, P.v.data(), sizeof(double) * P.v.size());
+    }
+  });
+}
+
+void deco_host_free(void* ptr) { free(ptr); }
+
+}  // extern "C"

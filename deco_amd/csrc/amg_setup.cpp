@@ -1,0 +1,344 @@
+// Host part of the AMG setup (the hypre BoomerAMG substitute): strength of connection,
+// C/F splitting and extended+i interpolation with truncation. Deterministic for any thread
+// count: every row is computed independently and coarsening uses fixed tie-breaking or a seeded
+// hash for its random measure.
+#include "amg_setup.hpp"
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <omp.h>
+
+namespace deco {
+namespace amg {
+
+static inline uint64_t mix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+void strength(const HostCSR& A, double theta, double max_row_sum, std::vector<char>& strong) {
+  strong.assign(A.c.size(), 0);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < A.n; ++i) {
+    const int b = A.r[i], e = A.r[i + 1];
+    double diag = 0.0;
+    for (int k = b; k < e; ++k)
+      if (A.c[k] == i) { diag = A.v[k]; break; }
+    double row_scale = 0.0, row_sum = diag;
+    for (int k = b; k < e; ++k) {
+      if (A.c[k] == i) continue;
+      const double a = A.v[k];
+      if (diag < 0.0) row_scale = std::max(row_scale, a);
+      else row_scale = std::min(row_scale, a);
+      row_sum += a;
+    }
+    row_sum = std::fabs(row_sum / diag);
+    if (row_sum > max_row_sum && max_row_sum < 1.0) continue;  // all dependencies weak
+    const double thr = theta * row_scale;
+    for (int k = b; k < e; ++k) {
+      if (A.c[k] == i) continue;
+      const double a = A.v[k];
+      strong[k] = (diag < 0.0) ? (a > thr) : (a < thr);
+    }
+  }
+}
+
+// strong dependencies S (rows) and strong influences S^T as CSR adjacency
+static void adjacency(const HostCSR& A, const std::vector<char>& strong, std::vector<int>& sr, std::vector<int>& sc,
+                      std::vector<int>& tr, std::vector<int>& tc) {
+  const int n = A.n;
+  sr.assign(n + 1, 0);
+  tr.assign(n + 1, 0);
+  for (int i = 0; i < n; ++i)
+    for (int k = A.r[i]; k < A.r[i + 1]; ++k)
+      if (strong[k]) { sr[i + 1]++; tr[A.c[k] + 1]++; }
+  for (int i = 0; i < n; ++i) { sr[i + 1] += sr[i]; tr[i + 1] += tr[i]; }
+  sc.resize(sr[n]);
+  tc.resize(tr[n]);
+  std::vector<int> cur(tr.begin(), tr.end() - 1);
+  int o = 0;
+  for (int i = 0; i < n; ++i)
+    for (int k = A.r[i]; k < A.r[i + 1]; ++k)
+      if (strong[k]) {
+        sc[o++] = A.c[k];
+        tc[cur[A.c[k]]++] = i;
+      }
+}
+
+// Ruge-Stueben first pass with bucket lists (max measure first, LIFO inside a bucket).
+static void coarsen_rs(int n, const std::vector<int>& sr, const std::vector<int>& sc, const std::vector<int>& tr,
+                       const std::vector<int>& tc, std::vector<int>& cf) {
+  cf.assign(n, 0);
+  std::vector<int> lam(n);
+  int maxdeg = 0;
+  for (int i = 0; i < n; ++i) {
+    lam[i] = tr[i + 1] - tr[i];
+    maxdeg = std::max(maxdeg, lam[i]);
+  }
+  const int nb = 2 * maxdeg + 2;
+  std::vector<int> head(nb, -1), nxt(n, -1), prv(n, -1);
+  auto insert = [&](int i) {
+    const int b = lam[i];
+    nxt[i] = head[b];
+    prv[i] = -1;
+    if (head[b] >= 0) prv[head[b]] = i;
+    head[b] = i;
+  };
+  auto remove = [&](int i) {
+    const int b = lam[i];
+    if (prv[i] >= 0) nxt[prv[i]] = nxt[i]; else head[b] = nxt[i];
+    if (nxt[i] >= 0) prv[nxt[i]] = prv[i];
+    nxt[i] = prv[i] = -1;
+  };
+  for (int i = 0; i < n; ++i) {
+    if (lam[i] == 0) cf[i] = -1;  // influences nobody (or isolated): F
+    else insert(i);
+  }
+  int top = nb - 1;
+  while (true) {
+    while (top > 0 && head[top] < 0) --top;
+    if (top <= 0) break;
+    const int i = head[top];
+    remove(i);
+    cf[i] = 1;
+    for (int k = tr[i]; k < tr[i + 1]; ++k) {  // points depending on i become F
+      const int j = tc[k];
+      if (cf[j] != 0) continue;
+      remove(j);
+      cf[j] = -1;
+      for (int q = sr[j]; q < sr[j + 1]; ++q) {  // their influencers gain weight
+        const int m = sc[q];
+        if (cf[m] != 0) continue;
+        remove(m);
+        lam[m] = std::min(lam[m] + 1, nb - 1);
+        insert(m);
+        if (lam[m] > top) top = lam[m];
+      }
+    }
+    for (int q = sr[i]; q < sr[i + 1]; ++q) {  // points i depends on lose weight
+      const int m = sc[q];
+      if (cf[m] != 0) continue;
+      remove(m);
+      lam[m] = std::max(lam[m] - 1, 0);
+      insert(m);
+    }
+  }
+  for (int i = 0; i < n; ++i)
+    if (cf[i] == 0) cf[i] = -1;
+}
+
+// PMIS: measure |S^T_i| + U[0,1) from a seeded hash; independent sets of local maxima.
+static void coarsen_pmis(int n, const std::vector<int>& sr, const std::vector<int>& sc, const std::vector<int>& tr,
+                         const std::vector<int>& tc, uint64_t seed, std::vector<int>& cf) {
+  cf.assign(n, 0);
+  std::vector<double> meas(n);
+  for (int i = 0; i < n; ++i) {
+    const double u = (double)(mix64(seed ^ mix64((uint64_t)i)) >> 11) * (1.0 / 9007199254740992.0);
+    meas[i] = (double)(tr[i + 1] - tr[i]) + u;
+    if (meas[i] < 1.0) cf[i] = -1;
+  }
+  std::vector<char> newc(n, 0);
+  while (true) {
+    int undecided = 0;
+#pragma omp parallel for schedule(static) reduction(+ : undecided)
+    for (int i = 0; i < n; ++i) {
+      newc[i] = 0;
+      if (cf[i] != 0) continue;
+      ++undecided;
+      bool mx = true;
+      for (int q = sr[i]; q < sr[i + 1] && mx; ++q) {
+        const int j = sc[q];
+        if (cf[j] == 0 && meas[j] >= meas[i] && j != i) mx = false;
+      }
+      for (int q = tr[i]; q < tr[i + 1] && mx; ++q) {
+        const int j = tc[q];
+        if (cf[j] == 0 && meas[j] >= meas[i] && j != i) mx = false;
+      }
+      newc[i] = mx;
+    }
+    if (undecided == 0) break;
+    for (int i = 0; i < n; ++i)
+      if (newc[i]) cf[i] = 1;
+    for (int i = 0; i < n; ++i)
+      if (newc[i])
+        for (int q = tr[i]; q < tr[i + 1]; ++q)
+          if (cf[tc[q]] == 0) cf[tc[q]] = -1;
+  }
+}
+
+void coarsen(const HostCSR& A, const std::vector<char>& strong, int coarse_type, uint64_t seed, std::vector<int>& cf) {
+  std::vector<int> sr, sc, tr, tc;
+  adjacency(A, strong, sr, sc, tr, tc);
+  if (coarse_type == 8) coarsen_pmis(A.n, sr, sc, tr, tc, seed, cf);
+  else coarsen_rs(A.n, sr, sc, tr, tc, cf);
+}
+
+HostCSR interp_extpi(const HostCSR& A, const std::vector<char>& strong, const std::vector<int>& cf, double trunc,
+                     int p_max, int& nc) {
+  const int n = A.n;
+  std::vector<int> cidx(n, -1);
+  nc = 0;
+  for (int i = 0; i < n; ++i)
+    if (cf[i] == 1) cidx[i] = nc++;
+  std::vector<std::vector<std::pair<int, double>>> rows(n);
+#pragma omp parallel
+  {
+    std::vector<int> pos(n, -1);
+    std::vector<int> list;
+    std::vector<double> coef;
+#pragma omp for schedule(dynamic, 512)
+    for (int i = 0; i < n; ++i) {
+      auto& out = rows[i];
+      if (cf[i] == 1) { out.push_back({cidx[i], 1.0}); continue; }
+      list.clear();
+      coef.clear();
+      auto add = [&](int j) {
+        if (pos[j] < 0) { pos[j] = (int)list.size(); list.push_back(j); coef.push_back(0.0); }
+      };
+      const int b = A.r[i], e = A.r[i + 1];
+      for (int k = b; k < e; ++k) {
+        const int j = A.c[k];
+        if (j == i || !strong[k]) continue;
+        if (cf[j] == 1) add(j);
+        else
+          for (int kk = A.r[j]; kk < A.r[j + 1]; ++kk)
+            if (strong[kk] && cf[A.c[kk]] == 1) add(A.c[kk]);
+      }
+      double diagonal = 0.0;
+      for (int k = b; k < e; ++k)
+        if (A.c[k] == i) diagonal += A.v[k];
+      for (int k = b; k < e; ++k) {
+        const int j = A.c[k];
+        if (j == i) continue;
+        const double a = A.v[k];
+        if (pos[j] >= 0) {
+          coef[pos[j]] += a;
+        } else if (strong[k] && cf[j] != 1) {
+          double ajj = 0.0;
+          for (int kk = A.r[j]; kk < A.r[j + 1]; ++kk)
+            if (A.c[kk] == j) { ajj = A.v[kk]; break; }
+          const double sgn = (ajj < 0.0) ? -1.0 : 1.0;
+          double sum = 0.0;
+          for (int kk = A.r[j]; kk < A.r[j + 1]; ++kk) {
+            const int jj = A.c[kk];
+            if ((pos[jj] >= 0 || jj == i) && sgn * A.v[kk] < 0.0) sum += A.v[kk];
+          }
+          if (sum != 0.0) {
+            const double dist = a / sum;
+            for (int kk = A.r[j]; kk < A.r[j + 1]; ++kk) {
+              const int jj = A.c[kk];
+              if (sgn * A.v[kk] >= 0.0) continue;
+              if (pos[jj] >= 0) coef[pos[jj]] += dist * A.v[kk];
+              else if (jj == i) diagonal += dist * A.v[kk];
+            }
+          } else {
+            diagonal += a;
+          }
+        } else {
+          diagonal += a;
+        }
+      }
+      for (int j : list) pos[j] = -1;
+      if (list.empty()) continue;
+      for (size_t q = 0; q < coef.size(); ++q) coef[q] = (diagonal != 0.0) ? -coef[q] / diagonal : 0.0;
+      std::vector<std::pair<int, double>> ent(list.size());
+      for (size_t q = 0; q < list.size(); ++q) ent[q] = {list[q], coef[q]};
+      // truncation relative to the row maximum, keeping the row sum
+      if (trunc > 0.0) {
+        double mx = 0.0, rs = 0.0;
+        for (auto& p : ent) { mx = std::max(mx, std::fabs(p.second)); rs += p.second; }
+        const double thr = trunc * mx;
+        std::vector<std::pair<int, double>> kept;
+        double ks = 0.0;
+        for (auto& p : ent)
+          if (std::fabs(p.second) >= thr) { kept.push_back(p); ks += p.second; }
+        if (kept.size() < ent.size() && ks != 0.0) {
+          const double sc = rs / ks;
+          for (auto& p : kept) p.second *= sc;
+        }
+        ent.swap(kept);
+      }
+      if (p_max > 0 && (int)ent.size() > p_max) {
+        double rs = 0.0;
+        for (auto& p : ent) rs += p.second;
+        std::sort(ent.begin(), ent.end(), [](const std::pair<int, double>& a, const std::pair<int, double>& b) {
+          const double fa = std::fabs(a.second), fb = std::fabs(b.second);
+          if (fa != fb) return fa > fb;
+          return a.first < b.first;
+        });
+        ent.resize(p_max);
+        double ks = 0.0;
+        for (auto& p : ent) ks += p.second;
+        if (ks != 0.0) {
+          const double sc = rs / ks;
+          for (auto& p : ent) p.second *= sc;
+        }
+      }
+      for (auto& p : ent) out.push_back({cidx[p.first], p.second});
+      std::sort(out.begin(), out.end());
+    }
+  }
+  HostCSR P;
+  P.n = n;
+  P.m = nc;
+  P.r.assign(n + 1, 0);
+  for (int i = 0; i < n; ++i) P.r[i + 1] = P.r[i] + (int)rows[i].size();
+  P.c.resize(P.r[n]);
+  P.v.resize(P.r[n]);
+  for (int i = 0; i < n; ++i)
+    for (size_t q = 0; q < rows[i].size(); ++q) {
+      P.c[P.r[i] + q] = rows[i][q].first;
+      P.v[P.r[i] + q] = rows[i][q].second;
+    }
+  return P;
+}
+
+bool dense_inverse(const HostCSR& A, std::vector<double>& inv) {
+  const int n = A.n;
+  std::vector<double> a((size_t)n * n, 0.0);  // column-major
+  for (int i = 0; i < n; ++i)
+    for (int k = A.r[i]; k < A.r[i + 1]; ++k) a[i + (size_t)A.c[k] * n] = A.v[k];
+  std::vector<int> piv(n);
+  for (int j = 0; j < n; ++j) {
+    int p = j;
+    double mx = std::fabs(a[j + (size_t)j * n]);
+    for (int i = j + 1; i < n; ++i)
+      if (std::fabs(a[i + (size_t)j * n]) > mx) { mx = std::fabs(a[i + (size_t)j * n]); p = i; }
+    piv[j] = p;
+    if (a[p + (size_t)j * n] == 0.0) return false;
+    if (p != j)
+      for (int q = 0; q < n; ++q) std::swap(a[j + (size_t)q * n], a[p + (size_t)q * n]);
+    const double rp = 1.0 / a[j + (size_t)j * n];
+    for (int i = j + 1; i < n; ++i) a[i + (size_t)j * n] *= rp;
+    for (int q = j + 1; q < n; ++q) {
+      const double t = a[j + (size_t)q * n];
+      if (t != 0.0)
+        for (int i = j + 1; i < n; ++i) a[i + (size_t)q * n] -= a[i + (size_t)j * n] * t;
+    }
+  }
+  inv.assign((size_t)n * n, 0.0);
+  std::vector<double> col(n);
+  for (int k = 0; k < n; ++k) {
+    std::fill(col.begin(), col.end(), 0.0);
+    col[k] = 1.0;
+    for (int j = 0; j < n; ++j)
+      if (piv[j] != j) std::swap(col[j], col[piv[j]]);
+    for (int j = 0; j < n; ++j) {
+      const double t = col[j];
+      if (t != 0.0)
+        for (int i = j + 1; i < n; ++i) col[i] -= t * a[i + (size_t)j * n];
+    }
+    for (int j = n - 1; j >= 0; --j) {
+      col[j] /= a[j + (size_t)j * n];
+      const double t = col[j];
+      for (int i = 0; i < j; ++i) col[i] -= t * a[i + (size_t)j * n];
+    }
+    for (int i = 0; i < n; ++i) inv[(size_t)i * n + k] = col[i];
+  }
+  return true;
+}
+
+}  // namespace amg
+}  // namespace deco

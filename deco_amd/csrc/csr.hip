@@ -456,6 +456,34 @@ void to_f32(const double* a, float* b, int64_t n) {
 
 deco_csr::~deco_csr() { delete At; }
 
+namespace deco {
+// rb[k] = first row i in [0, n] with r[i] >= k*CAP; rb[nchunks] = n
+__global__ void k_row_chunks(int n, const int* __restrict__ r, int nchunks, int* __restrict__ rb) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > nchunks) return;
+  if (k == nchunks) { rb[k] = n; return; }
+  const int target = k * kStreamCap;
+  int lo = 0, hi = n;  // answer in [0, n]
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (r[mid] >= target) hi = mid; else lo = mid + 1;
+  }
+  rb[k] = lo;
+}
+}  // namespace deco
+
+const int* deco_csr::row_chunks() const {
+  if (!rb_valid) {
+    nchunks = (int)std::max<int64_t>(1, ((int64_t)nnz + deco::kStreamCap - 1) / deco::kStreamCap);
+    rb.ensure(nchunks + 1);
+    hipLaunchKernelGGL(deco::k_row_chunks, dim3(deco::grid_for(nchunks + 1, 256)), dim3(256), 0, deco::stream(), n,
+                       r.p, nchunks, rb.p);
+    DECO_CHECK_LAUNCH();
+    rb_valid = true;
+  }
+  return rb.p;
+}
+
 const float* deco_csr::values_f32() {
   if (!vf_valid) {
     vf.ensure(nnz);
@@ -642,8 +670,10 @@ int deco_spmv(const deco_csr* A, const double* x, double* y, int bs) {
   return guard([&] {
     if (!A) fail(DECO_ERR_ARG, "null CSR");
     if (bs < 1 || A->n % bs != 0) fail(DECO_ERR_ARG, "n=%d not divisible by bs=%d", A->n, bs);
-    launch_spmv<EPI_SET, double, double, double, double>(csr_group(A), A->n / bs, bs, A->r.p, A->c.p, A->v.p, x, y,
-                                                         nullptr, 0, 0);
+    if (bs == 1) spmv(A, x, y);
+    else
+      launch_spmv<EPI_SET, double, double, double, double>(csr_group(A), A->n / bs, bs, A->r.p, A->c.p, A->v.p, x, y,
+                                                           nullptr, 0, 0);
     sync();
   });
 }
@@ -669,7 +699,7 @@ int deco_spmv_f32(deco_csr* A, const float* x, float* y) {
   return guard([&] {
     if (!A) fail(DECO_ERR_ARG, "null CSR");
     const float* vf = A->values_f32();
-    launch_spmv<EPI_SET, float, float, float, float>(csr_group(A), A->n, 1, A->r.p, A->c.p, vf, x, y, nullptr, 0, 0);
+    spmv_stream<EPI_SET, float, float, float, float>(A, vf, x, y, (const float*)nullptr);
     sync();
   });
 }
@@ -680,25 +710,20 @@ int deco_spmv_time(deco_csr* A, int precision, int iters, double* avg_ms) {
     hipEvent_t e0, e1;
     DECO_HIP(hipEventCreate(&e0));
     DECO_HIP(hipEventCreate(&e1));
-    const int G = csr_group(A);
     if (precision == DECO_PREC_FP32) {
       const float* vf = A->values_f32();
       DBuf<float> x(A->m), y(A->n);
       x.zero();
-      launch_spmv<EPI_SET, float, float, float, float>(G, A->n, 1, A->r.p, A->c.p, vf, x.p, y.p, nullptr, 0, 0);
+      spmv_stream<EPI_SET, float, float, float, float>(A, vf, x.p, y.p, (const float*)nullptr);
       DECO_HIP(hipEventRecord(e0, stream()));
-      for (int k = 0; k < iters; ++k)
-        launch_spmv<EPI_SET, float, float, float, float>(G, A->n, 1, A->r.p, A->c.p, vf, x.p, y.p, nullptr, 0, 0);
+      for (int k = 0; k < iters; ++k) spmv_stream<EPI_SET, float, float, float, float>(A, vf, x.p, y.p, (const float*)nullptr);
       DECO_HIP(hipEventRecord(e1, stream()));
     } else {
       DBuf<double> x(A->m), y(A->n);
       x.zero();
-      launch_spmv<EPI_SET, double, double, double, double>(G, A->n, 1, A->r.p, A->c.p, A->v.p, x.p, y.p, nullptr, 0,
-                                                           0);
+      spmv(A, x.p, y.p);
       DECO_HIP(hipEventRecord(e0, stream()));
-      for (int k = 0; k < iters; ++k)
-        launch_spmv<EPI_SET, double, double, double, double>(G, A->n, 1, A->r.p, A->c.p, A->v.p, x.p, y.p, nullptr,
-                                                             0, 0);
+      for (int k = 0; k < iters; ++k) spmv(A, x.p, y.p);
       DECO_HIP(hipEventRecord(e1, stream()));
     }
     DECO_HIP(hipEventSynchronize(e1));

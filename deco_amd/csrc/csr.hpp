@@ -12,9 +12,14 @@ struct deco_csr {
   deco::DBuf<float> vf;      // f32 copy of v, valid when vf_valid
   bool vf_valid = false;
   deco_csr* At = nullptr;    // cached transpose (adjoint SpMV)
+  // SpMV row chunks: chunk k holds the rows whose first entry lies in [k*CAP, (k+1)*CAP)
+  mutable deco::DBuf<int> rb;
+  mutable int nchunks = 0;
+  mutable bool rb_valid = false;
   ~deco_csr();
   const float* values_f32();  // creates / refreshes the shadow
-  void invalidate() { vf_valid = false; }
+  const int* row_chunks() const;  // creates the chunk table on first use
+  void invalidate() { vf_valid = false; rb_valid = false; }
 };
 
 namespace deco {
@@ -92,6 +97,94 @@ void launch_spmv(int G, int nout, int rs, const int* r, const int* c, const VT* 
   DECO_CHECK_LAUNCH();
 }
 
+// ------------------------------------------------------------------ CSR-stream SpMV ------
+// Block k streams the contiguous entry range of its row chunk into LDS: every lane issues
+// kStreamUnroll independent (column, value) loads and then the x gathers, so many bytes are in
+// flight per wave; the products are then summed per row in column order (the same order as
+// alg::spmv, core/alg.hpp:59-61). Chunks whose range exceeds the LDS tile (very long rows)
+// fall back to a block reduction per row.
+constexpr int kStreamCap = 2048;                 // target entries per chunk
+constexpr int kStreamLds = 2 * kStreamCap;       // LDS tile (entries), room for the last row
+constexpr int kStreamBlock = 256;
+constexpr int kStreamUnroll = kStreamCap / kStreamBlock;
+
+// Epilogue functors: called once per row with the row sum (accumulator type AT).
+template <int EPI, class YT, class ZT>
+struct EpiStd {
+  YT* __restrict__ y;
+  const ZT* __restrict__ z;
+  template <class AT>
+  __device__ __forceinline__ void operator()(int o, AT acc) const {
+    if constexpr (EPI == EPI_SET) y[o] = (YT)acc;
+    else if constexpr (EPI == EPI_RES) y[o] = (YT)((AT)z[o] - acc);
+    else if constexpr (EPI == EPI_ACC) y[o] = (YT)((AT)y[o] + acc);
+    else y[o] = (YT)((AT)z[o] + acc);
+  }
+};
+
+template <class VT, class XT, class E>
+__global__ __launch_bounds__(kStreamBlock) void k_spmv_stream(const int* __restrict__ rb, const int* __restrict__ r,
+                                                             const int* __restrict__ c, const VT* __restrict__ v,
+                                                             const XT* __restrict__ x, E epi) {
+  using AT = acc_t<VT, XT>;
+  __shared__ AT prod[kStreamLds];
+  const int r0 = rb[blockIdx.x], r1 = rb[blockIdx.x + 1];
+  if (r0 >= r1) return;
+  const int j0 = r[r0], j1 = r[r1];
+  const int cnt = j1 - j0;
+  const int tid = threadIdx.x;
+  if (cnt <= kStreamLds) {
+    // first tile: kStreamUnroll independent loads per lane
+    int cc[kStreamUnroll];
+    VT vv[kStreamUnroll];
+#pragma unroll
+    for (int u = 0; u < kStreamUnroll; ++u) {
+      const int t = tid + u * kStreamBlock;
+      if (t < cnt) {
+        cc[u] = __builtin_nontemporal_load(c + j0 + t);
+        vv[u] = __builtin_nontemporal_load(v + j0 + t);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kStreamUnroll; ++u) {
+      const int t = tid + u * kStreamBlock;
+      if (t < cnt) prod[t] = (AT)vv[u] * (AT)x[cc[u]];
+    }
+    for (int t = tid + kStreamCap; t < cnt; t += kStreamBlock)
+      prod[t] = (AT)__builtin_nontemporal_load(v + j0 + t) * (AT)x[__builtin_nontemporal_load(c + j0 + t)];
+    __syncthreads();
+    for (int row = r0 + tid; row < r1; row += kStreamBlock) {
+      const int b = r[row] - j0, e = r[row + 1] - j0;
+      AT acc = AT(0);
+      for (int k = b; k < e; ++k) acc += prod[k];
+      epi(row, acc);
+    }
+  } else {
+    __shared__ AT red[kStreamBlock / 64];
+    for (int row = r0; row < r1; ++row) {
+      const int b = r[row], e = r[row + 1];
+      AT acc = AT(0);
+      for (int j = b + tid; j < e; j += kStreamBlock) acc += (AT)v[j] * (AT)x[c[j]];
+      acc = block_sum<kStreamBlock>(acc, red);
+      if (tid == 0) epi(row, acc);
+    }
+  }
+}
+
+template <class VT, class XT, class E>
+void spmv_stream_f(const deco_csr* A, const VT* v, const XT* x, E epi) {
+  if (A->n <= 0) return;
+  const int* rb = A->row_chunks();
+  hipLaunchKernelGGL((k_spmv_stream<VT, XT, E>), dim3(A->nchunks), dim3(kStreamBlock), 0, stream(), rb, A->r.p,
+                     A->c.p, v, x, epi);
+  DECO_CHECK_LAUNCH();
+}
+
+template <int EPI, class VT, class XT, class YT, class ZT>
+void spmv_stream(const deco_csr* A, const VT* v, const XT* x, YT* y, const ZT* z) {
+  spmv_stream_f(A, v, x, EpiStd<EPI, YT, ZT>{y, z});
+}
+
 // Convenience wrappers on a deco_csr (values f64 or the f32 shadow).
 inline int csr_group(const deco_csr* A, int rs = 1) {
   const int rows = A->n / rs;
@@ -100,12 +193,11 @@ inline int csr_group(const deco_csr* A, int rs = 1) {
 
 template <class XT, class YT>
 void spmv(const deco_csr* A, const XT* x, YT* y) {
-  launch_spmv<EPI_SET, double, XT, YT, double>(csr_group(A), A->n, 1, A->r.p, A->c.p, A->v.p, x, y,
-                                               nullptr, 0, 0);
+  spmv_stream<EPI_SET, double, XT, YT, double>(A, A->v.p, x, y, nullptr);
 }
 template <class XT, class YT, class ZT>
 void spmv_res(const deco_csr* A, const XT* x, const ZT* z, YT* y) {
-  launch_spmv<EPI_RES, double, XT, YT, ZT>(csr_group(A), A->n, 1, A->r.p, A->c.p, A->v.p, x, y, z, 1, 0);
+  spmv_stream<EPI_RES, double, XT, YT, ZT>(A, A->v.p, x, y, z);
 }
 
 // ------------------------------------------------------------------ scans --------------

@@ -1,0 +1,69 @@
+// Preconditioner interface shared by CPR, FSAI, Jacobi, AMG and identity.
+// apply(r, x): x = M^{-1} r on device buffers, queued on the library stream
+// (CPRBase::apply, precond/CPR.hpp:3-8; the Mglobal/Mlocal apply members).
+#pragma once
+#include "csr.hpp"
+
+struct deco_prec {
+  enum Kind { IDENTITY = 0, JACOBI = 1, FSAI = 2, AMG = 3, CPR = 4 };
+  int kind = IDENTITY;
+  int n = 0;
+  int precision = DECO_PREC_FP64;
+  virtual ~deco_prec() = default;
+  virtual void apply(const double* r, double* x) = 0;
+  virtual void apply_t(const double* r, double* x) {
+    (void)r; (void)x;
+    deco::fail(DECO_ERR_UNSUPPORTED, "transposed apply not available for this preconditioner");
+  }
+  // FP32 entry points used inside CPR (input may be f64 or f32, output f32)
+  virtual void apply32(const float* r, float* x) {
+    (void)r; (void)x;
+    deco::fail(DECO_ERR_UNSUPPORTED, "FP32 apply not available for this preconditioner");
+  }
+  virtual void apply32_t(const float* r, float* x) {
+    (void)r; (void)x;
+    deco::fail(DECO_ERR_UNSUPPORTED, "FP32 transposed apply not available for this preconditioner");
+  }
+  virtual void set_precision(int p) { precision = p; }
+};
+
+namespace deco {
+
+// elementwise helpers used across preconditioners
+template <class A, class B>
+__global__ void k_convert(int64_t n, const A* __restrict__ a, B* __restrict__ b) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (B)a[i];
+}
+template <class A, class B>
+void convert(const A* a, B* b, int64_t n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL((k_convert<A, B>), dim3(grid_for(n, 256)), dim3(256), 0, stream(), n, a, b);
+  DECO_CHECK_LAUNCH();
+}
+
+struct Identity : deco_prec {
+  explicit Identity(int n_) { kind = IDENTITY; n = n_; }
+  void apply(const double* r, double* x) override { copy_d2d(x, r, n); }
+  void apply_t(const double* r, double* x) override { copy_d2d(x, r, n); }
+  void apply32(const float* r, float* x) override { copy_d2d(x, r, n); }
+  void apply32_t(const float* r, float* x) override { copy_d2d(x, r, n); }
+};
+
+// Point / block Jacobi (precond/Jacobi/Jacobi.hpp:15-68, Jacobi.tpp:3-69).
+struct Jacobi : deco_prec {
+  int bs = 1;
+  DBuf<double> J;     // bs==1: 1/a_ii ; else per cell bs*bs row-major inverse
+  DBuf<float> Jf;
+  Jacobi(const deco_csr* A, int bs_);
+  void apply(const double* r, double* x) override;
+  void apply_t(const double* r, double* x) override;
+  void apply32(const float* r, float* x) override;
+  void apply32_t(const float* r, float* x) override;
+};
+
+deco_prec* make_fsai(const deco_cpr_params& p, const deco_csr* A);
+deco_prec* make_amg(const deco_cpr_params& p, const deco_csr* Ap);
+deco_prec* make_cpr(const deco_cpr_params& p, deco_csr* A, double* b);
+
+}  // namespace deco

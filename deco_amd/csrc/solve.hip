@@ -1,0 +1,675 @@
+// Krylov solvers on the device: rgmres, lgmres, bicgstab, lbicgstab, pcg (solver/*.tpp).
+//
+// The host drives the iteration; every vector operation is a device kernel on the library
+// stream. Reductions are deterministic (fixed grid, fixed-order partial sums). The GMRES
+// Hessenberg column comes back to the host each inner step (j+2 doubles) and the Givens
+// rotations and back-substitution run on the host with the reference's arithmetic
+// (solver/rgmres.tpp:80-104,141-160), so they are bit-identical given the same column.
+#include "prec.hpp"
+#include "blas.hpp"
+#include <cmath>
+#include <vector>
+#include <chrono>
+
+namespace deco {
+
+// ------------------------------------------------------------------ kernels ------------
+__global__ void k_scale(int n, double a, const double* __restrict__ x, double* __restrict__ y) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = a * x[i];
+}
+__global__ void k_add(int n, const double* __restrict__ z, double* __restrict__ x) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] += z[i];
+}
+__global__ void k_sub_from(int n, const double* __restrict__ b, double* __restrict__ r) {  // r = b - r
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) r[i] = b[i] - r[i];
+}
+
+// One fused modified-Gram-Schmidt step. If vprev: h = reduce(part_prev) (every block gets the
+// same value), w = 1.0*w + (-h)*vprev (rgmres.tpp:74) and block 0 stores h. Then the partial
+// sums of vcur.w (rgmres.tpp:73), or of w.w when vcur is null (rgmres.tpp:76).
+__global__ __launch_bounds__(kRedBlock) void k_mgs_step(int n, const double* __restrict__ part_prev, int np,
+                                                       const double* __restrict__ vprev, double* __restrict__ w,
+                                                       const double* __restrict__ vcur, double* __restrict__ part_out,
+                                                       double* __restrict__ hout) {
+  __shared__ double sm[kRedBlock / 64];
+  double h = 0.0;
+  if (vprev) {
+    h = reduce_partials(part_prev, np);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *hout = h;
+  }
+  const double mh = -h;
+  double s = 0.0;
+  for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock) {
+    double wi = w[i];
+    if (vprev) {
+      wi = 1.0 * wi + mh * vprev[i];
+      w[i] = wi;
+    }
+    s += (vcur ? vcur[i] : wi) * wi;
+  }
+  s = block_sum<kRedBlock>(s, sm);
+  if (threadIdx.x == 0) part_out[blockIdx.x] = s;
+}
+
+// nrm = sqrt(reduce(part)); v = (1/nrm) * w (rgmres.tpp:76-77); block 0 stores nrm.
+__global__ __launch_bounds__(kRedBlock) void k_mgs_norm(int n, const double* __restrict__ part, int np,
+                                                       const double* __restrict__ w, double* __restrict__ v,
+                                                       double* __restrict__ hout) {
+  const double nrm = sqrt(reduce_partials(part, np));
+  if (blockIdx.x == 0 && threadIdx.x == 0) *hout = nrm;
+  const double inv = 1.0 / nrm;
+  for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock) v[i] = inv * w[i];
+}
+
+// w = g0*V0 + sum_{i=1}^{j-1} g_i V_i in the order of rgmres.tpp:106-109
+__global__ void k_combine(int n, int j, const double* __restrict__ V, int64_t ldv, const double* __restrict__ g,
+                          double* __restrict__ w) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double s = g[0] * V[k];
+  for (int i = 1; i < j; ++i) s = 1.0 * s + g[i] * V[(int64_t)i * ldv + k];
+  w[k] = s;
+}
+
+// partial sums of up to three products over the grid: part[s*kRedMaxGrid + block]
+template <int K>
+__global__ __launch_bounds__(kRedBlock) void k_dots(int n, const double* __restrict__ a0, const double* __restrict__ b0,
+                                                   const double* __restrict__ a1, const double* __restrict__ b1,
+                                                   double* __restrict__ part) {
+  __shared__ double sm[kRedBlock / 64];
+  double s0 = 0.0, s1 = 0.0;
+  for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock) {
+    s0 += a0[i] * b0[i];
+    if (K > 1) s1 += a1[i] * b1[i];
+  }
+  s0 = block_sum<kRedBlock>(s0, sm);
+  if (threadIdx.x == 0) part[blockIdx.x] = s0;
+  if (K > 1) {
+    s1 = block_sum<kRedBlock>(s1, sm);
+    if (threadIdx.x == 0) part[kRedMaxGrid + blockIdx.x] = s1;
+  }
+}
+
+// BiCGSTAB: p = beta*p + (-beta*omega)*v + 1.0*r (bicgstab.tpp:254, axpby 3-term form)
+__global__ void k_bicg_p(int n, double beta, double bo, const double* __restrict__ r, const double* __restrict__ v,
+                         double* __restrict__ p) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = beta * p[i] + bo * v[i] + 1.0 * r[i];
+}
+
+// r = 1.0*r + (-a)*v ; [x = 1.0*x + a*y] ; partials of r.r (and r.rt)
+template <bool UPDX, bool DOT_RT>
+__global__ __launch_bounds__(kRedBlock) void k_bicg_update(int n, double a, const double* __restrict__ v,
+                                                          double* __restrict__ r, double ax,
+                                                          const double* __restrict__ y, double* __restrict__ x,
+                                                          const double* __restrict__ rt, double* __restrict__ part) {
+  __shared__ double sm[kRedBlock / 64];
+  double s0 = 0.0, s1 = 0.0;
+  for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock) {
+    if (UPDX) x[i] = 1.0 * x[i] + ax * y[i];
+    const double ri = 1.0 * r[i] + (-a) * v[i];
+    r[i] = ri;
+    s0 += ri * ri;
+    if (DOT_RT) s1 += ri * rt[i];
+  }
+  s0 = block_sum<kRedBlock>(s0, sm);
+  if (threadIdx.x == 0) part[blockIdx.x] = s0;
+  if (DOT_RT) {
+    s1 = block_sum<kRedBlock>(s1, sm);
+    if (threadIdx.x == 0) part[kRedMaxGrid + blockIdx.x] = s1;
+  }
+}
+
+// y = b*y + a*x
+__global__ void k_axpby2(int n, double a, const double* __restrict__ x, double b, double* __restrict__ y) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = b * y[i] + a * x[i];
+}
+
+// ------------------------------------------------------------------ host helpers -------
+struct Solver {
+  int n;
+  int nb;                 // reduction grid
+  RedScratch rs;
+  std::vector<double> hpart;
+  explicit Solver(int n_) : n(n_), nb(red_grid(n_)), hpart(2 * kRedMaxGrid) {}
+
+  unsigned g256() const { return grid_for(n, 256); }
+  // sum the first k partial slots on the host (fixed order)
+  void host_sums(int k, double* out) {
+    DECO_HIP(hipMemcpyAsync(ctx().pinned, rs.part.p, sizeof(double) * kRedMaxGrid * k, hipMemcpyDeviceToHost,
+                            stream()));
+    sync();
+    for (int s = 0; s < k; ++s) {
+      double t = 0.0;
+      for (int b = 0; b < nb; ++b) t += ctx().pinned[s * kRedMaxGrid + b];
+      out[s] = t;
+    }
+  }
+  double dot(const double* a, const double* b) {
+    hipLaunchKernelGGL((k_dots<1>), dim3(nb), dim3(kRedBlock), 0, stream(), n, a, b, nullptr, nullptr, rs.part.p);
+    DECO_CHECK_LAUNCH();
+    double s;
+    host_sums(1, &s);
+    return s;
+  }
+  double nrm2(const double* a) { return std::sqrt(dot(a, a)); }
+  void dot2(const double* a0, const double* b0, const double* a1, const double* b1, double* out) {
+    hipLaunchKernelGGL((k_dots<2>), dim3(nb), dim3(kRedBlock), 0, stream(), n, a0, b0, a1, b1, rs.part.p);
+    DECO_CHECK_LAUNCH();
+    host_sums(2, out);
+  }
+  void scale(double a, const double* x, double* y) {
+    hipLaunchKernelGGL(k_scale, dim3(g256()), dim3(256), 0, stream(), n, a, x, y);
+    DECO_CHECK_LAUNCH();
+  }
+  void add(const double* z, double* x) {
+    hipLaunchKernelGGL(k_add, dim3(g256()), dim3(256), 0, stream(), n, z, x);
+    DECO_CHECK_LAUNCH();
+  }
+  void axpby(double a, const double* x, double b, double* y) {
+    hipLaunchKernelGGL(k_axpby2, dim3(g256()), dim3(256), 0, stream(), n, a, x, b, y);
+    DECO_CHECK_LAUNCH();
+  }
+};
+
+static void apply_givens(double c, double s, double& dx, double& dy) {   // rgmres.tpp:156-160
+  const double t = c * dx + s * dy;
+  dy = -s * dx + c * dy;
+  dx = t;
+}
+static void comp_givens(double x, double y, double& c, double& s) {      // rgmres.tpp:141-154
+  const double EPS = 2.220446049250312e-16;
+  if (std::fabs(y) < EPS) { c = 1.0; s = 0.0; }
+  else if (std::fabs(y) > std::fabs(x)) { const double nu = x / y; s = 1.0 / std::sqrt(1.0 + nu * nu); c = nu * s; }
+  else { const double nu = y / x; c = 1.0 / std::sqrt(1.0 + nu * nu); s = nu * c; }
+}
+
+struct AOp {
+  const deco_csr* A;
+  bool transpose;
+  void apply(const double* x, double* y) const {
+    if (!transpose) spmv(A, x, y);
+    else spmv(A->At, x, y);
+  }
+  void residual(const double* x, const double* b, double* r) const {  // r = b - A x
+    spmv_res(transpose ? A->At : A, x, b, r);
+  }
+};
+
+struct MOp {
+  deco_prec* M;
+  bool transpose;
+  int n;
+  void apply(const double* r, double* x) const {
+    if (!M) copy_d2d(x, r, n);
+    else if (transpose) M->apply_t(r, x);
+    else M->apply(r, x);
+  }
+};
+
+// rgmres (solver/rgmres.tpp:9-139) and lgmres (solver/lgmres.tpp:2-134)
+static void gmres(bool left, const deco_sol_params& sp, const AOp& A, const MOp& M, const double* b, double* x,
+                  deco_sol_info& info) {
+  const int n = A.transpose ? A.A->m : A.A->n;
+  Solver S(n);
+  const double tol = sp.reltol;
+  const int m = sp.restart, itmax = sp.itmax;
+  if (m <= 0) fail(DECO_ERR_ARG, "restart must be positive");
+  DBuf<double> r(n), z(n), w(n), V((size_t)n * (m + 1)), gdev(m + 1), hcol(m + 2);
+  std::vector<double> H((size_t)(m + 1) * (m + 3), 0.0);
+  double* g = H.data() + (size_t)(m + 1) * m;
+  double* cs = g + (m + 1);
+  double* sn = cs + (m + 1);
+  auto Hm = [&](int i, int j) -> double& { return H[i + (size_t)j * (m + 1)]; };
+  int flag = 1, iter = 0, napp = 0, nspmv = 0;
+  double bnorm, absTol, rnorm = 0.0;
+  if (!left) {
+    bnorm = S.nrm2(b);
+    absTol = tol * bnorm;
+    if (!sp.use_x0) { DECO_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, stream())); copy_d2d(r.p, b, n); }
+    else { A.residual(x, b, r.p); ++nspmv; }
+  } else {
+    if (!sp.use_x0) { DECO_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, stream())); M.apply(b, r.p); ++napp; }
+    else { A.residual(x, b, z.p); ++nspmv; M.apply(z.p, r.p); ++napp; }
+    bnorm = S.nrm2(r.p);
+    absTol = tol * bnorm;
+  }
+  const int nb = S.nb;
+  std::vector<double> hh(m + 2);
+  while (iter < itmax) {
+    std::fill(H.begin(), H.end(), 0.0);
+    g[0] = S.nrm2(r.p);
+    S.scale(1.0 / g[0], r.p, V.p);
+    int j = 0;
+    while (j < m && iter < itmax) {
+      rnorm = std::fabs(g[j]);
+      if (rnorm < absTol) { flag = 0; break; }
+      double* vj = V.p + (size_t)j * n;
+      if (!left) { M.apply(vj, z.p); A.apply(z.p, w.p); }
+      else { A.apply(vj, z.p); M.apply(z.p, w.p); }
+      ++napp; ++nspmv;
+      // fused MGS: j+2 kernels, partial slots ping-pong
+      double* slot[2] = {S.rs.slot(0), S.rs.slot(1)};
+      hipLaunchKernelGGL(k_mgs_step, dim3(nb), dim3(kRedBlock), 0, stream(), n, (const double*)nullptr, 0,
+                         (const double*)nullptr, w.p, (const double*)V.p, slot[0], (double*)nullptr);
+      DECO_CHECK_LAUNCH();
+      for (int i = 1; i <= j + 1; ++i) {
+        const double* vprev = V.p + (size_t)(i - 1) * n;
+        const double* vcur = (i <= j) ? V.p + (size_t)i * n : nullptr;
+        hipLaunchKernelGGL(k_mgs_step, dim3(nb), dim3(kRedBlock), 0, stream(), n, (const double*)slot[(i - 1) & 1], nb,
+                           vprev, w.p, vcur, slot[i & 1], hcol.p + (i - 1));
+        DECO_CHECK_LAUNCH();
+      }
+      hipLaunchKernelGGL(k_mgs_norm, dim3(nb), dim3(kRedBlock), 0, stream(), n, (const double*)slot[(j + 1) & 1], nb,
+                         (const double*)w.p, V.p + (size_t)(j + 1) * n, hcol.p + (j + 1));
+      DECO_CHECK_LAUNCH();
+      DECO_HIP(hipMemcpyAsync(hh.data(), hcol.p, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, stream()));
+      sync();
+      for (int i = 0; i <= j + 1; ++i) Hm(i, j) = hh[i];
+      for (int i = 0; i <= j - 1; ++i) apply_givens(cs[i], sn[i], Hm(i, j), Hm(i + 1, j));
+      comp_givens(Hm(j, j), Hm(j + 1, j), cs[j], sn[j]);
+      apply_givens(cs[j], sn[j], Hm(j, j), Hm(j + 1, j));
+      apply_givens(cs[j], sn[j], g[j], g[j + 1]);
+      ++j;
+      ++iter;
+    }
+    for (int i = j - 1; i >= 0; --i) {                      // rgmres.tpp:95-104
+      if (Hm(i, i) == 0.0) fail(DECO_ERR_BREAKDOWN, "Zero diagonal element in upper triangular matrix");
+      g[i] /= Hm(i, i);
+      for (int k = 0; k < i; ++k) g[k] -= Hm(k, i) * g[i];
+    }
+    copy_h2d(gdev.p, g, std::max(j, 1));
+    hipLaunchKernelGGL(k_combine, dim3(S.g256()), dim3(256), 0, stream(), n, j, (const double*)V.p, (int64_t)n,
+                       (const double*)gdev.p, w.p);
+    DECO_CHECK_LAUNCH();
+    if (!left) {
+      M.apply(w.p, z.p); ++napp;
+      S.add(z.p, x);
+      A.residual(x, b, r.p); ++nspmv;
+    } else {
+      S.add(w.p, x);
+      A.residual(x, b, z.p); ++nspmv;
+      M.apply(z.p, r.p); ++napp;
+    }
+    rnorm = S.nrm2(r.p);
+    if (rnorm < absTol) { flag = 0; break; }
+  }
+  info.iter = iter;
+  info.res_abs = rnorm;
+  info.res_rel = rnorm / bnorm;
+  info.flag = flag;
+  info.bnorm = bnorm;
+  info.n_prec = napp;
+  info.n_spmv = nspmv;
+}
+
+// bicgstab (solver/bicgstab.tpp:2-99) and lbicgstab (solver/lbicgstab.tpp:2-95)
+static void bicgstab(bool left, const deco_sol_params& sp, const AOp& A, const MOp& M, const double* b, double* x,
+                     deco_sol_info& info) {
+  const int n = A.transpose ? A.A->m : A.A->n;
+  Solver S(n);
+  DBuf<double> r(n), rt(n), p(n), y(n), v(n), t(n);
+  double rho_old, rho = 1, alpha = 1, beta, omega = 1;
+  const double reltol = sp.reltol;
+  const int itmax = sp.itmax;
+  float it = 0.0f;
+  int flag = 1, napp = 0, nspmv = 0;
+  double norm_b, tol, res_norm;
+  if (!sp.use_x0) DECO_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, stream()));
+  if (!left) {
+    norm_b = S.nrm2(b);
+    tol = norm_b * reltol;
+    if (!sp.use_x0) copy_d2d(r.p, b, n);
+    else { A.residual(x, b, r.p); ++nspmv; }
+  } else {
+    if (!sp.use_x0) { M.apply(b, r.p); ++napp; }
+    else { A.residual(x, b, y.p); ++nspmv; M.apply(y.p, r.p); ++napp; }
+  }
+  copy_d2d(rt.p, r.p, n);
+  res_norm = S.nrm2(r.p);
+  if (left) { norm_b = res_norm; tol = norm_b * reltol; }
+  double next_rho = 0.0;
+  bool have_rho = false;
+  const int nb = S.nb;
+  double sums[2];
+  for (it = 0.5f; it < itmax; it += 0.5f) {
+    rho_old = rho;
+    rho = have_rho ? next_rho : S.dot(r.p, rt.p);           // bicgstab.tpp:248
+    if (it < 1) copy_d2d(p.p, r.p, n);
+    else {
+      beta = (std::fabs(res_norm) < 1e-40) ? 0.0 : (rho / rho_old) * (alpha / omega);
+      hipLaunchKernelGGL(k_bicg_p, dim3(S.g256()), dim3(256), 0, stream(), n, beta, -beta * omega,
+                         (const double*)r.p, (const double*)v.p, p.p);
+      DECO_CHECK_LAUNCH();
+    }
+    if (!left) { M.apply(p.p, y.p); A.apply(y.p, v.p); }
+    else { A.apply(p.p, y.p); M.apply(y.p, v.p); }
+    ++napp; ++nspmv;
+    alpha = (std::fabs(res_norm) < 1e-40) ? 0.0 : rho / S.dot(v.p, rt.p);
+    if (!left)
+      hipLaunchKernelGGL((k_bicg_update<true, false>), dim3(nb), dim3(kRedBlock), 0, stream(), n, alpha,
+                         (const double*)v.p, r.p, alpha, (const double*)y.p, x, (const double*)nullptr, S.rs.part.p);
+    else
+      hipLaunchKernelGGL((k_bicg_update<false, false>), dim3(nb), dim3(kRedBlock), 0, stream(), n, alpha,
+                         (const double*)v.p, r.p, 0.0, (const double*)nullptr, (double*)nullptr,
+                         (const double*)nullptr, S.rs.part.p);
+    DECO_CHECK_LAUNCH();
+    S.host_sums(1, sums);
+    res_norm = std::sqrt(sums[0]);
+    if (res_norm <= tol) {
+      if (left) S.axpby(alpha, y.p, 1.0, x);                  // lbicgstab.tpp:193 (y = A p)
+      flag = 0;
+      break;
+    }
+    it += 0.5f;
+    if (!left) { M.apply(r.p, y.p); A.apply(y.p, t.p); }
+    else { A.apply(r.p, y.p); M.apply(y.p, t.p); }
+    ++napp; ++nspmv;
+    if (std::fabs(res_norm) < 1e-40) omega = 0.0;
+    else {
+      S.dot2(t.p, r.p, t.p, t.p, sums);
+      omega = sums[0] / sums[1];
+    }
+    if (!left)
+      hipLaunchKernelGGL((k_bicg_update<true, true>), dim3(nb), dim3(kRedBlock), 0, stream(), n, omega,
+                         (const double*)t.p, r.p, omega, (const double*)y.p, x, (const double*)rt.p, S.rs.part.p);
+    else
+      hipLaunchKernelGGL((k_bicg_update<true, true>), dim3(nb), dim3(kRedBlock), 0, stream(), n, omega,
+                         (const double*)t.p, r.p, alpha * omega, (const double*)y.p, x, (const double*)rt.p,
+                         S.rs.part.p);
+    DECO_CHECK_LAUNCH();
+    S.host_sums(2, sums);
+    res_norm = std::sqrt(sums[0]);
+    next_rho = sums[1];
+    have_rho = true;
+    if (res_norm <= tol) { flag = 0; break; }
+  }
+  info.iter = (int)it;
+  info.res_abs = res_norm;
+  info.res_rel = res_norm / norm_b;
+  info.flag = flag;
+  info.bnorm = norm_b;
+  info.n_prec = napp;
+  info.n_spmv = nspmv;
+}
+
+// pcg (solver/pcg.tpp:3-62)
+static void pcg(const deco_sol_params& sp, const AOp& A, const MOp& M, const double* b, double* x,
+                deco_sol_info& info) {
+  const int n = A.transpose ? A.A->m : A.A->n;
+  Solver S(n);
+  DBuf<double> r(n), p(n), z(n), g(n);
+  const double norm_b = S.nrm2(b), tol = norm_b * sp.reltol;
+  int it = 0, napp = 0, nspmv = 0;
+  if (!sp.use_x0) { DECO_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, stream())); copy_d2d(r.p, b, n); }
+  else { A.residual(x, b, r.p); ++nspmv; }
+  M.apply(r.p, p.p); ++napp;
+  double rho1 = S.dot(r.p, p.p);
+  double res_norm = S.nrm2(r.p);
+  while (res_norm > tol && it < sp.itmax) {
+    A.apply(p.p, z.p); ++nspmv;
+    const double alpha = rho1 / S.dot(z.p, p.p);
+    S.axpby(alpha, p.p, 1.0, x);
+    S.axpby(-alpha, z.p, 1.0, r.p);
+    M.apply(r.p, g.p); ++napp;
+    const double rho2 = S.dot(r.p, g.p);
+    const double beta = rho2 / rho1;
+    rho1 = rho2;
+    S.axpby(1.0, g.p, beta, p.p);
+    res_norm = S.nrm2(r.p);
+    ++it;
+  }
+  info.iter = it;
+  info.res_abs = res_norm;
+  info.res_rel = res_norm / norm_b;
+  info.flag = res_norm > tol ? 1 : 0;
+  info.bnorm = norm_b;
+  info.n_prec = napp;
+  info.n_spmv = nspmv;
+}
+
+static void run_solver(const deco_sol_params& sp, const AOp& A, const MOp& M, const double* b, double* x,
+                       deco_sol_info& info) {
+  hipEvent_t e0, e1;
+  DECO_HIP(hipEventCreate(&e0));
+  DECO_HIP(hipEventCreate(&e1));
+  DECO_HIP(hipEventRecord(e0, stream()));
+  switch (sp.type) {
+    case DECO_SOL_RGMRES: gmres(false, sp, A, M, b, x, info); break;
+    case DECO_SOL_LGMRES: gmres(true, sp, A, M, b, x, info); break;
+    case DECO_SOL_BICGSTAB: bicgstab(false, sp, A, M, b, x, info); break;
+    case DECO_SOL_LBICGSTAB: bicgstab(true, sp, A, M, b, x, info); break;
+    case DECO_SOL_PCG: pcg(sp, A, M, b, x, info); break;
+    default: fail(DECO_ERR_ARG, "Invalid solver type: %d", sp.type);
+  }
+  DECO_HIP(hipEventRecord(e1, stream()));
+  DECO_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  DECO_HIP(hipEventElapsedTime(&ms, e0, e1));
+  info.time_ms = ms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (sp.verbosity > 0) {
+    printf("\n--------------------------------\n");
+    printf("Profile of %s:\n", sp.type == DECO_SOL_RGMRES ? "rGMRES" : sp.type == DECO_SOL_LGMRES ? "lGMRES"
+                                : sp.type == DECO_SOL_PCG ? "PCG" : "BICGstab");
+    printf("iteration count =  %d\n", info.iter);
+    printf("residual = %.2e\n", info.res_rel);
+  }
+}
+
+// ------------------------------------------------------------------ Jacobi -------------
+// per-cell LU with partial pivoting (dgetf2) and inverse by unit solves (dgetrf + dgetri,
+// Jacobi.tpp:64-65); the block is read as in Jacobi.tpp:311-323
+constexpr int kMaxBs = 8;
+__global__ void k_jacobi_setup(int sn, int bs, const int* __restrict__ r, const int* __restrict__ c,
+                               const double* __restrict__ v, double* __restrict__ J) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= sn) return;
+  double B[kMaxBs * kMaxBs];  // column-major
+  int piv[kMaxBs];
+  for (int q = 0; q < bs * bs; ++q) B[q] = 0.0;
+  for (int p = 0; p < bs; ++p) {
+    const int i = s * bs + p;
+    const int ia = r[i], ib = r[i + 1];
+    int mn = ia, mx = ib, mid = (ia + ib) >> 1;  // core::binarysearch
+    while (mx - mn > 1) { if (c[mid] > i) mx = mid; else mn = mid; mid = (mn + mx) >> 1; }
+    for (int k = 0; k < bs; ++k) {
+      const int jj = mid + k - p;
+      if (jj < ib && jj >= ia && c[jj] / bs == s) B[p + (c[jj] % bs) * bs] = v[jj];
+    }
+  }
+  for (int j = 0; j < bs; ++j) {
+    int pv = j;
+    double mxv = fabs(B[j + j * bs]);
+    for (int i = j + 1; i < bs; ++i) { double t = fabs(B[i + j * bs]); if (t > mxv) { mxv = t; pv = i; } }
+    piv[j] = pv;
+    if (B[pv + j * bs] != 0.0) {
+      if (pv != j) for (int q = 0; q < bs; ++q) { double t = B[j + q * bs]; B[j + q * bs] = B[pv + q * bs]; B[pv + q * bs] = t; }
+      const double pivv = B[j + j * bs];
+      if (fabs(pivv) >= 2.2250738585072014e-308) { const double rp = 1.0 / pivv; for (int i = j + 1; i < bs; ++i) B[i + j * bs] *= rp; }
+      else for (int i = j + 1; i < bs; ++i) B[i + j * bs] /= pivv;
+    }
+    for (int q = j + 1; q < bs; ++q) {
+      const double t = B[j + q * bs];
+      if (t != 0.0) for (int i = j + 1; i < bs; ++i) B[i + q * bs] -= B[i + j * bs] * t;
+    }
+  }
+  for (int k = 0; k < bs; ++k) {
+    double col[kMaxBs];
+    for (int q = 0; q < bs; ++q) col[q] = (q == k) ? 1.0 : 0.0;
+    for (int j = 0; j < bs; ++j) if (piv[j] != j) { double t = col[j]; col[j] = col[piv[j]]; col[piv[j]] = t; }
+    for (int j = 0; j < bs; ++j) { const double t = col[j]; if (t != 0.0) for (int i = j + 1; i < bs; ++i) col[i] -= t * B[i + j * bs]; }
+    for (int j = bs - 1; j >= 0; --j) {
+      if (col[j] != 0.0) { col[j] /= B[j + j * bs]; const double t = col[j]; for (int i = 0; i < j; ++i) col[i] -= t * B[i + j * bs]; }
+    }
+    for (int p = 0; p < bs; ++p) J[(size_t)s * bs * bs + p * bs + k] = col[p];
+  }
+}
+
+__global__ void k_jacobi_point_setup(int n, const int* __restrict__ r, const int* __restrict__ c,
+                                     const double* __restrict__ v, double* __restrict__ J) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int mn = r[i], mx = r[i + 1], mid = (mn + mx) >> 1;
+  while (mx - mn > 1) { if (c[mid] > i) mx = mid; else mn = mid; mid = (mn + mx) >> 1; }
+  J[i] = 1 / v[mid];
+}
+
+template <class T, bool TR>
+__global__ void k_jacobi_apply(int n, int bs, const T* __restrict__ J, const T* __restrict__ x, T* __restrict__ b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (bs == 1) { b[i] = J[i] * x[i]; return; }
+  const int s = i / bs, p = i - s * bs;
+  const T* Ap = J + (size_t)s * bs * bs;
+  T sum = (TR ? Ap[p] : Ap[p * bs]) * x[s * bs];
+  for (int k = 1; k < bs; ++k) sum += (TR ? Ap[k * bs + p] : Ap[p * bs + k]) * x[s * bs + k];
+  b[i] = sum;
+}
+
+Jacobi::Jacobi(const deco_csr* A, int bs_) {
+  kind = JACOBI;
+  n = A->n;
+  bs = bs_;
+  if (bs < 1 || bs > kMaxBs || n % bs != 0) fail(DECO_ERR_ARG, "Jacobi block size %d invalid for n=%d", bs, n);
+  if (bs == 1) {
+    J.alloc(n);
+    hipLaunchKernelGGL(k_jacobi_point_setup, dim3(grid_for(n, 256)), dim3(256), 0, stream(), n, A->r.p, A->c.p,
+                       A->v.p, J.p);
+  } else {
+    const int sn = n / bs;
+    J.alloc((size_t)sn * bs * bs);
+    hipLaunchKernelGGL(k_jacobi_setup, dim3(grid_for(sn, 128)), dim3(128), 0, stream(), sn, bs, A->r.p, A->c.p,
+                       A->v.p, J.p);
+  }
+  DECO_CHECK_LAUNCH();
+  Jf.alloc(J.n);
+  convert(J.p, Jf.p, J.n);
+}
+void Jacobi::apply(const double* r, double* x) {
+  hipLaunchKernelGGL((k_jacobi_apply<double, false>), dim3(grid_for(n, 256)), dim3(256), 0, stream(), n, bs,
+                     (const double*)J.p, r, x);
+  DECO_CHECK_LAUNCH();
+}
+void Jacobi::apply_t(const double* r, double* x) {
+  hipLaunchKernelGGL((k_jacobi_apply<double, true>), dim3(grid_for(n, 256)), dim3(256), 0, stream(), n, bs,
+                     (const double*)J.p, r, x);
+  DECO_CHECK_LAUNCH();
+}
+void Jacobi::apply32(const float* r, float* x) {
+  hipLaunchKernelGGL((k_jacobi_apply<float, false>), dim3(grid_for(n, 256)), dim3(256), 0, stream(), n, bs,
+                     (const float*)Jf.p, r, x);
+  DECO_CHECK_LAUNCH();
+}
+void Jacobi::apply32_t(const float* r, float* x) {
+  hipLaunchKernelGGL((k_jacobi_apply<float, true>), dim3(grid_for(n, 256)), dim3(256), 0, stream(), n, bs,
+                     (const float*)Jf.p, r, x);
+  DECO_CHECK_LAUNCH();
+}
+
+// ------------------------------------------------------------------ adjoint helpers ----
+__global__ void k_grad_A(int n, const int* __restrict__ r, const int* __restrict__ c, const double* __restrict__ lam,
+                         const double* __restrict__ x, double* __restrict__ dA) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double li = -lam[i];
+  for (int j = r[i]; j < r[i + 1]; ++j) dA[j] = li * x[c[j]];
+}
+
+}  // namespace deco
+
+using namespace deco;
+
+extern "C" {
+
+int deco_identity_create(int n, deco_prec** out) {
+  return guard([&] {
+    if (n < 0 || !out) fail(DECO_ERR_ARG, "bad argument");
+    *out = new Identity(n);
+  });
+}
+
+int deco_jacobi_create(const deco_cpr_params* p, const deco_csr* A, deco_prec** out) {
+  return guard([&] {
+    if (!p || !A || !out) fail(DECO_ERR_ARG, "null argument");
+    *out = new Jacobi(A, p->glo_block_size);
+    sync();
+  });
+}
+
+int deco_prec_destroy(deco_prec* M) {
+  return guard([&] { delete M; });
+}
+
+int deco_prec_apply(deco_prec* M, const double* r, double* x) {
+  return guard([&] {
+    if (!M) fail(DECO_ERR_ARG, "null preconditioner");
+    M->apply(r, x);
+    sync();
+  });
+}
+
+int deco_prec_apply_transpose(deco_prec* M, const double* r, double* x) {
+  return guard([&] {
+    if (!M) fail(DECO_ERR_ARG, "null preconditioner");
+    M->apply_t(r, x);
+    sync();
+  });
+}
+
+int deco_prec_size(const deco_prec* M, int* n) {
+  return guard([&] {
+    if (!M || !n) fail(DECO_ERR_ARG, "null argument");
+    *n = M->n;
+  });
+}
+
+int deco_prec_set_precision(deco_prec* M, int precision) {
+  return guard([&] {
+    if (!M) fail(DECO_ERR_ARG, "null preconditioner");
+    if (precision != DECO_PREC_FP64 && precision != DECO_PREC_FP32) fail(DECO_ERR_ARG, "bad precision %d", precision);
+    M->set_precision(precision);
+    sync();
+  });
+}
+
+int deco_solve(const deco_sol_params* sp, const deco_csr* A, deco_prec* M, const double* b, double* x,
+               deco_sol_info* info) {
+  return guard([&] {
+    if (!sp || !A || !b || !x) fail(DECO_ERR_ARG, "null argument");
+    if (A->n != A->m) fail(DECO_ERR_ARG, "matrix is not square");
+    if (M && M->n != A->n) fail(DECO_ERR_ARG, "preconditioner size %d != %d", M->n, A->n);
+    deco_sol_info inf{};
+    run_solver(*sp, AOp{A, false}, MOp{M, false, A->n}, b, x, inf);
+    if (info) *info = inf;
+  });
+}
+
+int deco_solve_transpose(const deco_sol_params* sp, deco_csr* A, deco_prec* M, const double* g, double* y,
+                         deco_sol_info* info) {
+  return guard([&] {
+    if (!sp || !A || !g || !y) fail(DECO_ERR_ARG, "null argument");
+    if (A->n != A->m) fail(DECO_ERR_ARG, "matrix is not square");
+    if (!A->At) A->At = csr_transpose(A);
+    deco_sol_info inf{};
+    run_solver(*sp, AOp{A, true}, MOp{M, true, A->n}, g, y, inf);
+    if (info) *info = inf;
+  });
+}
+
+int deco_grad_A_pattern(const deco_csr* A, const double* lam, const double* x, double* dA) {
+  return guard([&] {
+    if (!A) fail(DECO_ERR_ARG, "null CSR");
+    if (A->n == 0) return;
+    hipLaunchKernelGGL(k_grad_A, dim3(grid_for(A->n, 256)), dim3(256), 0, stream(), A->n, A->r.p, A->c.p, lam, x, dA);
+    DECO_CHECK_LAUNCH();
+    sync();
+  });
+}
+
+}  // extern "C"

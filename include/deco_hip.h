@@ -224,6 +224,12 @@ int deco_amg_level(const deco_prec* M, int level, const deco_csr** A, const deco
                    const deco_csr** R, const double** smoother_dev, const int** cf_dev);
 /* Dense FP64 inverse of the coarsest matrix (row-major nc*nc) and its size. */
 int deco_amg_coarse_inverse(const deco_prec* M, int* nc, const double** inv_dev);
+/* Host-only: one level of the AMG setup (strength, C/F splitting, extended+i interpolation and
+ * truncation) on host CSR arrays, as deco_amg_create runs it per level. cf[i] = 1 (C) / -1 (F);
+ * P (n x nc) is returned in malloc'd arrays released with deco_host_free. No device is used. */
+int deco_amg_interp_host(const deco_cpr_params* p, int n, const int* r, const int* c, const double* v,
+                         int* cf, int* nc, int* p_nnz, int** P_r, int** P_c, double** P_v);
+void deco_host_free(void* ptr);
 
 /* ---------------------------------------------------------------- solvers ------------- */
 /* SOL(param, A, precond, b, x) (solver/SOL.hpp:19-39): runs rgmres/lgmres/bicgstab/

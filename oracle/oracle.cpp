@@ -1277,10 +1277,18 @@ int orc_num_threads(void) { return omp_get_max_threads(); }
 void orc_set_threads(int t) { if (t > 0) omp_set_num_threads(t); }
 
 int orc_spmv(const orc_csr* A, const double* x, double* y, int bs) {
-  return guard([&] {
-    CSR B = from_c(A);
-    spmv(B, x, y, bs);
-  });
+  // alg::spmv (core/alg.hpp:51-64) directly on the caller's arrays
+  const int n = A->n;
+  const int* r = A->r;
+  const int* c = A->c;
+  const double* v = A->v;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; i += bs) {
+    double s = 0.0;
+    for (int j = r[i]; j < r[i + 1]; ++j) s += v[j] * x[c[j]];
+    y[i / bs] = s;
+  }
+  return 0;
 }
 double orc_dot(int n, const double* x, const double* y) { return dot(n, x, y); }
 double orc_nrm2(int n, const double* x) { return nrm2(n, x); }

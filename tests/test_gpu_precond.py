@@ -1,0 +1,164 @@
+"""GPU parity of SpGEMM, FSAI, the CPR setup and apply, and the AMG V-cycle against the CPU
+oracle (through the C ABI)."""
+import numpy as np
+import pytest
+
+import deco_amd
+import oracle
+from conftest import gen_system, to_mat, small_params
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def dev_csr(g):
+    return deco_amd.CSR(g["n"], g["n"], g["r"], g["c"], g["v"])
+
+
+def same_pattern(gpu_csr, mat):
+    r, c, v = gpu_csr.download()
+    return np.array_equal(r, mat.r) and np.array_equal(c, mat.c[:mat.nnz]), v
+
+
+def oracle_amg(G, relax):
+    levels, cinv = G.export()
+    lv = [dict(A=oracle.Mat(l["n"], l["n"], *l["A"]), P=oracle.Mat(l["n"], l["nc"], *l["P"]),
+               R=oracle.Mat(l["nc"], l["n"], *l["R"]), d=l["d"], cf=l["cf"]) for l in levels]
+    return oracle.amg(lv, relax, cinv), levels, cinv
+
+
+@pytest.mark.parametrize("cfg", [1, 4])
+def test_spgemm_and_power2_pattern_bitexact(gpu, cfg):
+    g = gen_system(cfg)
+    A = dev_csr(g)
+    M = to_mat(g)
+    C = A @ A
+    Co = oracle.spgemm(M, M)
+    ok, v = same_pattern(C, Co)
+    assert ok
+    assert np.array_equal(v, Co.v)          # same (j,k) accumulation order
+    F = A.tril_square_pattern()
+    Fo = oracle.fsai_pattern(M, 2)
+    assert same_pattern(F, Fo)[0]
+
+
+def test_spgemm_rectangular(gpu):
+    import scipy.sparse as sp
+    rng = np.random.default_rng(0)
+    S1 = sp.random(300, 200, density=0.05, random_state=rng, format="csr")
+    S2 = sp.random(200, 150, density=0.05, random_state=rng, format="csr")
+    C = deco_amd.CSR.from_scipy(S1) @ deco_amd.CSR.from_scipy(S2)
+    Co = oracle.spgemm(oracle.Mat.from_scipy(S1), oracle.Mat.from_scipy(S2))
+    ok, v = same_pattern(C, Co)
+    assert ok and np.array_equal(v, Co.v)
+
+
+@pytest.mark.parametrize("cfg,power", [(1, 1), (2, 1), (4, 2)])
+def test_fsai_factors_parity(gpu, cfg, power):
+    g = gen_system(cfg)
+    bs = g["bs"]
+    A = dev_csr(g)
+    M = to_mat(g)
+    tree = small_params(bs, **{"prec_glo.power": str(power)})
+    F = deco_amd.FSAI(tree, A)
+    Fo = oracle.fsai(deco_amd.cpr_params(tree), M)
+    FL, FUt = F.factors()
+    FLo, FUto = oracle.fsai_factors(Fo)
+    okL, vL = same_pattern(FL, FLo)
+    okU, vU = same_pattern(FUt, FUto)
+    assert okL and okU
+    # same LAPACK-order LU: equal up to the last bits
+    assert rel(vL, FLo.v) <= 1e-13 and rel(vU, FUto.v) <= 1e-13
+    x = np.random.default_rng(1).uniform(-1, 1, g["n"])
+    assert rel(F.apply(x).numpy(), Fo.apply(x)) <= 1e-13
+    assert rel(F.apply_t(x).numpy(), Fo.apply_t(x)) <= 1e-13
+
+
+@pytest.mark.parametrize("cfg", [4, 2])
+def test_cpr_setup_bitexact_and_apply_parity(gpu, cfg):
+    g = gen_system(cfg)
+    bs = g["bs"]
+    tree = small_params(bs)
+    A = dev_csr(g)
+    b = deco_amd.DeviceArray.from_numpy(g["b"])
+    M = deco_amd.CPR(tree, A, b)
+    Mo_A = to_mat(g)
+    bo = g["b"].copy()
+    Mo = oracle.CPR(deco_amd.cpr_params(tree), Mo_A, bo)
+    # decoupled system and pressure matrix (index maps) identical
+    r, c, v = A.download()
+    assert np.array_equal(r, Mo_A.r) and np.array_equal(c, Mo_A.c) and np.array_equal(v, Mo_A.v)
+    assert np.array_equal(b.numpy(), bo)
+    Ap = M.pressure_matrix()
+    Apo = Mo.pressure_matrix()
+    ok, vp = same_pattern(Ap, Apo)
+    assert ok and np.array_equal(vp, Apo.v)
+    assert np.array_equal(M.left_transform(), Mo.left_transform(bs))
+    # AMG V-cycle on the same hierarchy
+    G = M.amg()
+    Go, levels, cinv = oracle_amg(G, 18)
+    assert len(levels) >= 1
+    xp = np.random.default_rng(2).uniform(-1, 1, Ap.n)
+    assert rel(G.apply(xp).numpy(), Go.apply(xp)) <= 1e-13
+    assert rel(G.apply_t(xp).numpy(), Go.apply_t(xp)) <= 1e-12
+    # hierarchy invariants: Galerkin product, coarse inverse
+    import scipy.sparse as sp
+    for lv, nxt in zip(levels, levels[1:] + [None]):
+        P = sp.csr_matrix((lv["P"][2], lv["P"][1], lv["P"][0]), shape=(lv["n"], lv["nc"]))
+        Al = sp.csr_matrix((lv["A"][2], lv["A"][1], lv["A"][0]), shape=(lv["n"], lv["n"]))
+        RAP = (P.T @ Al @ P).toarray()
+        if nxt is not None:
+            Ac = sp.csr_matrix((nxt["A"][2], nxt["A"][1], nxt["A"][0]), shape=(nxt["n"], nxt["n"])).toarray()
+            kept = Ac != 0
+            assert np.allclose(Ac[kept], RAP[kept], rtol=1e-12, atol=1e-14 * np.abs(RAP).max())
+    # CPR apply on the same hierarchy
+    Go2, _, _ = oracle_amg(G, 18)
+    Mo.set_local(Go2)
+    x = np.random.default_rng(3).uniform(-1, 1, g["n"])
+    assert rel(M.apply(x).numpy(), Mo.apply(x)) <= 1e-12
+    assert rel(M.apply_t(x).numpy(), Mo.apply_t(x)) <= 1e-11
+    # FP32 preconditioner: same operator to single precision
+    M.set_precision(deco_amd.PREC_FP32)
+    assert rel(M.apply(x).numpy(), Mo.apply(x)) <= 2e-4
+    M.set_precision(deco_amd.PREC_FP64)
+
+
+@pytest.mark.parametrize("seq", [0, 1, 2])
+def test_cpr_sequences(gpu, seq):
+    g = gen_system(4)
+    tree = small_params(2, seq=str(seq))
+    A = dev_csr(g)
+    M = deco_amd.CPR(tree, A, None)
+    Mo_A = to_mat(g)
+    Mo = oracle.CPR(deco_amd.cpr_params(tree), Mo_A, None)
+    Go, _, _ = oracle_amg(M.amg(), 18)
+    Mo.set_local(Go)
+    x = np.random.default_rng(seq).uniform(-1, 1, g["n"])
+    assert rel(M.apply(x).numpy(), Mo.apply(x)) <= 1e-12
+
+
+@pytest.mark.parametrize("relax,coarse", [(17, 10), (18, 8), (0, 10)])
+def test_amg_variants(gpu, relax, coarse):
+    g = gen_system(1)
+    A = dev_csr(g)
+    Ap = A.local_matrix(2, 0)
+    tree = {"amg_SoC_alpha": "0.5", "amg_coarse_type": str(coarse), "amg_relax_type": str(relax),
+            "amg_prol_trunc_fac": "1e-3", "amg_A_drop_tol": "1e-4", "amg_max_coarse": "50"}
+    G = deco_amd.AMG(tree, Ap)
+    Go, levels, _ = oracle_amg(G, relax)
+    assert len(levels) >= 2
+    x = np.random.default_rng(4).uniform(-1, 1, Ap.n)
+    assert rel(G.apply(x).numpy(), Go.apply(x)) <= 1e-12
+
+
+def test_jacobi_block_parity(gpu):
+    g = gen_system(2)
+    A = dev_csr(g)
+    J = deco_amd.Jacobi({"block_size": "3"}, A)
+    Jo = oracle.jacobi(3, to_mat(g))
+    x = np.random.default_rng(5).uniform(-1, 1, g["n"])
+    assert rel(J.apply(x).numpy(), Jo.apply(x)) <= 1e-15
+    assert rel(J.apply_t(x).numpy(), Jo.apply_t(x)) <= 1e-15

@@ -1,0 +1,117 @@
+"""GPU CPR-Krylov solves against the CPU oracle on the same decoupled system and the same AMG
+hierarchy: iteration counts within +-2, solutions within 1e-6 relative at the same rtol."""
+import numpy as np
+import pytest
+
+import deco_amd
+import oracle
+from conftest import gen_system, to_mat, small_params
+from test_gpu_precond import oracle_amg, dev_csr
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def setup_pair(cfg, tree):
+    g = gen_system(cfg)
+    A = dev_csr(g)
+    b = deco_amd.DeviceArray.from_numpy(g["b"])
+    M = deco_amd.CPR(tree, A, b)
+    Ao = to_mat(g)
+    bo = g["b"].copy()
+    Mo = oracle.CPR(deco_amd.cpr_params(tree), Ao, bo)
+    Go, _, _ = oracle_amg(M.amg(), deco_amd.cpr_params(tree).amg_relax_type if
+                          deco_amd.cpr_params(tree).amg_relax_type != -1 else 18)
+    Mo.set_local(Go)
+    return g, A, b, M, Ao, bo, Mo
+
+
+@pytest.mark.parametrize("cfg,typ,m", [(1, "rgmres", 30), (4, "lgmres", 30), (2, "bicgstab", 30)])
+def test_solve_parity_fp64(gpu, cfg, typ, m):
+    g, A, b, M, Ao, bo, Mo = setup_pair(cfg, small_params(g_bs(cfg)))
+    sol = {"type": typ, "reltol": "1e-8", "restart": str(m), "itmax": "300", "verbosity": "0"}
+    S = deco_amd.SOL(sol, A, M, b)
+    xo, info = oracle.solve(deco_amd.sol_params(sol), Ao, Mo, bo)
+    assert S.flag == 0 and info.flag == 0
+    assert abs(S.iter - info.iter) <= 2
+    x = S.x.numpy()
+    assert rel(x, xo) <= 1e-6
+    assert rel(x, g["xstar"]) <= 1e-5
+
+
+def g_bs(cfg):
+    return 3 if cfg == 2 else 2
+
+
+def test_fp32_preconditioner_reaches_rtol_with_fp64_accuracy(gpu):
+    cfg = 1
+    g, A, b, M, Ao, bo, Mo = setup_pair(cfg, small_params(2))
+    sol = {"type": "rgmres", "reltol": "1e-8", "restart": "50", "itmax": "400", "verbosity": "0"}
+    ref = deco_amd.SOL(dict(sol, reltol="1e-12"), A, M, b).x.numpy()      # FP64 oracle solution
+    M.set_precision(deco_amd.PREC_FP32)
+    S = deco_amd.SOL(sol, A, M, b)
+    assert S.flag == 0 and S.res_rel < 1e-8
+    assert rel(S.x.numpy(), ref) <= 1e-6
+    M.set_precision(deco_amd.PREC_FP64)
+    S64 = deco_amd.SOL(sol, A, M, b)
+    assert S64.flag == 0 and rel(S64.x.numpy(), ref) <= 1e-6
+
+
+def test_pcg_and_left_bicgstab_match_oracle(gpu):
+    import scipy.sparse as sp
+    n = 2000
+    S = (sp.diags([-np.ones(n - 1), 2.05 * np.ones(n), -np.ones(n - 1)], [-1, 0, 1])).tocsr()
+    A = deco_amd.CSR.from_scipy(S)
+    Ao = oracle.Mat.from_scipy(S)
+    b = np.random.default_rng(0).standard_normal(n)
+    J = deco_amd.Jacobi({"block_size": "1"}, A)
+    Jo = oracle.jacobi(1, Ao)
+    for typ in ("pcg", "lbicgstab", "bicgstab"):
+        sol = {"type": typ, "reltol": "1e-10", "itmax": "1000", "verbosity": "0"}
+        Sg = deco_amd.SOL(sol, A, J, b)
+        xo, info = oracle.solve(deco_amd.sol_params(sol), Ao, Jo, b)
+        assert Sg.flag == info.flag == 0
+        assert abs(Sg.iter - info.iter) <= 2
+        assert rel(Sg.x.numpy(), xo) <= 1e-6
+
+
+def test_gmres_breakdown_and_itmax_flags(gpu):
+    import scipy.sparse as sp
+    # b an eigenvector of A: exact (happy) breakdown after one step
+    n = 100
+    S = sp.diags(np.full(n, 3.0)).tocsr()
+    A = deco_amd.CSR.from_scipy(S)
+    b = np.ones(n)
+    Sg = deco_amd.SOL({"type": "rgmres", "reltol": "1e-10", "restart": "10", "itmax": "20"}, A, None, b)
+    assert Sg.flag == 0 and np.allclose(Sg.x.numpy(), b / 3.0)
+    # itmax reached: flag 1
+    g = gen_system(4)
+    A2 = dev_csr(g)
+    Sg = deco_amd.SOL({"type": "rgmres", "reltol": "1e-14", "restart": "5", "itmax": "3"}, A2, None, g["b"])
+    assert Sg.flag == 1 and Sg.iter == 3
+
+
+def test_adjoint_transposed_solve_and_gradient(gpu):
+    cfg = 4
+    tree = small_params(2)
+    g, A, b, M, Ao, bo, Mo = setup_pair(cfg, tree)
+    sol = {"type": "rgmres", "reltol": "1e-12", "restart": "50", "itmax": "500", "verbosity": "0"}
+    w = np.random.default_rng(7).standard_normal(g["n"])
+    S = deco_amd.SOL(sol, A, M, b)
+    x = S.x.numpy()
+    St = deco_amd.SOL(sol, A, M, w, transpose=True)      # (W A)^T mu = w
+    mu = St.x.numpy()
+    yo, info = oracle.solve_transpose(deco_amd.sol_params(sol), Ao, Mo, w)
+    assert St.flag == 0 and info.flag == 0 and abs(St.iter - info.iter) <= 2
+    assert rel(mu, yo) <= 1e-8
+    lam = M.transform_transpose(mu).numpy()               # lambda = W^T mu solves A0^T lambda = w
+    A0 = to_mat(g).to_scipy()
+    assert rel(A0.T @ lam, w) <= 1e-9
+    # dL/dA on the original pattern: -lambda_i x_j ; dL/db = lambda
+    Aorig = dev_csr(g)
+    dA = deco_amd.grad_A_pattern(Aorig, lam, x).numpy()
+    rows = np.repeat(np.arange(g["n"]), np.diff(g["r"]))
+    assert np.allclose(dA, -lam[rows] * x[g["c"]], rtol=0, atol=0)
