@@ -64,14 +64,17 @@ __global__ void k_fsai_bin(int n, const int* __restrict__ rf, int* __restrict__ 
 }
 
 // ---------------------------------------------------------------- batched LU ------------
-// G lanes per row system (G in 8..64), GPB groups per 256-thread block.
+// G lanes per row system (G in 8..64), GPB groups per block; LDS per block <= 32 KiB.
 template <int G>
-__global__ __launch_bounds__(256) void k_fsai_numeric(const int* __restrict__ list, int nrows,
+constexpr int fsai_gpb() { return G == 64 ? 1 : G == 32 ? 4 : G == 16 ? 16 : 32; }
+
+template <int G>
+__global__ __launch_bounds__(G * fsai_gpb<G>()) void k_fsai_numeric(const int* __restrict__ list, int nrows,
                                                      const int* __restrict__ ra, const int* __restrict__ ca,
                                                      const double* __restrict__ va, const int* __restrict__ rf,
                                                      const int* __restrict__ cf, double* __restrict__ vl,
                                                      double* __restrict__ vu, int* __restrict__ singular) {
-  constexpr int GPB = 256 / G;
+  constexpr int GPB = fsai_gpb<G>();
   __shared__ double sM[GPB][G * G];     // column-major: M(a, b) = sM[b*G + a]
   __shared__ int sP[GPB][G];
   __shared__ int sPiv[GPB][G];
@@ -346,8 +349,8 @@ struct FSAI : deco_prec {
 
   template <int G>
   void launch(const int* list, int cnt, const deco_csr* A, int* singular) {
-    constexpr int GPB = 256 / G;
-    hipLaunchKernelGGL(k_fsai_numeric<G>, dim3(grid_for(cnt, GPB)), dim3(256), 0, stream(), list, cnt, A->r.p, A->c.p,
+    constexpr int GPB = fsai_gpb<G>();
+    hipLaunchKernelGGL(k_fsai_numeric<G>, dim3(grid_for(cnt, GPB)), dim3(G * GPB), 0, stream(), list, cnt, A->r.p, A->c.p,
                        A->v.p, (const int*)FL->r.p, (const int*)FL->c.p, FL->v.p, FU->v.p, singular);
     DECO_CHECK_LAUNCH();
   }

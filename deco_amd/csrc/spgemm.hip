@@ -153,6 +153,68 @@ __global__ __launch_bounds__(64 * WPB) void k_spgemm_rows(const int* __restrict_
   }
 }
 
+// Rows with more than 4096 candidate keys: one block per row with a dense flag/value
+// accumulator of width B->m in global memory (slot per block), filled in the order of the A
+// row's entries (same summation order as the LDS path), compacted in column order.
+template <bool TRIL, bool NUMERIC>
+__global__ __launch_bounds__(256) void k_spgemm_big(const int* __restrict__ list, int nrows, int m,
+                                                   const int* __restrict__ ra, const int* __restrict__ ca,
+                                                   const double* __restrict__ va, const int* __restrict__ rb,
+                                                   const int* __restrict__ cb, const double* __restrict__ vb,
+                                                   int* __restrict__ flags, double* __restrict__ vals,
+                                                   int* __restrict__ count, const int* __restrict__ rc,
+                                                   int* __restrict__ cc, double* __restrict__ vc) {
+  __shared__ int sm[4];
+  const int idx = blockIdx.x;
+  if (idx >= nrows) return;
+  const int i = list[idx];
+  int* fl = flags + (size_t)idx * m;
+  double* vl = vals + (size_t)idx * m;
+  for (int k = threadIdx.x; k < m; k += 256) {
+    fl[k] = 0;
+    if (NUMERIC) vl[k] = 0.0;
+  }
+  __syncthreads();
+  for (int j = ra[i]; j < ra[i + 1]; ++j) {
+    const int col = ca[j];
+    const double a = va[j];
+    for (int kk = rb[col] + threadIdx.x; kk < rb[col + 1]; kk += 256) {
+      const int key = cb[kk];
+      if (TRIL && key > i) continue;
+      fl[key] = 1;
+      if (NUMERIC) vl[key] += vb[kk] * a;
+    }
+    __syncthreads();
+  }
+  // ordered compaction over the m columns
+  int base = 0;
+  for (int t0 = 0; t0 < m; t0 += 256) {
+    const int k = t0 + threadIdx.x;
+    const int f = (k < m) ? fl[k] : 0;
+    int v = f;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int t = __shfl_up(v, off, 64);
+      if (lane >= off) v += t;
+    }
+    if (lane == 63) sm[wid] = v;
+    __syncthreads();
+    int add = 0, tot = 0;
+    for (int w = 0; w < 4; ++w) {
+      if (w < wid) add += sm[w];
+      tot += sm[w];
+    }
+    if (NUMERIC && f) {
+      const int pos = rc[i] + base + add + v - 1;
+      cc[pos] = k;
+      vc[pos] = vl[k];
+    }
+    base += tot;
+    __syncthreads();
+  }
+  if (!NUMERIC && threadIdx.x == 0) count[i] = base;
+}
+
 template <bool TRIL>
 static deco_csr* spgemm_impl(const deco_csr* A, const deco_csr* B, bool numeric) {
   if (A->m != B->n) fail(DECO_ERR_ARG, "SpGEMM shape mismatch: %dx%d times %dx%d", A->n, A->m, B->n, B->m);
@@ -172,7 +234,28 @@ static deco_csr* spgemm_impl(const deco_csr* A, const deco_csr* B, bool numeric)
   DECO_CHECK_LAUNCH();
   int hc[4];
   copy_d2h(hc, cnt.p, 4);
-  if (hc[3] > 0) fail(DECO_ERR_UNSUPPORTED, "SpGEMM row with more than 4096 candidate entries (%d rows)", hc[3]);
+  const int mB = B->m;
+  const int chunk = 64;
+  DBuf<int> bflags;
+  DBuf<double> bvals;
+  if (hc[3] > 0) {
+    const int slots = std::min(hc[3], chunk);
+    bflags.alloc((size_t)slots * mB);
+    bvals.alloc((size_t)slots * mB);
+  }
+  auto run_big = [&](bool num, const int* rc, int* cc, double* vc) {
+    for (int s0 = 0; s0 < hc[3]; s0 += chunk) {
+      const int cntb = std::min(chunk, hc[3] - s0);
+      const int* L = lists.p + 3 * (size_t)n + s0;
+      if (num)
+        hipLaunchKernelGGL((k_spgemm_big<TRIL, true>), dim3(cntb), dim3(256), 0, stream(), L, cntb, mB, A->r.p, A->c.p,
+                           A->v.p, B->r.p, B->c.p, B->v.p, bflags.p, bvals.p, counts.p, rc, cc, vc);
+      else
+        hipLaunchKernelGGL((k_spgemm_big<TRIL, false>), dim3(cntb), dim3(256), 0, stream(), L, cntb, mB, A->r.p,
+                           A->c.p, A->v.p, B->r.p, B->c.p, B->v.p, bflags.p, bvals.p, counts.p, rc, cc, vc);
+      DECO_CHECK_LAUNCH();
+    }
+  };
   auto run = [&](bool num, const int* rc, int* cc, double* vc) {
     if (hc[0])
       hipLaunchKernelGGL((num ? k_spgemm_rows<256, 4, TRIL, true> : k_spgemm_rows<256, 4, TRIL, false>),
@@ -189,13 +272,11 @@ static deco_csr* spgemm_impl(const deco_csr* A, const deco_csr* B, bool numeric)
     DECO_CHECK_LAUNCH();
   };
   run(false, nullptr, nullptr, nullptr);
+  run_big(false, nullptr, nullptr, nullptr);
   build_row_ptr(C, counts.p, n);
-  if (numeric) {
-    run(true, C->r.p, C->c.p, C->v.p);
-  } else {
-    // pattern only: keys through the numeric kernel would need B values; use zeros
-    run(true, C->r.p, C->c.p, C->v.p);
-  }
+  (void)numeric;  // the pattern-only product also writes values; the caller zeroes them
+  run(true, C->r.p, C->c.p, C->v.p);
+  run_big(true, C->r.p, C->c.p, C->v.p);
   return C;
 }
 

@@ -162,3 +162,15 @@ def test_jacobi_block_parity(gpu):
     x = np.random.default_rng(5).uniform(-1, 1, g["n"])
     assert rel(J.apply(x).numpy(), Jo.apply(x)) <= 1e-15
     assert rel(J.apply_t(x).numpy(), Jo.apply_t(x)) <= 1e-15
+
+
+def test_spgemm_long_rows(gpu):
+    import scipy.sparse as sp
+    rng = np.random.default_rng(9)
+    # rows whose candidate count exceeds the LDS tables (global accumulator path)
+    S1 = sp.random(64, 3000, density=0.5, random_state=rng, format="csr")
+    S2 = sp.random(3000, 9000, density=0.004, random_state=rng, format="csr")
+    C = deco_amd.CSR.from_scipy(S1) @ deco_amd.CSR.from_scipy(S2)
+    Co = oracle.spgemm(oracle.Mat.from_scipy(S1), oracle.Mat.from_scipy(S2))
+    ok, v = same_pattern(C, Co)
+    assert ok and np.array_equal(v, Co.v)
