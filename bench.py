@@ -1,9 +1,9 @@
 #!/usr/bin/env python3
 """Benchmark of the CPR-preconditioned Krylov solve path on one MI355X.
 
-Workload: the 168x100x120 two-unknown storage-aquifer system (4,032,000 unknowns, 56,056,320
-nonzeros, FP64 CSR), CPR (AMG pressure stage + FSAI) right-preconditioned GMRES(50) to
-rtol 1e-8. A "step" is one solve from a zero initial guess with the system and the
+Default workload: the 168x100x120 two-unknown storage-aquifer system (4,032,000 unknowns,
+56,056,320 nonzeros, FP64 CSR), CPR (AMG pressure stage + FSAI) right-preconditioned GMRES(50)
+to rtol 1e-8 (--cfg selects another generator configuration). A "step" is one solve from a zero initial guess with the system and the
 preconditioner already resident in HBM. The FP32-preconditioner solve is the headline value;
 the FP64-only solve runs beside it. The dominant kernel's roofline (the FP64 CSR SpMV with A)
 is measured with hipEvents on the library stream.
@@ -33,15 +33,26 @@ def spmv_bytes(n_rows, n_cols, nnz, sv, sx, sy):
     return nnz * (sv + 4) + 4 * (n_rows + 1) + sx * n_cols + sy * n_rows
 
 
-# CPR parameters of the bench: the reference's param_norne.json "prec" set with block size 2 and
-# the l1-Jacobi smoother (hypre relax type 18).
+# CPR parameters of the bench: the reference's own set for its two-unknown system
+# (param_sleipner.json "prec": FSAI power 1 without filtering, HMIS, default relaxation,
+# truncation 1e-3, drop 1e-4, strength 0.5, no DRS, tol_zero 1e-30).
 PREC_TREE = {
-    "block_size": "2", "pressure_id": "0", "seq": "3", "weights": "quasiimpes_GE", "use_drs": "1",
-    "drs": {"e_dd": "0.2", "e_ps": "0.0"}, "CPD": "1", "tol_zero": "0.0",
-    "prec_glo": {"type": "FSAI", "tol_rel": "0.2", "tol_abs": "1e-16", "power": "1", "block_size": "1"},
-    "prec_loc": {"tol_rel": "1e-4", "amg_coarse_type": "10", "amg_relax_type": "18", "amg_agg_levs": "0",
-                 "amg_agg_path": "1", "amg_A_drop_tol": "1e-4", "amg_prol_trunc_fac": "1e-3",
+    "block_size": "2", "pressure_id": "0", "seq": "3", "weights": "quasiimpes_GE", "use_drs": "0",
+    "drs": {"e_dd": "0.9999", "e_ps": "0.0"}, "CPD": "1", "tol_zero": "1e-30",
+    "prec_glo": {"type": "FSAI", "tol_rel": "0", "tol_abs": "0", "power": "1", "block_size": "1"},
+    "prec_loc": {"tol_rel": "0", "amg_coarse_type": "10", "amg_relax_type": "-1", "amg_agg_levs": "0",
+                 "amg_agg_path": "4", "amg_A_drop_tol": "1e-4", "amg_prol_trunc_fac": "1e-3",
                  "amg_SoC_alpha": "0.5"},
+}
+
+
+# generator configurations a bench run can select: description and Krylov method
+WORKLOADS = {
+    1: ("two-phase 32x32x16, 2 unknowns/cell", "rgmres", "30"),
+    11: ("two-phase 32x32x32, 2 unknowns/cell", "rgmres", "30"),
+    2: ("black-oil 46x112x22 (60% active, 2% fault links), 3 unknowns/cell", "bicgstab", "50"),
+    3: ("storage aquifer 168x100x120 (12 layers, sigma 2), 2 unknowns/cell", "rgmres", "50"),
+    5: ("anisotropic 64x118x263 (sigma 3, kz/kx 1e-3), 2 unknowns/cell", "rgmres", "50"),
 }
 
 
@@ -59,7 +70,7 @@ def cpu_solve_baseline(g, M, prec_tree, sol, gpu_iters, args, log):
     levels, cinv = M.amg().export()
     lv = [dict(A=oracle.Mat(l["n"], l["n"], *l["A"]), P=oracle.Mat(l["n"], l["nc"], *l["P"]),
                R=oracle.Mat(l["nc"], l["n"], *l["R"]), d=l["d"], cf=l["cf"]) for l in levels]
-    Mo.set_local(oracle.amg(lv, 18, cinv))
+    Mo.set_local(oracle.amg(lv, 0, cinv))
     setup = time.perf_counter() - t0
     k = args.cpu_iters
     sp = deco_amd.sol_params(dict(sol, itmax=str(k)))
@@ -93,6 +104,10 @@ def main():
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
         return 2
 
+    if args.cfg not in WORKLOADS:
+        print(f"bench.py: unknown --cfg {args.cfg}; choose one of {sorted(WORKLOADS)}", file=sys.stderr)
+        return 2
+    desc, method, restart = WORKLOADS[args.cfg]
     import deco_amd
     deco_amd.init(0)
     t0 = time.time()
@@ -114,8 +129,7 @@ def main():
         "metric": "CPR-GMRES time-to-1e-8 (FP32-precond vs FP64, 1 GPU); SpMV HBM GB/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "scaling": "weak", "vs_baseline": None, "data": "synthetic",
-        "config": {"workload": f"cfg{args.cfg}: 168x100x120 grid, 2 unknowns/cell, n={n}, nnz={nnz}, "
-                               "CPR(AMG+FSAI) GMRES(50) rtol 1e-8",
+        "config": {"workload": f"{desc}, n={n}, nnz={nnz}, CPR(AMG+FSAI) {method}({restart}) rtol 1e-8",
                    "n": n, "nnz": nnz, "parallelism": "single-gpu"},
         "spmv": {"fp64_us": ms64 * 1e3, "fp64_GBps": gbs64, "fp32_us": ms32 * 1e3, "fp32_GBps": gbs32,
                  "bytes_fp64": b64, "bytes_fp32": b32},
@@ -126,8 +140,9 @@ def main():
     }
 
     # ---------------------------------------------------------------- CPR-GMRES solves
-    prec_tree = deco_amd.PTree(PREC_TREE)
-    sol = {"type": "rgmres", "reltol": "1e-8", "restart": "50", "itmax": "500", "verbosity": "0"}
+    tree = dict(PREC_TREE, block_size=str(g["bs"]))
+    prec_tree = deco_amd.PTree(tree)
+    sol = {"type": method, "reltol": "1e-8", "restart": restart, "itmax": "500", "verbosity": "0"}
     b_dev = deco_amd.DeviceArray.from_numpy(g["b"])
     deco_amd.lib().deco_sync()
     t0 = time.perf_counter()
@@ -136,7 +151,7 @@ def main():
     setup_s = time.perf_counter() - t0
     levels, _ = M.amg().export() if args.export_levels else ([], None)
     log(f"CPR setup {setup_s:.2f}s")
-    x_ref = deco_amd.SOL(dict(sol, reltol="1e-12", itmax="2000"), A, M, b_dev)   # FP64 oracle solution
+    x_ref = deco_amd.SOL(dict(sol, type="rgmres", restart="50", reltol="1e-12", itmax="2000"), A, M, b_dev)
     xr = x_ref.x.numpy()
     log(f"FP64 reference solve: iter {x_ref.iter} res {x_ref.res_rel:.2e}")
 
@@ -172,21 +187,22 @@ def main():
     # ---------------------------------------------------------------- CPU baseline
     if not args.no_cpu:
         import oracle
-        M = oracle.Mat(n, n, g["r"], g["c"], g["v"])
+        Ao = oracle.Mat(n, n, g["r"], g["c"], g["v"])
         x = np.random.default_rng(0).uniform(-1, 1, n)
         threads = oracle.num_threads()
-        oracle.spmv(M, x)
+        oracle.spmv(Ao, x)
         t1 = time.perf_counter()
         for _ in range(args.cpu_sample):
-            oracle.spmv(M, x)
+            oracle.spmv(Ao, x)
         dt = (time.perf_counter() - t1) / args.cpu_sample
-        result["cpu_baseline"] = {
-            "value": b64 / dt / 1e9, "unit": "GB/s", "cores": threads, "kind": "port",
-            "sample": f"{args.cpu_sample} FP64 CSR SpMVs with A of the same system (oracle/oracle.cpp "
-                      f"alg::spmv restatement, OpenMP {threads} threads); {dt * 1e3:.1f} ms each",
-        }
+        del Ao
+        result["spmv"]["cpu_GBps"] = b64 / dt / 1e9
+        result["spmv"]["cpu_sample"] = (f"{args.cpu_sample} FP64 CSR SpMVs with A (oracle restatement, "
+                                        f"OpenMP {threads} threads); {dt * 1e3:.1f} ms each")
+        result["cpu_baseline"] = {"value": b64 / dt / 1e9, "unit": "GB/s", "cores": threads, "kind": "port",
+                                  "sample": result["spmv"]["cpu_sample"]}
         if not args.no_cpu_solve:
-            result["cpu_baseline"] = cpu_solve_baseline(g, M, PREC_TREE, sol, r64["iter"], args, log)
+            result["cpu_baseline"] = cpu_solve_baseline(g, M, tree, sol, r64["iter"], args, log)
     print(json.dumps(result), flush=True)
     return 0
 

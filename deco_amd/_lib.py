@@ -4,6 +4,7 @@ The shared library is loaded from this package directory only; there is no fallb
 missing the import of the binding raises, so nothing can silently run a CPU path.
 """
 import ctypes as C
+import importlib.util
 import os
 import re
 
@@ -40,7 +41,7 @@ class CprParams(C.Structure):
         ("amg_agg_levs", C.c_int), ("amg_agg_path", C.c_int), ("amg_A_drop_tol", C.c_double),
         ("amg_prol_trunc_fac", C.c_double), ("amg_SoC_alpha", C.c_double),
         ("amg_max_coarse", C.c_int), ("amg_max_levels", C.c_int), ("amg_P_max_elmts", C.c_int),
-        ("amg_max_row_sum", C.c_double), ("precision", C.c_int),
+        ("amg_max_row_sum", C.c_double), ("precision", C.c_int), ("amg_relax_weight", C.c_double),
     ]
 
 
@@ -125,6 +126,7 @@ SIGNATURES = {
     "deco_cpr_transform_transpose": (i, [vp, vp, vp]),
     "deco_gen_create": (i, [i, C.c_uint64, pp]),
     "deco_gen_custom": (i, [i, i, i, i, C.c_uint64, pp]),
+    "deco_gen_scaled": (i, [i, i, i, i, C.c_uint64, pp]),
     "deco_gen_info": (i, [vp, ip, ip, ip, ip]),
     "deco_gen_arrays": (i, [vp, pp, pp, pp, pp, pp]),
     "deco_gen_destroy": (i, [vp]),
@@ -148,6 +150,11 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+    # PyTorch-ROCm opens its bundled libamdhip64.so by RPATH. When torch is installed it is
+    # loaded first, so that this library binds to the same HIP runtime (same soname) instead
+    # of the system one: two runtimes in one process cannot share the device.
+    if importlib.util.find_spec("torch") is not None:
+        import torch  # noqa: F401
     lib = C.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         f = getattr(lib, name)

@@ -228,11 +228,14 @@ def axpby(a, x, b, y):
 
 
 # ---------------------------------------------------------------------------- generator --
-def generate(cfg, seed=0, custom=None):
-    """Seeded synthetic reservoir system (host arrays): dict with r, c, v, b, xstar, n, nnz, bs."""
+def generate(cfg, seed=0, custom=None, grid=None):
+    """Seeded synthetic reservoir system (host arrays): dict with r, c, v, b, xstar, n, nnz, bs.
+    custom=(nx,ny,nz,bs) uses the 32x32x16 physics; grid=(nx,ny,nz) keeps cfg's physics."""
     L = lib()
     h = C.c_void_p()
-    if custom is not None:
+    if grid is not None:
+        check(L.deco_gen_scaled(cfg, grid[0], grid[1], grid[2], seed, C.byref(h)))
+    elif custom is not None:
         nx, ny, nz, bs = custom
         check(L.deco_gen_custom(nx, ny, nz, bs, seed, C.byref(h)))
     else:
@@ -302,6 +305,7 @@ def cpr_params(prec, precision=_lib.PREC_FP64):
     p.amg_max_levels = loc.get("amg_max_levels", int, 25)
     p.amg_P_max_elmts = loc.get("amg_P_max_elmts", int, 4)
     p.amg_max_row_sum = loc.get("amg_max_row_sum", float, 0.9)
+    p.amg_relax_weight = loc.get("amg_relax_weight", float, 0.0)
     p.precision = precision
     return p
 

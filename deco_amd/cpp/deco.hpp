@@ -93,6 +93,7 @@ inline deco_cpr_params params_from(const deco_ptree& prec) {
   p.amg_max_levels = prec.get<int>("prec_loc.amg_max_levels", 25);
   p.amg_P_max_elmts = prec.get<int>("prec_loc.amg_P_max_elmts", 4);
   p.amg_max_row_sum = prec.get<double>("prec_loc.amg_max_row_sum", 0.9);
+  p.amg_relax_weight = prec.get<double>("prec_loc.amg_relax_weight", 0.0);
   p.precision = prec.get<std::string>("precision", std::string("fp64")) == "fp32" ? DECO_PREC_FP32 : DECO_PREC_FP64;
   return p;
 }

@@ -8,7 +8,8 @@
 // inverted densely in FP64 on the host and uploaded.
 //
 // Apply: one V-cycle from a zero initial guess (hypre_AMG.hpp:28-33), one sweep down and up.
-// Smoothers: relax 18 and -1 = l1-Jacobi (diagonal sum_j |a_ij|), 0/7 = Jacobi, 17 = F-C-F
+// Smoothers: relax 18 = l1-Jacobi (diagonal sum_j |a_ij|), 0/7 = Jacobi with amg_relax_weight,
+// -1 = damped Jacobi (weight 2/3) in place of hypre's hybrid Gauss-Seidel, 17 = F-C-F
 // Jacobi half-sweeps. Relaxation is fused into the stream SpMV epilogue. FP32 mode runs the
 // levels on f32 shadows and the coarsest solve in FP64.
 #include "prec.hpp"
@@ -63,9 +64,9 @@ static deco_csr* drop_small(const deco_csr* A, double tol) {
   return B;
 }
 
-// smoother diagonal: l1 (sum |a_ij|) or the diagonal entry
+// smoother diagonal: l1 (sum |a_ij|) or the diagonal entry divided by the Jacobi weight
 __global__ void k_smoother_diag(int n, const int* __restrict__ r, const int* __restrict__ c,
-                                const double* __restrict__ v, int l1, double* __restrict__ d) {
+                                const double* __restrict__ v, int l1, double weight, double* __restrict__ d) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double s = 0.0;
@@ -73,7 +74,7 @@ __global__ void k_smoother_diag(int n, const int* __restrict__ r, const int* __r
     if (l1) s += fabs(v[j]);
     else if (c[j] == i) s += v[j];
   }
-  d[i] = s;
+  d[i] = l1 ? s : s / weight;
 }
 
 // x = b/d on the rows of the subset (others 0): first half-sweep from a zero guess
@@ -131,15 +132,18 @@ struct AMG : deco_prec {
   DBuf<double> xc, bc;
   DBuf<float> xcf, bcf;
   int relax = 18;
+  double weight = 1.0;
   bool have_t = false;
 
   AMG(const deco_cpr_params& p, const deco_csr* A0) {
     kind = deco_prec::AMG;
     n = A0->n;
     relax = p.amg_relax_type;
-    if (relax == -1) relax = 18;
-    if (relax != 0 && relax != 7 && relax != 17 && relax != 18)
+    if (relax != -1 && relax != 0 && relax != 7 && relax != 17 && relax != 18)
       fail(DECO_ERR_UNSUPPORTED, "amg_relax_type %d is not supported (use -1, 0, 7, 17 or 18)", p.amg_relax_type);
+    // -1 (hypre's CPU default, hybrid Gauss-Seidel) runs as damped Jacobi with weight 2/3
+    weight = p.amg_relax_weight > 0.0 ? p.amg_relax_weight : (relax == -1 ? 2.0 / 3.0 : 1.0);
+    if (relax == -1 || relax == 7) relax = 0;
     if (p.amg_agg_levs > 0) fail(DECO_ERR_UNSUPPORTED, "aggressive coarsening (amg_agg_levs > 0) is not supported");
     if (A0->n != A0->m) fail(DECO_ERR_ARG, "AMG matrix is not square");
     const int max_coarse = p.amg_max_coarse > 0 ? p.amg_max_coarse : 300;
@@ -199,7 +203,7 @@ struct AMG : deco_prec {
       const int nl = lv.A->n;
       lv.d.alloc(nl);
       hipLaunchKernelGGL(k_smoother_diag, dim3(grid_for(nl, 256)), dim3(256), 0, stream(), nl, lv.A->r.p, lv.A->c.p,
-                         lv.A->v.p, l1, lv.d.p);
+                         lv.A->v.p, l1, relax == 17 ? 1.0 : weight, lv.d.p);
       DECO_CHECK_LAUNCH();
       lv.df.alloc(nl);
       convert(lv.d.p, lv.df.p, nl);

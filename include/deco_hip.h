@@ -87,7 +87,9 @@ typedef struct deco_cpr_params {
 
   double loc_tol_rel;       /* prec.prec_loc.tol_rel (filter_proper of Ap)      */
   int    amg_coarse_type;   /* prec.prec_loc.amg_coarse_type: 8 PMIS, 10 HMIS   */
-  int    amg_relax_type;    /* prec.prec_loc.amg_relax_type: -1, 0, 7, 17, 18   */
+  int    amg_relax_type;    /* prec.prec_loc.amg_relax_type: -1, 0, 7, 17, 18
+                               (-1: damped Jacobi, the parallel stand-in for hypre's
+                               hybrid Gauss-Seidel default)                    */
   int    amg_agg_levs;      /* prec.prec_loc.amg_agg_levs (only 0 supported)    */
   int    amg_agg_path;      /* prec.prec_loc.amg_agg_path (unused when 0 levels)*/
   double amg_A_drop_tol;    /* prec.prec_loc.amg_A_drop_tol                     */
@@ -99,6 +101,7 @@ typedef struct deco_cpr_params {
   int    amg_P_max_elmts;   /* default 4                                        */
   double amg_max_row_sum;   /* default 0.9                                      */
   int    precision;         /* DECO_PREC_FP64 / DECO_PREC_FP32                  */
+  double amg_relax_weight;  /* Jacobi weight; 0 = default (1 for relax 0/7, 2/3 for -1) */
 } deco_cpr_params;
 
 /* "sol" keys (solver/rgmres.tpp:13-16, bicgstab.tpp:16-22). */
@@ -252,6 +255,8 @@ int deco_cpr_transform_transpose(const deco_prec* M, const double* mu_dev, doubl
  * 5. seed 0 selects the default seed 1000+cfg. */
 int deco_gen_create(int cfg, uint64_t seed, deco_gen** out);
 int deco_gen_custom(int nx, int ny, int nz, int bs, uint64_t seed, deco_gen** out);
+/* The physics of configuration cfg on an nx x ny x nz grid (tests at reduced size). */
+int deco_gen_scaled(int cfg, int nx, int ny, int nz, uint64_t seed, deco_gen** out);
 int deco_gen_info(const deco_gen* g, int* n, int* nnz, int* bs, int* ncells);
 /* Borrowed host pointers valid until deco_gen_destroy. */
 int deco_gen_arrays(const deco_gen* g, const int** r, const int** c, const double** v,

@@ -93,10 +93,11 @@ class Mat:
     """Host CSR owned by numpy arrays, passed to the oracle as an orc_csr view."""
 
     def __init__(self, n, m, r, c, v):
+        # own copies: in-place oracle calls (CPR decoupling, filters) must not touch the caller's data
         self.n, self.m = int(n), int(m)
-        self.r = np.ascontiguousarray(r, np.int32)
-        self.c = np.ascontiguousarray(c, np.int32)
-        self.v = np.ascontiguousarray(v, np.float64)
+        self.r = np.array(r, np.int32, copy=True)
+        self.c = np.array(c, np.int32, copy=True)
+        self.v = np.array(v, np.float64, copy=True)
 
     @classmethod
     def from_scipy(cls, S):
