@@ -22,6 +22,7 @@ W_NONE, W_QUASIIMPES, W_QUASIIMPES_GE = 0, 1, 2
 GLO_FSAI, GLO_JACOBI = 0, 1
 PREC_FP64, PREC_FP32 = 0, 1
 SOL_TYPES = {"rgmres": 0, "lgmres": 1, "bicgstab": 2, "lbicgstab": 3, "pcg": 4}
+ORTHO = {"dcgs2": 0, "mgs": 1}
 
 
 class DecoError(RuntimeError):
@@ -47,7 +48,7 @@ class CprParams(C.Structure):
 
 class SolParams(C.Structure):
     _fields_ = [("type", C.c_int), ("reltol", C.c_double), ("restart", C.c_int), ("itmax", C.c_int),
-                ("verbosity", C.c_int), ("use_x0", C.c_int)]
+                ("verbosity", C.c_int), ("use_x0", C.c_int), ("ortho", C.c_int)]
 
 
 class SolInfo(C.Structure):

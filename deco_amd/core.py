@@ -323,6 +323,10 @@ def sol_params(sol):
     s.itmax = sol.get("itmax", int)
     s.verbosity = sol.get("verbosity", int, 0)
     s.use_x0 = sol.get("use_x0", int, 0)
+    ortho = sol.get("ortho", str, "dcgs2")
+    if ortho not in _lib.ORTHO:
+        raise ValueError("Invalid Gram-Schmidt variant: " + ortho)
+    s.ortho = _lib.ORTHO[ortho]
     return s
 
 

@@ -373,6 +373,10 @@ class SOL {
     sp.itmax = param.template get<int>("itmax");
     sp.verbosity = param.template get<int>("verbosity", 0);
     sp.use_x0 = x ? 1 : 0;
+    const std::string ortho = param.template get<std::string>("ortho", "dcgs2");
+    if (ortho == "dcgs2") sp.ortho = DECO_ORTHO_DCGS2;
+    else if (ortho == "mgs") sp.ortho = DECO_ORTHO_MGS;
+    else throw std::invalid_argument("Invalid Gram-Schmidt variant: " + ortho);
     deco_prec* M = nullptr;
     if (precond) {
       auto* dp = precond.template target<deco_facade::DevicePrecond<V>>();

@@ -103,14 +103,20 @@ struct EpiRelax {
   }
 };
 
-// coarse solve x = Cinv b with CT[k*nc + i] = Cinv(i,k); input/output in T, FP64 arithmetic
+// coarse solve x = C b, C row-major with leading dimension nc (the inverse for the forward
+// cycle, its transpose for the transposed one): one wavefront per row, lanes stride the
+// columns, fixed-order butterfly sum; input/output in T, FP64 arithmetic
 template <class T>
-__global__ void k_coarse(int nc, const double* __restrict__ CT, const T* __restrict__ b, T* __restrict__ x) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nc) return;
+__global__ __launch_bounds__(256) void k_coarse(int nc, const double* __restrict__ C, const T* __restrict__ b,
+                                                T* __restrict__ x) {
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= nc) return;
+  const double* cr = C + (size_t)row * nc;
   double s = 0.0;
-  for (int k = 0; k < nc; ++k) s += CT[(size_t)k * nc + i] * (double)b[k];
-  x[i] = (T)s;
+  for (int k = lane; k < nc; k += 64) s += cr[k] * (double)b[k];
+  s = wave_sum(s);
+  if (lane == 0) x[row] = (T)s;
 }
 
 struct AMG : deco_prec {
@@ -128,7 +134,7 @@ struct AMG : deco_prec {
   std::vector<Level> L;       // levels 0..nl-1 with interpolation; L.back() may be the coarsest
   deco_csr* Ac = nullptr;     // coarsest matrix
   int nc = 0;
-  DBuf<double> CT, Cinv;      // coarse inverse: transposed (forward) and row-major (transposed)
+  DBuf<double> CT, Cinv;      // coarse inverse: row-major (forward) and its transpose (transposed cycle)
   DBuf<double> xc, bc;
   DBuf<float> xcf, bcf;
   int relax = 18;
@@ -263,8 +269,8 @@ struct AMG : deco_prec {
   void cycle(size_t l, const T* b, T* x, bool tr) {
     if (l == L.size()) {
       if (nc == 0) return;
-      hipLaunchKernelGGL(k_coarse<T>, dim3(grid_for(nc, 64)), dim3(64), 0, stream(), nc,
-                         (const double*)(tr ? Cinv.p : CT.p), b, x);
+      hipLaunchKernelGGL(k_coarse<T>, dim3((nc + 3) / 4), dim3(256), 0, stream(), nc,
+                         (const double*)(tr ? CT.p : Cinv.p), b, x);
       DECO_CHECK_LAUNCH();
       return;
     }
@@ -311,8 +317,8 @@ struct AMG : deco_prec {
   void run(const T* r, T* x, bool tr) {
     if (tr) ensure_t();
     if (L.empty()) {
-      hipLaunchKernelGGL(k_coarse<T>, dim3(grid_for(nc, 64)), dim3(64), 0, stream(), nc,
-                         (const double*)(tr ? Cinv.p : CT.p), r, x);
+      hipLaunchKernelGGL(k_coarse<T>, dim3((nc + 3) / 4), dim3(256), 0, stream(), nc,
+                         (const double*)(tr ? CT.p : Cinv.p), r, x);
       DECO_CHECK_LAUNCH();
       return;
     }

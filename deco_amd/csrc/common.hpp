@@ -135,6 +135,15 @@ __device__ __forceinline__ T group_sum(T v) {
 template <class T>
 __device__ __forceinline__ T wave_sum(T v) { return group_sum<64>(v); }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup i runs on XCD i % 8), each
+// with its own L2. xcd_block maps the hardware index to a logical one so that every XCD gets a
+// contiguous range of logical blocks (a bijection on [0, G)): neighbouring row chunks, and the
+// x entries they gather, then share one L2.
+__device__ __forceinline__ int xcd_block(int i, int G) {
+  const int q = G >> 3, r = G & 7, x = i & 7, k = i >> 3;
+  return (x < r) ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
 // Block-wide sum of one value per thread, fixed order (deterministic). BLOCK multiple of 64.
 template <int BLOCK, class T>
 __device__ __forceinline__ T block_sum(T v, T* smem /* BLOCK/64 entries */) {

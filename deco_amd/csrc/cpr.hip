@@ -380,6 +380,7 @@ struct CPR : deco_prec {
   deco_cpr_params p{};
   deco_csr* A = nullptr;                 // caller's matrix, decoupled in place
   deco_csr* Ap = nullptr;
+  deco_csr* Apr = nullptr;               // pressure rows (bs*i) of the decoupled A, copied at setup
   deco_prec* glo = nullptr;
   deco_prec* loc = nullptr;
   deco_prec* jac = nullptr;
@@ -408,9 +409,9 @@ struct CPR : deco_prec {
   }
 
   void release() {
-    delete glo; delete loc; delete jac; delete Ap;
+    delete glo; delete loc; delete jac; delete Ap; delete Apr;
     glo = loc = jac = nullptr;
-    Ap = nullptr;
+    Ap = Apr = nullptr;
   }
   ~CPR() override { release(); }
 
@@ -486,6 +487,9 @@ struct CPR : deco_prec {
                        (const unsigned char*)flag.p);
     DECO_CHECK_LAUNCH();
     A->invalidate();
+    // the strided pressure residual of CPR.hpp:241-246 runs as a contiguous stream SpMV over
+    // a copy of the pressure rows (same entries, same per-row summation order)
+    if (p.seq != 1) Apr = csr_row_subset(A, bs, 0);
     rt.alloc(n); xp.alloc(sn); tmp.alloc(n); tmp2.alloc(n); x2.alloc(n);
     x32.alloc(n); rt32.alloc(n); xp32.alloc(sn); t32.alloc(n);
     sync();
@@ -507,9 +511,11 @@ struct CPR : deco_prec {
   }
 
   // strided pressure residual rt[i] = r[bs i] - (A x)[bs i]   (CPR.hpp:241-246)
-  template <class VT, class XT, class YT>
-  void pressure_residual(const VT* v, const XT* x, const double* r, YT* out) {
-    launch_spmv<EPI_RES, VT, XT, YT, double>(csr_group(A), sn, bs, A->r.p, A->c.p, v, x, out, r, bs, 0);
+  void pressure_residual(const double* x, const double* r, double* out) {
+    spmv_stream_f(Apr, Apr->v.p, x, EpiStrideRes<double, double>{out, r, bs});
+  }
+  void pressure_residual(const float* x, const double* r, float* out) {
+    spmv_stream_f(Apr, Apr->values_f32(), x, EpiStrideRes<float, double>{out, r, bs});
   }
 
   void apply(const double* r, double* x) override {
@@ -529,7 +535,7 @@ struct CPR : deco_prec {
       return;
     }
     if (seq == 2) jac->apply(r, x); else glo_apply(r, x);
-    pressure_residual(A->v.p, x, r, rt.p);
+    pressure_residual((const double*)x, r, rt.p);
     loc->apply(rt.p, xp.p);
     hipLaunchKernelGGL((k_add_pressure<double, double>), dim3(gs()), dim3(256), 0, stream(), sn, bs, x,
                        (const double*)xp.p);
@@ -567,7 +573,7 @@ struct CPR : deco_prec {
     }
     if (seq == 2) { convert(r, t32.p, n); jac->apply32(t32.p, x32.p); }
     else glo32_from64(r, x32.p);
-    pressure_residual(A32, x32.p, r, rt32.p);
+    pressure_residual((const float*)x32.p, r, rt32.p);
     loc->apply32(rt32.p, xp32.p);
     hipLaunchKernelGGL((k_add_pressure<float, float>), dim3(gs()), dim3(256), 0, stream(), sn, bs, x32.p,
                        (const float*)xp32.p);

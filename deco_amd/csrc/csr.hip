@@ -437,6 +437,45 @@ deco_csr* local_matrix(const deco_csr* A, int bs, int p, const double* w) {
   return L;
 }
 
+// ------------------------------------------------------------------ row subset --------
+__global__ void k_subset_count(int ns, int stride, int off, const int* __restrict__ r, int* __restrict__ cnt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ns) cnt[i] = r[i * stride + off + 1] - r[i * stride + off];
+}
+// one wavefront per subset row copies its entries
+__global__ void k_subset_fill(int ns, int stride, int off, const int* __restrict__ r, const int* __restrict__ c,
+                              const double* __restrict__ v, const int* __restrict__ rs, int* __restrict__ cs,
+                              double* __restrict__ vs) {
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= ns) return;
+  const int b = r[i * stride + off], e = r[i * stride + off + 1], o = rs[i];
+  for (int k = b + lane; k < e; k += 64) {
+    cs[o + k - b] = c[k];
+    vs[o + k - b] = v[k];
+  }
+}
+
+deco_csr* csr_row_subset(const deco_csr* A, int stride, int off) {
+  if (stride <= 0 || off < 0 || off >= stride || A->n % stride != 0)
+    fail(DECO_ERR_ARG, "row subset stride %d offset %d does not fit n=%d", stride, off, A->n);
+  const int ns = A->n / stride;
+  deco_csr* S = csr_new(ns, A->m);
+  DBuf<int> cnt(ns + 1);
+  if (ns > 0) {
+    hipLaunchKernelGGL(k_subset_count, dim3(grid_for(ns, 256)), dim3(256), 0, stream(), ns, stride, off, A->r.p,
+                       cnt.p);
+    DECO_CHECK_LAUNCH();
+  }
+  build_row_ptr(S, cnt.p, ns);
+  if (ns > 0) {
+    hipLaunchKernelGGL(k_subset_fill, dim3((ns + 3) / 4), dim3(256), 0, stream(), ns, stride, off, A->r.p, A->c.p,
+                       A->v.p, S->r.p, S->c.p, S->v.p);
+    DECO_CHECK_LAUNCH();
+  }
+  return S;
+}
+
 void csr_filter(deco_csr* A, double tol) { filter_impl<0>(A, tol); }
 void csr_filter_proper(deco_csr* A, double tol) { filter_impl<1>(A, tol); }
 

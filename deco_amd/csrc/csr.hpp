@@ -103,9 +103,9 @@ void launch_spmv(int G, int nout, int rs, const int* r, const int* c, const VT* 
 // flight per wave; the products are then summed per row in column order (the same order as
 // alg::spmv, core/alg.hpp:59-61). Chunks whose range exceeds the LDS tile (very long rows)
 // fall back to a block reduction per row.
-constexpr int kStreamCap = 2048;                 // target entries per chunk
-constexpr int kStreamLds = 2 * kStreamCap;       // LDS tile (entries), room for the last row
-constexpr int kStreamBlock = 256;
+constexpr int kStreamCap = 3072;                 // target entries per chunk
+constexpr int kStreamLds = kStreamCap + 128;     // LDS tile: a chunk holds <= cap + longest row - 1
+constexpr int kStreamBlock = 512;
 constexpr int kStreamUnroll = kStreamCap / kStreamBlock;
 
 // Epilogue functors: called once per row with the row sum (accumulator type AT).
@@ -122,13 +122,24 @@ struct EpiStd {
   }
 };
 
+// y[o] = z[o*zs] - sum: residual of a row subset (rows zs*o of the full matrix)
+template <class YT, class ZT>
+struct EpiStrideRes {
+  YT* __restrict__ y;
+  const ZT* __restrict__ z;
+  int zs;
+  template <class AT>
+  __device__ __forceinline__ void operator()(int o, AT acc) const { y[o] = (YT)((AT)z[(size_t)o * zs] - acc); }
+};
+
 template <class VT, class XT, class E>
 __global__ __launch_bounds__(kStreamBlock) void k_spmv_stream(const int* __restrict__ rb, const int* __restrict__ r,
                                                              const int* __restrict__ c, const VT* __restrict__ v,
                                                              const XT* __restrict__ x, E epi) {
   using AT = acc_t<VT, XT>;
   __shared__ AT prod[kStreamLds];
-  const int r0 = rb[blockIdx.x], r1 = rb[blockIdx.x + 1];
+  const int ch = xcd_block(blockIdx.x, gridDim.x);
+  const int r0 = rb[ch], r1 = rb[ch + 1];
   if (r0 >= r1) return;
   const int j0 = r[r0], j1 = r[r1];
   const int cnt = j1 - j0;
@@ -217,6 +228,8 @@ void sort_rows(int n, const int* r, int* c, double* v, int max_row);
 int max_row_length(const deco_csr* A);
 
 deco_csr* csr_transpose(const deco_csr* A);
+// rows stride*i + off of A (all columns), as an (n/stride) x m CSR
+deco_csr* csr_row_subset(const deco_csr* A, int stride, int off);
 deco_csr* csr_clone(const deco_csr* A);
 deco_csr* csr_new(int n, int m);
 

@@ -74,6 +74,120 @@ __global__ void k_combine(int n, int j, const double* __restrict__ V, int64_t ld
   w[k] = s;
 }
 
+// ---------------- one-reduction Arnoldi step (delayed classical Gram-Schmidt, DCGS2) -------
+// Each block owns kArnTile consecutive rows (kArnRpt per thread, strided by the block size so
+// every load instruction is coalesced). Values of one step, in this order:
+//   s_i = q_i.u (i < j), alpha = u.u, z_i = q_i.w (i < j), beta = u.w   (the last two only if w)
+// part[v*nb + block] = fixed-order block partial of value v.
+constexpr int kArnRpt = 8;
+constexpr int kArnBlock = 256;
+constexpr int kArnTile = kArnRpt * kArnBlock;
+constexpr int kDcgs2MaxRestart = 1000;   // LDS of the dots kernel: (2 restart + 2) * 4 doubles
+
+__global__ __launch_bounds__(kArnBlock) void k_arn_dots(int n, int j, const double* __restrict__ Q,
+                                                       const double* __restrict__ u, const double* __restrict__ w,
+                                                       double* __restrict__ part, int nb) {
+  extern __shared__ double red[];  // [value][wave]
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int64_t base = (int64_t)blockIdx.x * kArnTile + tid;
+  double uu[kArnRpt], ww[kArnRpt];
+#pragma unroll
+  for (int k = 0; k < kArnRpt; ++k) {
+    const int64_t idx = base + k * kArnBlock;
+    uu[k] = idx < n ? u[idx] : 0.0;
+    ww[k] = (w && idx < n) ? w[idx] : 0.0;
+  }
+  const bool hw = (w != nullptr);
+  for (int i = 0; i < j; ++i) {
+    const double* q = Q + (size_t)i * n;
+    double qq[kArnRpt];
+#pragma unroll
+    for (int k = 0; k < kArnRpt; ++k) {
+      const int64_t idx = base + k * kArnBlock;
+      qq[k] = idx < n ? __builtin_nontemporal_load(q + idx) : 0.0;
+    }
+    double ps = 0.0, pz = 0.0;
+#pragma unroll
+    for (int k = 0; k < kArnRpt; ++k) { ps += qq[k] * uu[k]; pz += qq[k] * ww[k]; }
+    ps = wave_sum(ps);
+    if (hw) pz = wave_sum(pz);
+    if (lane == 0) {
+      red[i * 4 + wid] = ps;
+      if (hw) red[(j + 1 + i) * 4 + wid] = pz;
+    }
+  }
+  double pa = 0.0, pb = 0.0;
+#pragma unroll
+  for (int k = 0; k < kArnRpt; ++k) { pa += uu[k] * uu[k]; pb += uu[k] * ww[k]; }
+  pa = wave_sum(pa);
+  if (hw) pb = wave_sum(pb);
+  if (lane == 0) {
+    red[j * 4 + wid] = pa;
+    if (hw) red[(2 * j + 1) * 4 + wid] = pb;
+  }
+  __syncthreads();
+  const int nv = hw ? 2 * j + 2 : j + 1;
+  for (int v = tid; v < nv; v += kArnBlock)
+    part[(size_t)v * nb + blockIdx.x] = ((red[v * 4] + red[v * 4 + 1]) + red[v * 4 + 2]) + red[v * 4 + 3];
+}
+
+// out[v] = sum over the nb block partials of value v (one wavefront per value, fixed order)
+__global__ __launch_bounds__(256) void k_arn_reduce(int nv, int nb, const double* __restrict__ part,
+                                                    double* __restrict__ out) {
+  const int v = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (v >= nv) return;
+  double s = 0.0;
+  for (int b = lane; b < nb; b += 64) s += part[(size_t)v * nb + b];
+  s = wave_sum(s);
+  if (lane == 0) out[v] = s;
+}
+
+// The step's scalars from the reduced values (same operation order on host and device):
+// rho = sqrt(alpha - s.s) (norm of the reorthogonalised u), gamma = (beta - s.z) / rho^2.
+__host__ __device__ inline void arn_scalars(int j, const double* red, double& rho, double& gam) {
+  double ss = 0.0, sz = 0.0;
+  for (int i = 0; i < j; ++i) { ss += red[i] * red[i]; sz += red[i] * red[j + 1 + i]; }
+  const double d = red[j] - ss;
+  rho = sqrt(d > 0.0 ? d : 0.0);
+  gam = (red[2 * j + 1] - sz) / (rho * rho);
+}
+
+// v_j = (u - Q s) / rho  and  u_next = (w - Q (z - gamma s) - gamma u) / rho, one sweep over Q
+__global__ __launch_bounds__(kArnBlock) void k_arn_update(int n, int j, const double* __restrict__ Q,
+                                                         const double* __restrict__ u, const double* __restrict__ w,
+                                                         const double* __restrict__ red, double* __restrict__ vout,
+                                                         double* __restrict__ unext) {
+  double rho, gam;
+  arn_scalars(j, red, rho, gam);
+  const double irho = 1.0 / rho;
+  const int64_t base = (int64_t)blockIdx.x * kArnTile + threadIdx.x;
+  double a1[kArnRpt], a2[kArnRpt];
+#pragma unroll
+  for (int k = 0; k < kArnRpt; ++k) { a1[k] = 0.0; a2[k] = 0.0; }
+  for (int i = 0; i < j; ++i) {
+    const double* q = Q + (size_t)i * n;
+    const double si = red[i], ci = red[j + 1 + i] - gam * red[i];
+#pragma unroll
+    for (int k = 0; k < kArnRpt; ++k) {
+      const int64_t idx = base + k * kArnBlock;
+      if (idx < n) {
+        const double qv = __builtin_nontemporal_load(q + idx);
+        a1[k] += qv * si;
+        a2[k] += qv * ci;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kArnRpt; ++k) {
+    const int64_t idx = base + k * kArnBlock;
+    if (idx < n) {
+      const double uv = u[idx];
+      vout[idx] = (uv - a1[k]) * irho;
+      unext[idx] = ((w[idx] - a2[k]) - gam * uv) * irho;
+    }
+  }
+}
+
 // partial sums of up to three products over the grid: part[s*kRedMaxGrid + block]
 template <int K>
 __global__ __launch_bounds__(kRedBlock) void k_dots(int n, const double* __restrict__ a0, const double* __restrict__ b0,
@@ -219,7 +333,13 @@ static void gmres(bool left, const deco_sol_params& sp, const AOp& A, const MOp&
   const double tol = sp.reltol;
   const int m = sp.restart, itmax = sp.itmax;
   if (m <= 0) fail(DECO_ERR_ARG, "restart must be positive");
+  // An FP32 preconditioner is not exactly linear (rounding), so M^{-1}(V y) differs from
+  // sum_j M^{-1}(v_j) y_j by O(eps_f32) and the Arnoldi residual estimate stops describing the
+  // update. For that case the right-preconditioned solve keeps Z = [M^{-1} v_j] (flexible GMRES)
+  // and updates x += Z y; the FP64 preconditioner keeps the form of rgmres.tpp:106-118.
+  const bool flex = !left && M.M && M.M->precision == DECO_PREC_FP32;
   DBuf<double> r(n), z(n), w(n), V((size_t)n * (m + 1)), gdev(m + 1), hcol(m + 2);
+  DBuf<double> Z(flex ? (size_t)n * m : 0);
   std::vector<double> H((size_t)(m + 1) * (m + 3), 0.0);
   double* g = H.data() + (size_t)(m + 1) * m;
   double* cs = g + (m + 1);
@@ -249,8 +369,11 @@ static void gmres(bool left, const deco_sol_params& sp, const AOp& A, const MOp&
       rnorm = std::fabs(g[j]);
       if (rnorm < absTol) { flag = 0; break; }
       double* vj = V.p + (size_t)j * n;
-      if (!left) { M.apply(vj, z.p); A.apply(z.p, w.p); }
-      else { A.apply(vj, z.p); M.apply(z.p, w.p); }
+      if (!left) {
+        double* zj = flex ? Z.p + (size_t)j * n : z.p;
+        M.apply(vj, zj);
+        A.apply(zj, w.p);
+      } else { A.apply(vj, z.p); M.apply(z.p, w.p); }
       ++napp; ++nspmv;
       // fused MGS: j+2 kernels, partial slots ping-pong
       double* slot[2] = {S.rs.slot(0), S.rs.slot(1)};
@@ -283,10 +406,15 @@ static void gmres(bool left, const deco_sol_params& sp, const AOp& A, const MOp&
       for (int k = 0; k < i; ++k) g[k] -= Hm(k, i) * g[i];
     }
     copy_h2d(gdev.p, g, std::max(j, 1));
-    hipLaunchKernelGGL(k_combine, dim3(S.g256()), dim3(256), 0, stream(), n, j, (const double*)V.p, (int64_t)n,
-                       (const double*)gdev.p, w.p);
-    DECO_CHECK_LAUNCH();
-    if (!left) {
+    if (!flex || j > 0) {
+      hipLaunchKernelGGL(k_combine, dim3(S.g256()), dim3(256), 0, stream(), n, j, (const double*)(flex ? Z.p : V.p),
+                         (int64_t)n, (const double*)gdev.p, w.p);
+      DECO_CHECK_LAUNCH();
+    }
+    if (flex) {
+      if (j > 0) S.add(w.p, x);
+      A.residual(x, b, r.p); ++nspmv;
+    } else if (!left) {
       M.apply(w.p, z.p); ++napp;
       S.add(z.p, x);
       A.residual(x, b, r.p); ++nspmv;
@@ -297,6 +425,163 @@ static void gmres(bool left, const deco_sol_params& sp, const AOp& A, const MOp&
     }
     rnorm = S.nrm2(r.p);
     if (rnorm < absTol) { flag = 0; break; }
+  }
+  info.iter = iter;
+  info.res_abs = rnorm;
+  info.res_rel = rnorm / bnorm;
+  info.flag = flag;
+  info.bnorm = bnorm;
+  info.n_prec = napp;
+  info.n_spmv = nspmv;
+}
+
+// rgmres / lgmres with the one-reduction delayed-reorthogonalisation Arnoldi (DCGS2:
+// Swirydowicz, Langou, Ananthan, Yang, Thomas, "Low synchronization Gram-Schmidt and GMRES
+// algorithms", 2020; Bielich et al., Parallel Computing 2022). Step j applies the operator to
+// u_j, the once-orthogonalised (unnormalised) candidate for v_j, and one sweep computes
+// s = Q^T u, |u|^2, z = Q^T w, u.w. Then u_j = Q s + rho v_j (rho by Pythagoras), which
+// finalises Hessenberg column j-1 (H(:,j-1) += s, H(j,j-1) = rho), and a second sweep writes
+// v_j and the next candidate u_{j+1} = B v_j - Q_{j+1} c. Reading the basis twice per step
+// instead of 2(j+1) times. The residual estimate of column j-1 is known at step j, so a solve
+// applies the operator once more than it finalises columns. In exact arithmetic the iterates
+// are those of solver/rgmres.tpp and lgmres.tpp (same Krylov space, same least-squares problem,
+// the same Givens arithmetic, rgmres.tpp:80-104,141-160).
+// With an FP32 preconditioner (right variant) Z~ = [M^{-1} u_j] is stored; the directions that
+// span the Arnoldi relation are Z = Z~ T^{-1} with T(:,j) = [s; rho], so x += Z~ (T^{-1} y).
+static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, const MOp& M, const double* b,
+                        double* x, deco_sol_info& info) {
+  const int n = A.transpose ? A.A->m : A.A->n;
+  Solver S(n);
+  const double tol = sp.reltol;
+  const int m = sp.restart, itmax = sp.itmax;
+  if (m <= 0) fail(DECO_ERR_ARG, "restart must be positive");
+  const bool flex = !left && M.M && M.M->precision == DECO_PREC_FP32;
+  const int nbk = (int)std::max<int64_t>(1, ((int64_t)n + kArnTile - 1) / kArnTile);
+  DBuf<double> r(n), z(n), w(n), ua(n), ub(n), V((size_t)n * (m + 1)), gdev(m + 1);
+  DBuf<double> part((size_t)(2 * m + 2) * nbk), redd(2 * m + 2);
+  DBuf<double> Z(flex ? (size_t)n * m : 0);
+  const size_t ld = (size_t)m + 1;
+  std::vector<double> Hr(ld * m), Rm(ld * m), T(flex ? ld * m : 0), g(m + 1), cs(m), sn(m), hs(m + 1);
+  double* red = ctx().pinned;
+  int flag = 1, iter = 0, napp = 0, nspmv = 0;
+  double bnorm, absTol, rnorm = 0.0;
+  if (!left) {
+    bnorm = S.nrm2(b);
+    absTol = tol * bnorm;
+    if (!sp.use_x0) { DECO_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, stream())); copy_d2d(r.p, b, n); }
+    else { A.residual(x, b, r.p); ++nspmv; }
+  } else {
+    if (!sp.use_x0) { DECO_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, stream())); M.apply(b, r.p); ++napp; }
+    else { A.residual(x, b, z.p); ++nspmv; M.apply(z.p, r.p); ++napp; }
+    bnorm = S.nrm2(r.p);
+    absTol = tol * bnorm;
+  }
+  while (iter < itmax) {
+    std::fill(Hr.begin(), Hr.end(), 0.0);
+    std::fill(Rm.begin(), Rm.end(), 0.0);
+    const int it0 = iter;
+    double* bufs[2] = {ua.p, ub.p};
+    double* u = r.p;
+    int j = 0, k = 0;
+    while (true) {
+      const bool last = (j == m) || (it0 + j >= itmax);
+      const double* wp = nullptr;
+      if (!last) {
+        if (!left) {
+          double* zj = flex ? Z.p + (size_t)j * n : z.p;
+          M.apply(u, zj);
+          A.apply(zj, w.p);
+        } else { A.apply(u, z.p); M.apply(z.p, w.p); }
+        ++napp; ++nspmv;
+        wp = w.p;
+      }
+      const int nv = wp ? 2 * j + 2 : j + 1;
+      hipLaunchKernelGGL(k_arn_dots, dim3(nbk), dim3(kArnBlock), sizeof(double) * 4 * (2 * j + 2), stream(), n, j,
+                         (const double*)V.p, (const double*)u, wp, part.p, nbk);
+      DECO_CHECK_LAUNCH();
+      hipLaunchKernelGGL(k_arn_reduce, dim3((nv + 3) / 4), dim3(256), 0, stream(), nv, nbk, (const double*)part.p,
+                         redd.p);
+      DECO_CHECK_LAUNCH();
+      double* un = bufs[j & 1];
+      if (wp) {   // queued before the host decision; unused if the cycle ends here
+        hipLaunchKernelGGL(k_arn_update, dim3(nbk), dim3(kArnBlock), 0, stream(), n, j, (const double*)V.p,
+                           (const double*)u, wp, (const double*)redd.p, V.p + (size_t)j * n, un);
+        DECO_CHECK_LAUNCH();
+      }
+      DECO_HIP(hipMemcpyAsync(red, redd.p, sizeof(double) * nv, hipMemcpyDeviceToHost, stream()));
+      sync();
+      double rho, gam = 0.0;
+      if (wp) arn_scalars(j, red, rho, gam);
+      else {
+        double ss = 0.0;
+        for (int i = 0; i < j; ++i) ss += red[i] * red[i];
+        const double d = red[j] - ss;
+        rho = std::sqrt(d > 0.0 ? d : 0.0);
+      }
+      if (j == 0) {
+        g[0] = rho;
+        rnorm = std::fabs(g[0]);
+        if (rnorm < absTol) { flag = 0; break; }
+        if (last || !(rho > 0.0)) break;
+      } else {
+        // finalise column j-1 and rotate it (rgmres.tpp:80-93)
+        for (int i = 0; i < j; ++i) Hr[i + (size_t)(j - 1) * ld] += red[i];
+        Hr[j + (size_t)(j - 1) * ld] = rho;
+        double* col = &Rm[(size_t)(j - 1) * ld];
+        for (int i = 0; i <= j; ++i) col[i] = Hr[i + (size_t)(j - 1) * ld];
+        for (int i = 0; i <= j - 2; ++i) apply_givens(cs[i], sn[i], col[i], col[i + 1]);
+        comp_givens(col[j - 1], col[j], cs[j - 1], sn[j - 1]);
+        apply_givens(cs[j - 1], sn[j - 1], col[j - 1], col[j]);
+        g[j] = 0.0;
+        apply_givens(cs[j - 1], sn[j - 1], g[j - 1], g[j]);
+        ++iter;
+        k = j;
+        rnorm = std::fabs(g[j]);
+        if (rnorm < absTol) { flag = 0; break; }
+        if (last || !(rho > 0.0)) break;
+      }
+      // u_j = Q s + rho v_j: column j of T, provisional Hessenberg column j
+      if (flex) {
+        for (int i = 0; i < j; ++i) T[i + (size_t)j * ld] = red[i];
+        T[j + (size_t)j * ld] = rho;
+      }
+      for (int i = 0; i <= j; ++i) {
+        double t = 0.0;
+        for (int q = 0; q < j; ++q) t += Hr[i + (size_t)q * ld] * red[q];
+        hs[i] = t;
+      }
+      double sz = 0.0;
+      for (int i = 0; i < j; ++i) sz += red[i] * red[j + 1 + i];
+      for (int i = 0; i < j; ++i) Hr[i + (size_t)j * ld] = (red[j + 1 + i] - hs[i]) / rho;
+      Hr[j + (size_t)j * ld] = ((red[2 * j + 1] - sz) / rho - hs[j]) / rho;
+      u = un;
+      ++j;
+    }
+    if (k > 0) {
+      for (int i = k - 1; i >= 0; --i) {                      // rgmres.tpp:95-104
+        const double d = Rm[i + (size_t)i * ld];
+        if (d == 0.0) fail(DECO_ERR_BREAKDOWN, "Zero diagonal element in upper triangular matrix");
+        g[i] /= d;
+        for (int q = 0; q < i; ++q) g[q] -= Rm[q + (size_t)i * ld] * g[i];
+      }
+      if (flex) {
+        for (int i = k - 1; i >= 0; --i) {
+          g[i] /= T[i + (size_t)i * ld];
+          for (int q = 0; q < i; ++q) g[q] -= T[q + (size_t)i * ld] * g[i];
+        }
+      }
+      copy_h2d(gdev.p, g.data(), k);
+      hipLaunchKernelGGL(k_combine, dim3(S.g256()), dim3(256), 0, stream(), n, k, (const double*)(flex ? Z.p : V.p),
+                         (int64_t)n, (const double*)gdev.p, w.p);
+      DECO_CHECK_LAUNCH();
+      if (flex || left) S.add(w.p, x);
+      else { M.apply(w.p, z.p); ++napp; S.add(z.p, x); }
+    }
+    if (!left) { A.residual(x, b, r.p); ++nspmv; }
+    else { A.residual(x, b, z.p); ++nspmv; M.apply(z.p, r.p); ++napp; }
+    rnorm = S.nrm2(r.p);
+    if (rnorm < absTol) { flag = 0; break; }
+    if (k == 0) break;   // zero residual norm with a zero tolerance: no direction to extend
   }
   info.iter = iter;
   info.res_abs = rnorm;
@@ -439,8 +724,14 @@ static void run_solver(const deco_sol_params& sp, const AOp& A, const MOp& M, co
   DECO_HIP(hipEventCreate(&e1));
   DECO_HIP(hipEventRecord(e0, stream()));
   switch (sp.type) {
-    case DECO_SOL_RGMRES: gmres(false, sp, A, M, b, x, info); break;
-    case DECO_SOL_LGMRES: gmres(true, sp, A, M, b, x, info); break;
+    case DECO_SOL_RGMRES:
+    case DECO_SOL_LGMRES: {
+      const bool left = sp.type == DECO_SOL_LGMRES;
+      if (sp.ortho == DECO_ORTHO_MGS || sp.restart > kDcgs2MaxRestart) gmres(left, sp, A, M, b, x, info);
+      else if (sp.ortho == DECO_ORTHO_DCGS2) gmres_dcgs2(left, sp, A, M, b, x, info);
+      else fail(DECO_ERR_ARG, "unknown Gram-Schmidt variant %d", sp.ortho);
+      break;
+    }
     case DECO_SOL_BICGSTAB: bicgstab(false, sp, A, M, b, x, info); break;
     case DECO_SOL_LBICGSTAB: bicgstab(true, sp, A, M, b, x, info); break;
     case DECO_SOL_PCG: pcg(sp, A, M, b, x, info); break;

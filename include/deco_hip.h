@@ -112,7 +112,13 @@ typedef struct deco_sol_params {
   int    itmax;             /* sol.itmax           */
   int    verbosity;         /* sol.verbosity       */
   int    use_x0;            /* 1: x_dev holds the initial guess (rgmres.tpp:42-47) */
+  int    ortho;             /* GMRES Gram-Schmidt: DECO_ORTHO_DCGS2 (default) or DECO_ORTHO_MGS, the
+                               reference's modified Gram-Schmidt order (rgmres.tpp:70-78); no
+                               counterpart key in the reference's "sol" tree */
 } deco_sol_params;
+
+#define DECO_ORTHO_DCGS2 0   /* one reduction per step, delayed reorthogonalisation */
+#define DECO_ORTHO_MGS   1
 
 /* Results of a solve: the members SOL keeps (solver/SOL.hpp:10-13) plus timings. */
 typedef struct deco_sol_info {
