@@ -56,6 +56,10 @@ WORKLOADS = {
 }
 
 
+WORKLOAD_KEYS = {1: "two_phase_32x32x16", 11: "two_phase_32x32x32", 2: "black_oil_46x112x22",
+                 3: "storage_aquifer_168x100x120", 5: "anisotropic_64x118x263"}
+
+
 def cpu_solve_baseline(g, M, prec_tree, sol, gpu_iters, args, log):
     """The oracle's OpenMP CPR-GMRES on the host cores, on the same system and the same AMG
     hierarchy, timed over a bounded number of iterations and scaled to the GPU's iteration
@@ -81,49 +85,89 @@ def cpu_solve_baseline(g, M, prec_tree, sol, gpu_iters, args, log):
     threads = oracle.num_threads()
     log(f"CPU oracle: {info.iter} iterations in {dt:.2f}s ({threads} threads), setup+import {setup:.1f}s")
     return {"value": per_it * gpu_iters, "unit": "s", "cores": threads, "kind": "port",
-            "sample": f"oracle CPR-rgmres(50) on the same decoupled system and AMG hierarchy, {info.iter} "
+            "sample": f"oracle CPR-{sol['type']}({sol['restart']}) on the same decoupled system and AMG hierarchy, {info.iter} "
                       f"iterations timed ({dt:.2f}s, OpenMP {threads} threads), scaled to the {gpu_iters} "
                       f"iterations of the GPU FP64 solve; per-iteration {per_it * 1e3:.1f} ms"}
+
+
+def spmv_section(deco_amd, A, n, nnz, iters):
+    """FP64 and FP32 CSR SpMV with A, timed with hipEvents on the library stream."""
+    ms64 = A.spmv_time(deco_amd._lib.PREC_FP64, iters)
+    ms32 = A.spmv_time(deco_amd._lib.PREC_FP32, iters)
+    b64 = spmv_bytes(n, n, nnz, 8, 8, 8)
+    b32 = spmv_bytes(n, n, nnz, 4, 4, 4)
+    return {"fp64_us": ms64 * 1e3, "fp64_GBps": b64 / (ms64 * 1e-3) / 1e9, "fp32_us": ms32 * 1e3,
+            "fp32_GBps": b32 / (ms32 * 1e-3) / 1e9, "bytes_fp64": b64, "bytes_fp32": b32}
+
+
+def copy_calibration(deco_amd, nbytes=512 << 20, reps=5):
+    """Plain device-to-device copies of a known size (the reference reading for the FETCH_SIZE /
+    WRITE_SIZE counters of a profiled run)."""
+    a = deco_amd.DeviceArray(nbytes // 8)
+    b = deco_amd.DeviceArray(nbytes // 8)
+    L = deco_amd.lib()
+    for _ in range(reps):
+        deco_amd.check(L.deco_memcpy_d2d(b.ptr, a.ptr, nbytes))
+    deco_amd.check(L.deco_sync())
+    return nbytes
+
+
+def traffic_from_profiles(workload_key):
+    """HBM bytes per launch of the FP64 SpMV measured by the FETCH_SIZE and WRITE_SIZE passes,
+    calibrated against a plain copy (profiles/spmv_fp64_<workload>_traffic.md), or None."""
+    path = os.path.join(ROOT, "profiles", f"spmv_fp64_{workload_key}_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cfg", type=int, default=3)
     ap.add_argument("--spmv-iters", type=int, default=50)
+    ap.add_argument("--spmv-only", action="store_true",
+                    help="only the SpMV timing and a 512 MiB copy (for counter passes)")
     ap.add_argument("--cpu-sample", type=int, default=10, help="CPU SpMV repetitions in the baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cpu-solve", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=5, help="GMRES iterations in the CPU baseline sample")
-    ap.add_argument("--export-levels", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (every GPU box here has one GPU); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
         return 2
-
     if args.cfg not in WORKLOADS:
         print(f"bench.py: unknown --cfg {args.cfg}; choose one of {sorted(WORKLOADS)}", file=sys.stderr)
         return 2
     desc, method, restart = WORKLOADS[args.cfg]
+    key = WORKLOAD_KEYS[args.cfg]
     import deco_amd
     deco_amd.init(0)
     t0 = time.time()
     g = deco_amd.generate(args.cfg)
     n, nnz = g["n"], g["nnz"]
-    log(f"generated cfg {args.cfg}: n={n} nnz={nnz} in {time.time() - t0:.1f}s")
-    A = deco_amd.CSR(n, n, g["r"], g["c"], g["v"])
+    log(f"generated {key}: n={n} nnz={nnz} in {time.time() - t0:.1f}s")
+    t0 = time.perf_counter()
+    A = deco_amd.CSR(n, n, g["r"], g["c"], g["v"])   # host CSR -> HBM (pageable host arrays)
+    deco_amd.lib().deco_sync()
+    upload_s = time.perf_counter() - t0
+    log(f"CSR upload {upload_s * 1e3:.1f} ms ({(12 * nnz + 4 * (n + 1)) / upload_s / 1e9:.1f} GB/s)")
 
     # ---------------------------------------------------------------- SpMV roofline
-    ms64 = A.spmv_time(deco_amd._lib.PREC_FP64, args.spmv_iters)
-    ms32 = A.spmv_time(deco_amd._lib.PREC_FP32, args.spmv_iters)
-    b64 = spmv_bytes(n, n, nnz, 8, 8, 8)
-    b32 = spmv_bytes(n, n, nnz, 4, 4, 4)
-    gbs64 = b64 / (ms64 * 1e-3) / 1e9
-    gbs32 = b32 / (ms32 * 1e-3) / 1e9
-    log(f"spmv fp64 {ms64 * 1e3:.1f} us {gbs64:.0f} GB/s; fp32 {ms32 * 1e3:.1f} us {gbs32:.0f} GB/s")
+    spmv = spmv_section(deco_amd, A, n, nnz, args.spmv_iters)
+    log(f"spmv fp64 {spmv['fp64_us']:.1f} us {spmv['fp64_GBps']:.0f} GB/s; "
+        f"fp32 {spmv['fp32_us']:.1f} us {spmv['fp32_GBps']:.0f} GB/s")
+    if args.spmv_only:
+        cb = copy_calibration(deco_amd)
+        print(json.dumps({"workload": key, "n": n, "nnz": nnz, "spmv": spmv, "copy_bytes": cb}), flush=True)
+        return 0
 
     result = {
         "metric": "CPR-GMRES time-to-1e-8 (FP32-precond vs FP64, 1 GPU); SpMV HBM GB/s",
@@ -131,15 +175,15 @@ def main():
         "scaling": "weak", "vs_baseline": None, "data": "synthetic",
         "config": {"workload": f"{desc}, n={n}, nnz={nnz}, CPR(AMG+FSAI) {method}({restart}) rtol 1e-8",
                    "n": n, "nnz": nnz, "parallelism": "single-gpu"},
-        "spmv": {"fp64_us": ms64 * 1e3, "fp64_GBps": gbs64, "fp32_us": ms32 * 1e3, "fp32_GBps": gbs32,
-                 "bytes_fp64": b64, "bytes_fp32": b32},
-        "roofline": {"bound": "hbm", "achieved": gbs64, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs64 / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_spmv_stream<double,double,EpiStd<EPI_SET>> (FP64 CSR SpMV with A)",
-                     "bytes_per_launch": b64},
+        "spmv": spmv,
+        "upload_s": upload_s,
+        "roofline": {"bound": "hbm", "achieved": spmv["fp64_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": spmv["fp64_GBps"] / HBM_PEAK_GBS, "traffic": traffic_from_profiles(key),
+                     "kernel": "k_spmv_stream<double,double,EpiOuter> (FP64 CSR SpMV with A)",
+                     "bytes_per_launch": spmv["bytes_fp64"]},
     }
 
-    # ---------------------------------------------------------------- CPR-GMRES solves
+    # ---------------------------------------------------------------- CPR-Krylov solves
     tree = dict(PREC_TREE, block_size=str(g["bs"]))
     prec_tree = deco_amd.PTree(tree)
     sol = {"type": method, "reltol": "1e-8", "restart": restart, "itmax": "500", "verbosity": "0"}
@@ -149,11 +193,10 @@ def main():
     M = deco_amd.CPR(prec_tree, A, b_dev)            # decouples A and b in place (CPR.hpp:42-172)
     deco_amd.lib().deco_sync()
     setup_s = time.perf_counter() - t0
-    levels, _ = M.amg().export() if args.export_levels else ([], None)
     log(f"CPR setup {setup_s:.2f}s")
-    x_ref = deco_amd.SOL(dict(sol, type="rgmres", restart="50", reltol="1e-12", itmax="2000"), A, M, b_dev)
-    xr = x_ref.x.numpy()
-    log(f"FP64 reference solve: iter {x_ref.iter} res {x_ref.res_rel:.2e}")
+    tight = deco_amd.SOL(dict(sol, type="rgmres", restart="50", reltol="1e-12", itmax="2000"), A, M, b_dev)
+    xt = tight.x.numpy()
+    log(f"FP64 reference solve (rtol 1e-12): iter {tight.iter} res {tight.res_rel:.2e}")
 
     def run(precision):
         M.set_precision(precision)
@@ -167,22 +210,50 @@ def main():
             times.append(last.info.time_ms)
         deco_amd.lib().deco_sync()
         wall = (time.perf_counter() - t1) / args.steps
-        x = last.x.numpy()
-        return dict(ms=float(np.median(times)), wall_ms=wall * 1e3, iter=last.iter, res_rel=last.res_rel,
-                    flag=last.flag, err_vs_fp64_oracle=float(np.linalg.norm(x - xr) / np.linalg.norm(xr)),
-                    err_vs_xstar=float(np.linalg.norm(x - g["xstar"]) / np.linalg.norm(g["xstar"])))
+        return last, dict(ms=float(np.median(times)), wall_ms=wall * 1e3, iter=last.iter, res_rel=last.res_rel,
+                          flag=last.flag, n_prec=last.info.n_prec)
 
-    r64 = run(deco_amd.PREC_FP64)
-    log(f"FP64-only: {r64}")
-    r32 = run(deco_amd.PREC_FP32)
-    log(f"FP32-precond: {r32}")
+    S64, r64 = run(deco_amd.PREC_FP64)
+    x64 = S64.x.numpy()
+    S32, r32 = run(deco_amd.PREC_FP32)
+    x32 = S32.x.numpy()
     M.set_precision(deco_amd.PREC_FP64)
+    err = lambda x, ref: float(np.linalg.norm(x - ref) / np.linalg.norm(ref))
+    import scipy.sparse as sps
+    A0 = sps.csr_matrix((g["v"], g["c"], g["r"]), shape=(n, n))   # the system before decoupling
+    for r, x in ((r64, x64), (r32, x32)):
+        r["err_vs_fp64_same_rtol"] = err(x, x64)
+        r["err_vs_fp64_rtol1e-12"] = err(x, xt)
+        r["err_vs_xstar"] = err(x, g["xstar"])
+        r["res_rel_original"] = float(np.linalg.norm(g["b"] - A0 @ x) / np.linalg.norm(g["b"]))
+    del A0
+    log(f"FP64-only: {r64}")
+    log(f"FP32-precond: {r32}")
     result.update({
         "value": r32["wall_ms"] / 1e3, "unit": "s", "higher_is_better": False, "dtype": "f64",
         "ms_per_step": r32["wall_ms"],
         "solve": {"fp32_precond": r32, "fp64_only": r64, "speedup_fp32_vs_fp64": r64["wall_ms"] / r32["wall_ms"],
-                  "setup_s": setup_s, "reference_solve_iter": x_ref.iter},
+                  "setup_s": setup_s, "reference_solve_iter": tight.iter},
     })
+
+    # ---------------------------------------------------------------- adjoint
+    if not args.no_adjoint and method == "rgmres":
+        w = deco_amd.DeviceArray.from_numpy(np.random.default_rng(1004).standard_normal(n))
+        St = deco_amd.SOL(sol, A, M, w, transpose=True)          # warm-up (builds A^T and the transposes)
+        deco_amd.lib().deco_sync()
+        ts = []
+        for _ in range(args.steps):
+            t1 = time.perf_counter()
+            St = deco_amd.SOL(sol, A, M, w, transpose=True)      # (W A)^T mu = w
+            lam = M.transform_transpose(St.x)                    # lambda = W^T mu
+            dA = deco_amd.grad_A_pattern(A, lam, S64.x)          # dL/dA_ij = -lambda_i x_j
+            deco_amd.lib().deco_sync()
+            ts.append(time.perf_counter() - t1)
+        adj_ms = float(np.median(ts)) * 1e3
+        del dA
+        result["adjoint"] = {"wall_ms": adj_ms, "iter": St.iter, "flag": St.flag, "res_rel": St.res_rel,
+                             "ratio_to_fp64_forward": adj_ms / r64["wall_ms"]}
+        log(f"adjoint: {result['adjoint']}")
 
     # ---------------------------------------------------------------- CPU baseline
     if not args.no_cpu:
@@ -196,11 +267,11 @@ def main():
             oracle.spmv(Ao, x)
         dt = (time.perf_counter() - t1) / args.cpu_sample
         del Ao
-        result["spmv"]["cpu_GBps"] = b64 / dt / 1e9
+        result["spmv"]["cpu_GBps"] = spmv["bytes_fp64"] / dt / 1e9
         result["spmv"]["cpu_sample"] = (f"{args.cpu_sample} FP64 CSR SpMVs with A (oracle restatement, "
                                         f"OpenMP {threads} threads); {dt * 1e3:.1f} ms each")
-        result["cpu_baseline"] = {"value": b64 / dt / 1e9, "unit": "GB/s", "cores": threads, "kind": "port",
-                                  "sample": result["spmv"]["cpu_sample"]}
+        result["cpu_baseline"] = {"value": spmv["bytes_fp64"] / dt / 1e9, "unit": "GB/s", "cores": threads,
+                                  "kind": "port", "sample": result["spmv"]["cpu_sample"]}
         if not args.no_cpu_solve:
             result["cpu_baseline"] = cpu_solve_baseline(g, M, tree, sol, r64["iter"], args, log)
     print(json.dumps(result), flush=True)

@@ -9,4 +9,4 @@ from .core import (  # noqa: F401
     CSR, DeviceArray, as_dev, init, dot, nrm2, axpby, generate, cpr_params, sol_params, lib,
     Prec, Identity, Jacobi, FSAI, AMG, CPR, SOL, grad_A_pattern, amg_interp_host,
 )
-from ._lib import PREC_FP64, PREC_FP32  # noqa: F401
+from ._lib import PREC_FP64, PREC_FP32, check  # noqa: F401

@@ -760,9 +760,9 @@ int deco_spmv_time(deco_csr* A, int precision, int iters, double* avg_ms) {
     } else {
       DBuf<double> x(A->m), y(A->n);
       x.zero();
-      spmv(A, x.p, y.p);
+      spmv_stream_f(A, A->v.p, x.p, EpiOuter{y.p});
       DECO_HIP(hipEventRecord(e0, stream()));
-      for (int k = 0; k < iters; ++k) spmv(A, x.p, y.p);
+      for (int k = 0; k < iters; ++k) spmv_stream_f(A, A->v.p, x.p, EpiOuter{y.p});
       DECO_HIP(hipEventRecord(e1, stream()));
     }
     DECO_HIP(hipEventSynchronize(e1));

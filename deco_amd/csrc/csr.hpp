@@ -122,6 +122,14 @@ struct EpiStd {
   }
 };
 
+// y[o] = sum for the Krylov solvers' product with the system matrix: its own kernel
+// instantiation, so a profile separates it from the preconditioner's products
+struct EpiOuter {
+  double* __restrict__ y;
+  template <class AT>
+  __device__ __forceinline__ void operator()(int o, AT acc) const { y[o] = (double)acc; }
+};
+
 // y[o] = z[o*zs] - sum: residual of a row subset (rows zs*o of the full matrix)
 template <class YT, class ZT>
 struct EpiStrideRes {

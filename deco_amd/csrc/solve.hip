@@ -306,7 +306,7 @@ struct AOp {
   const deco_csr* A;
   bool transpose;
   void apply(const double* x, double* y) const {
-    if (!transpose) spmv(A, x, y);
+    if (!transpose) spmv_stream_f(A, A->v.p, x, EpiOuter{y});
     else spmv(A->At, x, y);
   }
   void residual(const double* x, const double* b, double* r) const {  // r = b - A x
