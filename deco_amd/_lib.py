@@ -91,6 +91,7 @@ SIGNATURES = {
     "deco_csr_filter_proper": (i, [vp, d]),
     "deco_csr_local_matrix": (i, [vp, i, i, vp, pp]),
     "deco_csr_transpose": (i, [vp, pp]),
+    "deco_block_reorder": (i, [vp, i, C.c_char_p, C.c_char_p, vp]),
     "deco_spmv": (i, [vp, vp, vp, i]),
     "deco_spmv_residual": (i, [vp, vp, vp, vp]),
     "deco_spmv_transpose": (i, [vp, vp, vp]),

@@ -169,6 +169,12 @@ class CSR:
         check(lib().deco_csr_copy(self.h, C.byref(h)))
         return CSR(handle=h.value)
 
+    def block_reorder(self, bs, row_order, col_order, b=None):
+        """AD::block_reorder (adapter/csr2coo.cpp:19-88) in place; b (DeviceArray) is permuted
+        like the rows when given."""
+        check(lib().deco_block_reorder(self.h, bs, row_order.encode(), col_order.encode(),
+                                       b.ptr if b is not None else None))
+
     def filter(self, tol):
         check(lib().deco_csr_filter(self.h, tol))
 

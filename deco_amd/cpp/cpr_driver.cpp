@@ -44,6 +44,7 @@ int main(int argc, char* argv[]) {
     A = IO::cpp_read_binary_matrix<int, int, double>(param.get<std::string>("IO.file_A"));
     b = IO::cpp_read_binary_rhs<int, int, double>(param.get<std::string>("IO.file_b"), A);
   }
+  AD::block_reorder(A, b, param);   // cpr.cpp:41
   auto t1 = Clock::now();
   std::printf("matrix size:\tn = %d, m = %d, nz = %d\n", A->n, A->m, A->nz);
   const std::string glo = param.get<std::string>("prec.prec_glo.type");

@@ -403,3 +403,21 @@ class SOL {
     std::free(b);
   }
 };
+
+// AD::block_reorder (adapter/csr2coo.cpp:19-88): the driver's step before the solve
+// (cpr.cpp:41), run on the device; the host arrays of A and b are refreshed from it.
+namespace AD {
+template <class R, class C, class V>
+void block_reorder(std::shared_ptr<CSR<R, C, V>> A, V*& b, const deco_ptree& param) {
+  const int bs = param.template get<int>("prec.block_size");
+  const std::string row_order = param.template get<std::string>("prec.row_order");
+  const std::string col_order = param.template get<std::string>("prec.col_order");
+  deco_facade::DevBuf bd;
+  bd.ensure(A->n);
+  deco_facade::check(deco_memcpy_h2d(bd.p, b, sizeof(V) * A->n), "h2d");
+  deco_facade::check(deco_block_reorder(A->device(), bs, row_order.c_str(), col_order.c_str(), bd.p),
+                     "block_reorder");
+  deco_facade::check(deco_memcpy_d2h(b, bd.p, sizeof(V) * A->n), "d2h");
+  A->pull();
+}
+}  // namespace AD

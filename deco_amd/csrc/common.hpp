@@ -144,6 +144,24 @@ __device__ __forceinline__ int xcd_block(int i, int G) {
   return (x < r) ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
 }
 
+// Appends `item` to list `bin` (lists[bin*stride + pos], pos from cnt[bin]); bin < 0 appends
+// nothing. One atomic per wavefront and bin: lanes with the same bin take consecutive slots
+// from a single atomicAdd by the lowest such lane. Every lane of the wave must call it.
+template <int NBINS>
+__device__ __forceinline__ void wave_bin_append(int bin, int item, int* cnt, int* lists, size_t stride) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int q = 0; q < NBINS; ++q) {
+    const unsigned long long m = __ballot(bin == q);
+    if (m == 0) continue;
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&cnt[q], __popcll(m));
+    base = __shfl(base, leader);
+    if (bin == q) lists[(size_t)q * stride + base + __popcll(m & ((1ull << lane) - 1ull))] = item;
+  }
+}
+
 // Block-wide sum of one value per thread, fixed order (deterministic). BLOCK multiple of 64.
 template <int BLOCK, class T>
 __device__ __forceinline__ T block_sum(T v, T* smem /* BLOCK/64 entries */) {

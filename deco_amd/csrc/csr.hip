@@ -476,6 +476,104 @@ deco_csr* csr_row_subset(const deco_csr* A, int stride, int off) {
   return S;
 }
 
+// ------------------------------------------------------------------ in-cell reorder ---
+// Unknown k of a cell moves to position reord[k] (rows: reord_row, columns: reord_col);
+// new row i holds old row (i/bs)*bs + ord_row[i%bs] (adapter/csr2coo.cpp:31-37,56-67,82-86).
+struct CellPerm {
+  int ord_row[9];
+  int reord_col[9];
+};
+
+__global__ void k_reorder_len(int n, int bs, CellPerm p, const int* __restrict__ r, int* __restrict__ len) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int o = (i / bs) * bs + p.ord_row[i % bs];
+  len[i] = r[o + 1] - r[o];
+}
+
+// One thread per new row: copy the old row with mapped columns. The old row is sorted and the
+// map permutes columns only inside aligned groups of bs, so sorting the runs of one column cell
+// gives the row sorted by column, the order of the reference's (row, col) sort
+// (csr2coo.cpp:69-72).
+__global__ void k_reorder_fill(int n, int bs, CellPerm p, const int* __restrict__ r, const int* __restrict__ c,
+                               const double* __restrict__ v, const int* __restrict__ r2, int* __restrict__ c2,
+                               double* __restrict__ v2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int o = (i / bs) * bs + p.ord_row[i % bs];
+  const int b = r[o], e = r[o + 1], d = r2[i];
+  int run = d;
+  for (int k = b; k < e; ++k) {
+    const int col = c[k], cell = col / bs, rest = col - cell * bs;
+    const int nc = cell * bs + p.reord_col[rest];
+    const double val = v[k];
+    int q = d + (k - b);
+    if (k > b && c[k - 1] / bs != cell) run = q;
+    while (q > run && c2[q - 1] > nc) {   // insertion inside the current column cell
+      c2[q] = c2[q - 1];
+      v2[q] = v2[q - 1];
+      --q;
+    }
+    c2[q] = nc;
+    v2[q] = val;
+  }
+}
+
+__global__ void k_reorder_vec(int n, int bs, CellPerm p, const double* __restrict__ b, double* __restrict__ b2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b2[i] = b[(i / bs) * bs + p.ord_row[i % bs]];
+}
+
+static void parse_order(const char* s, int bs, int* reord, int* ord, const char* what) {
+  if (!s) fail(DECO_ERR_ARG, "%s is null", what);
+  int seen[9] = {0};
+  for (int i = 0; i < bs; ++i) {
+    const int d = s[i] - '1';
+    if (s[i] == 0 || d < 0 || d >= bs || seen[d])
+      fail(DECO_ERR_ARG, "%s \"%s\" is not a permutation of 1..%d", what, s, bs);
+    seen[d] = 1;
+    reord[d] = i;
+    if (ord) ord[i] = d;
+  }
+}
+
+void block_reorder(deco_csr* A, int bs, const char* row_order, const char* col_order, double* b) {
+  if (bs < 1 || bs > 9) fail(DECO_ERR_UNSUPPORTED, "block size %d (1..9 supported)", bs);
+  if (A->n % bs != 0 || A->m % bs != 0) fail(DECO_ERR_ARG, "n=%d m=%d not divisible by bs=%d", A->n, A->m, bs);
+  if (bs == 1) return;                                             // csr2coo.cpp:39
+  CellPerm p{};
+  int reord_row[9];
+  parse_order(row_order, bs, reord_row, p.ord_row, "row_order");
+  parse_order(col_order, bs, p.reord_col, nullptr, "col_order");
+  if (bs == 2 && reord_row[0] == 0 && reord_row[1] == 1 && p.reord_col[0] == 0 && p.reord_col[1] == 1)
+    return;                                                        // csr2coo.cpp:40-41
+  const int n = A->n;
+  if (n == 0) return;
+  DBuf<int> len(n + 1), r2;
+  hipLaunchKernelGGL(k_reorder_len, dim3(grid_for(n, 256)), dim3(256), 0, stream(), n, bs, p, A->r.p, len.p);
+  DECO_CHECK_LAUNCH();
+  r2.alloc(n + 1);
+  scan_to_ptr_async(len.p, r2.p, n);
+  DBuf<int> c2(A->nnz);
+  DBuf<double> v2(A->nnz);
+  hipLaunchKernelGGL(k_reorder_fill, dim3(grid_for(n, 256)), dim3(256), 0, stream(), n, bs, p, A->r.p, A->c.p,
+                     A->v.p, r2.p, c2.p, v2.p);
+  DECO_CHECK_LAUNCH();
+  if (b) {
+    DBuf<double> b2(n);
+    hipLaunchKernelGGL(k_reorder_vec, dim3(grid_for(n, 256)), dim3(256), 0, stream(), n, bs, p, (const double*)b,
+                       b2.p);
+    DECO_CHECK_LAUNCH();
+    copy_d2d(b, b2.p, n);
+  }
+  A->r = std::move(r2);
+  A->c = std::move(c2);
+  A->v = std::move(v2);
+  A->invalidate();
+  if (A->At) { delete A->At; A->At = nullptr; }
+  sync();
+}
+
 void csr_filter(deco_csr* A, double tol) { filter_impl<0>(A, tol); }
 void csr_filter_proper(deco_csr* A, double tol) { filter_impl<1>(A, tol); }
 
@@ -694,6 +792,13 @@ int deco_csr_local_matrix(const deco_csr* A, int bs, int p, const double* w, dec
     if (!A || !out) fail(DECO_ERR_ARG, "null argument");
     *out = local_matrix(A, bs, p, w);
     sync();
+  });
+}
+
+int deco_block_reorder(deco_csr* A, int bs, const char* row_order, const char* col_order, double* b_dev) {
+  return guard([&] {
+    if (!A) fail(DECO_ERR_ARG, "null CSR");
+    block_reorder(A, bs, row_order, col_order, b_dev);
   });
 }
 

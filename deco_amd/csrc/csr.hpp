@@ -98,15 +98,14 @@ void launch_spmv(int G, int nout, int rs, const int* r, const int* c, const VT* 
 }
 
 // ------------------------------------------------------------------ CSR-stream SpMV ------
-// Block k streams the contiguous entry range of its row chunk into LDS: every lane issues
-// kStreamUnroll independent (column, value) loads and then the x gathers, so many bytes are in
-// flight per wave; the products are then summed per row in column order (the same order as
-// alg::spmv, core/alg.hpp:59-61). Chunks whose range exceeds the LDS tile (very long rows)
-// fall back to a block reduction per row.
+// Chunk k holds the rows whose first entry lies in [k*kStreamCap, (k+1)*kStreamCap). A block
+// streams the contiguous entry range of its chunk(s) into LDS: every lane issues 6 (FP64) or 12
+// (FP32) independent (column, value) loads and then the x gathers, so many bytes are in flight
+// per wave; the products are then summed per row in column order (the same order as alg::spmv,
+// core/alg.hpp:59-61). Ranges that exceed the LDS tile (rows over 129 entries) fall back to a
+// block reduction per row.
 constexpr int kStreamCap = 3072;                 // target entries per chunk
-constexpr int kStreamLds = kStreamCap + 128;     // LDS tile: a chunk holds <= cap + longest row - 1
 constexpr int kStreamBlock = 512;
-constexpr int kStreamUnroll = kStreamCap / kStreamBlock;
 
 // Epilogue functors: called once per row with the row sum (accumulator type AT).
 template <int EPI, class YT, class ZT>
@@ -140,24 +139,36 @@ struct EpiStrideRes {
   __device__ __forceinline__ void operator()(int o, AT acc) const { y[o] = (YT)((AT)z[(size_t)o * zs] - acc); }
 };
 
+// Chunks per block (one: two chunks per block for FP32, 12 loads per lane, measured 3-7% slower
+// on the config-3 operators and slower still on small AMG levels with half the blocks).
+template <class VT, class XT>
+constexpr int stream_cpb() {
+  return 1;
+}
+
 template <class VT, class XT, class E>
-__global__ __launch_bounds__(kStreamBlock) void k_spmv_stream(const int* __restrict__ rb, const int* __restrict__ r,
-                                                             const int* __restrict__ c, const VT* __restrict__ v,
-                                                             const XT* __restrict__ x, E epi) {
+__global__ __launch_bounds__(kStreamBlock) void k_spmv_stream(int nch, const int* __restrict__ rb,
+                                                             const int* __restrict__ r, const int* __restrict__ c,
+                                                             const VT* __restrict__ v, const XT* __restrict__ x,
+                                                             E epi) {
   using AT = acc_t<VT, XT>;
-  __shared__ AT prod[kStreamLds];
-  const int ch = xcd_block(blockIdx.x, gridDim.x);
-  const int r0 = rb[ch], r1 = rb[ch + 1];
+  constexpr int CPB = stream_cpb<VT, XT>();
+  constexpr int CAP = CPB * kStreamCap;
+  constexpr int LDSN = CAP + 128;      // a group of CPB chunks holds <= CAP + longest row - 1
+  constexpr int U = CAP / kStreamBlock;
+  __shared__ AT prod[LDSN];
+  const int grp = xcd_block(blockIdx.x, gridDim.x);
+  const int r0 = rb[grp * CPB], r1 = rb[min(grp * CPB + CPB, nch)];
   if (r0 >= r1) return;
   const int j0 = r[r0], j1 = r[r1];
   const int cnt = j1 - j0;
   const int tid = threadIdx.x;
-  if (cnt <= kStreamLds) {
-    // first tile: kStreamUnroll independent loads per lane
-    int cc[kStreamUnroll];
-    VT vv[kStreamUnroll];
+  if (cnt <= LDSN) {
+    // first tile: U independent loads per lane
+    int cc[U];
+    VT vv[U];
 #pragma unroll
-    for (int u = 0; u < kStreamUnroll; ++u) {
+    for (int u = 0; u < U; ++u) {
       const int t = tid + u * kStreamBlock;
       if (t < cnt) {
         cc[u] = __builtin_nontemporal_load(c + j0 + t);
@@ -165,11 +176,11 @@ __global__ __launch_bounds__(kStreamBlock) void k_spmv_stream(const int* __restr
       }
     }
 #pragma unroll
-    for (int u = 0; u < kStreamUnroll; ++u) {
+    for (int u = 0; u < U; ++u) {
       const int t = tid + u * kStreamBlock;
       if (t < cnt) prod[t] = (AT)vv[u] * (AT)x[cc[u]];
     }
-    for (int t = tid + kStreamCap; t < cnt; t += kStreamBlock)
+    for (int t = tid + CAP; t < cnt; t += kStreamBlock)
       prod[t] = (AT)__builtin_nontemporal_load(v + j0 + t) * (AT)x[__builtin_nontemporal_load(c + j0 + t)];
     __syncthreads();
     for (int row = r0 + tid; row < r1; row += kStreamBlock) {
@@ -194,8 +205,9 @@ template <class VT, class XT, class E>
 void spmv_stream_f(const deco_csr* A, const VT* v, const XT* x, E epi) {
   if (A->n <= 0) return;
   const int* rb = A->row_chunks();
-  hipLaunchKernelGGL((k_spmv_stream<VT, XT, E>), dim3(A->nchunks), dim3(kStreamBlock), 0, stream(), rb, A->r.p,
-                     A->c.p, v, x, epi);
+  constexpr int CPB = stream_cpb<VT, XT>();
+  hipLaunchKernelGGL((k_spmv_stream<VT, XT, E>), dim3((A->nchunks + CPB - 1) / CPB), dim3(kStreamBlock), 0, stream(),
+                     A->nchunks, rb, A->r.p, A->c.p, v, x, epi);
   DECO_CHECK_LAUNCH();
 }
 

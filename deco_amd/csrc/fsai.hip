@@ -50,17 +50,15 @@ __global__ void k_tril_fill(int n, const int* __restrict__ r, const int* __restr
 __global__ void k_fsai_bin(int n, const int* __restrict__ rf, int* __restrict__ cnt, int* __restrict__ lists,
                            int* __restrict__ bad) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int k = rf[i + 1] - rf[i] - 1;
-  if (k <= 0) return;
-  int b;
-  if (k <= 8) b = 0;
+  const int k = (i < n) ? rf[i + 1] - rf[i] - 1 : 0;
+  int b = -1;
+  if (k <= 0) b = -1;
+  else if (k <= 8) b = 0;
   else if (k <= 16) b = 1;
   else if (k <= 32) b = 2;
   else if (k <= 64) b = 3;
-  else { atomicMax(bad, k); return; }
-  const int pos = atomicAdd(&cnt[b], 1);
-  lists[(size_t)b * n + pos] = i;
+  else atomicMax(bad, k);
+  wave_bin_append<4>(b, i, cnt, lists, (size_t)n);
 }
 
 // ---------------------------------------------------------------- batched LU ------------

@@ -29,9 +29,16 @@
 //   carried by the pressure equation):
 //     A[s_i^k, s_i^k] = sacc(i,k) + sum_{j: up(ij)=i} lam(i,k)
 //     A[s_i^k, s_j^k] = -lam(j,k) if up(ij)=j else 0 (explicit zero)
-//     A[s_i^k, p_i]   = cpl * sum_j lam(up(ij),k)
-//     A[s_i^k, p_j]   = -cpl * lam(up(ij),k)
+//     A[s_i^k, p_j]   = -cpl_ij * lam(up(ij),k) * T_ij / Tbar_i   (Tbar_i = mean T of cell i's faces)
+//     A[s_i^k, p_i]   = sum_j cpl_ij * lam(up(ij),k) * T_ij / Tbar_i
 //     A[s_i^k, s_i^l] = cpl * sacc(i,l)  (l != k)   A[s_i^k, s_j^l] = -cpl * lam(up,l)
+//   The pressure couplings of a saturation row are its fluxes' pressure derivatives: they follow
+//   the face transmissibilities (normalised to the row's O(1) scale) and sum to zero, because a
+//   flux depends on pressure differences only. Then the decoupled pressure matrix keeps the
+//   pressure block's M-matrix structure and positive row sums. With couplings that ignore T_ij,
+//   or whose diagonal is an independent draw, the quasi-IMPES correction (~cpl^2 sum_j T_ij)
+//   puts positive off-diagonals and negative row sums into it wherever T varies strongly
+//   between faces (sigma 3), and CPR stagnated.
 // (With T_ij * lam as the saturation coefficient the saturation block is transport dominated,
 // T lam / sacc up to ~1e3 for the sigma-2 layered case, and CPR iteration counts grow without
 // bound with the grid; that variant is selected by sat_scale_T.)
@@ -98,6 +105,7 @@ struct Spec {
   double sacc_lo = 0.5, sacc_hi = 2.0;
   double cpl_lo = 0.01, cpl_hi = 0.1;
   bool sat_scale_T = false;    // saturation transport T*lam instead of lam
+  bool sat_p_flux = true;      // saturation-row pressure couplings weighted by T_ij / mean T
   uint64_t seed = 0;
 };
 
@@ -242,6 +250,9 @@ static void build_system(const Spec& sp, deco_gen* g) {
     if (!placed) cols.push_back(i);
     double sumT = 0.0;
     for (auto& e : N) sumT += e.second;
+    // face weights of the saturation rows' pressure couplings: T_ij over the cell's mean T
+    const double tbar = N.empty() ? 1.0 : sumT / (double)N.size();
+    auto fw = [&](double T) { return sp.sat_p_flux && !sp.sat_scale_T ? T / tbar : 1.0; };
     const double acc_i = uab(seed, S_ACC, (uint64_t)i, sp.acc_lo, sp.acc_hi);
     for (int k = 0; k < bs; ++k) {
       const int row = i * bs + k;
@@ -253,7 +264,7 @@ static void build_system(const Spec& sp, deco_gen* g) {
           int up = upwind(i, e.first);
           const double ts = sp.sat_scale_T ? e.second : 1.0;
           if (up == i) out_k += ts * lam(i, k);
-          flux_k += ts * lam(up, k);
+          flux_k += cpl_off(i, e.first, k, 0) * ts * fw(e.second) * lam(up, k);
         }
       size_t ni = 0;
       for (int cj : cols) {
@@ -273,7 +284,7 @@ static void build_system(const Spec& sp, deco_gen* g) {
           } else {
             const double ts = sp.sat_scale_T ? T : 1.0;
             if (l == k) val = diag ? sacc(i, k) + out_k : (up == cj ? -ts * lam(cj, k) : 0.0);
-            else if (l == 0) val = diag ? cpl_diag(i, k, 0) * flux_k : -cpl_off(i, cj, k, 0) * ts * lam(up, k);
+            else if (l == 0) val = diag ? flux_k : -cpl_off(i, cj, k, 0) * ts * fw(T) * lam(up, k);
             else val = diag ? cpl_diag(i, k, l) * sacc(i, l) : -cpl_off(i, cj, k, l) * ts * lam(up, l);
           }
           g->c[pos] = cj * bs + l;

@@ -476,6 +476,20 @@ static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, cons
     bnorm = S.nrm2(r.p);
     absTol = tol * bnorm;
   }
+  // w = B u_j for step j (B = A M^{-1} right, M^{-1} A left); z / Z column j hold M^{-1} u_j
+  auto apply_op = [&](int jj, const double* uu) {
+    if (!left) {
+      double* zj = flex ? Z.p + (size_t)jj * n : z.p;
+      M.apply(uu, zj);
+      A.apply(zj, w.p);
+    } else { A.apply(uu, z.p); M.apply(z.p, w.p); }
+    ++napp; ++nspmv;
+  };
+  struct Ev {
+    hipEvent_t e = nullptr;
+    Ev() { DECO_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming)); }
+    ~Ev() { (void)hipEventDestroy(e); }
+  } ev;
   while (iter < itmax) {
     std::fill(Hr.begin(), Hr.end(), 0.0);
     std::fill(Rm.begin(), Rm.end(), 0.0);
@@ -483,16 +497,13 @@ static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, cons
     double* bufs[2] = {ua.p, ub.p};
     double* u = r.p;
     int j = 0, k = 0;
+    bool have_w = false;   // step j's operator application is already queued
+    double rn1 = -1.0, rn2 = -1.0;   // the last two residual estimates of this cycle
     while (true) {
       const bool last = (j == m) || (it0 + j >= itmax);
       const double* wp = nullptr;
       if (!last) {
-        if (!left) {
-          double* zj = flex ? Z.p + (size_t)j * n : z.p;
-          M.apply(u, zj);
-          A.apply(zj, w.p);
-        } else { A.apply(u, z.p); M.apply(z.p, w.p); }
-        ++napp; ++nspmv;
+        if (!have_w) apply_op(j, u);
         wp = w.p;
       }
       const int nv = wp ? 2 * j + 2 : j + 1;
@@ -509,7 +520,16 @@ static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, cons
         DECO_CHECK_LAUNCH();
       }
       DECO_HIP(hipMemcpyAsync(red, redd.p, sizeof(double) * nv, hipMemcpyDeviceToHost, stream()));
-      sync();
+      DECO_HIP(hipEventRecord(ev.e, stream()));
+      // Queue step j+1's operator application on u_{j+1} before the host reads step j, so the
+      // device works while the host rotates the Hessenberg column; if the solve stops at step j
+      // this application is not used (stream order keeps every later use of w, z after it).
+      // Skipped where the last two estimates extrapolate below the tolerance at this step.
+      const bool next_last = (j + 1 == m) || (it0 + j + 1 >= itmax);
+      const bool likely_stop = rn1 > 0.0 && rn2 > 0.0 && rn1 * (rn1 / rn2) < absTol;
+      have_w = wp && !next_last && !likely_stop;
+      if (have_w) apply_op(j + 1, un);
+      DECO_HIP(hipEventSynchronize(ev.e));
       double rho, gam = 0.0;
       if (wp) arn_scalars(j, red, rho, gam);
       else {
@@ -521,6 +541,7 @@ static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, cons
       if (j == 0) {
         g[0] = rho;
         rnorm = std::fabs(g[0]);
+        rn1 = rnorm;
         if (rnorm < absTol) { flag = 0; break; }
         if (last || !(rho > 0.0)) break;
       } else {
@@ -537,6 +558,8 @@ static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, cons
         ++iter;
         k = j;
         rnorm = std::fabs(g[j]);
+        rn2 = rn1;
+        rn1 = rnorm;
         if (rnorm < absTol) { flag = 0; break; }
         if (last || !(rho > 0.0)) break;
       }

@@ -31,11 +31,12 @@ __global__ void k_spgemm_ub(int n, const int* __restrict__ ra, const int* __rest
 // bin rows by upper bound into lists (order inside a list is irrelevant for the result)
 __global__ void k_spgemm_bin(int n, const int* __restrict__ ub, int* __restrict__ cnt, int* __restrict__ lists) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int u = ub[i];
-  const int b = (u <= 256) ? 0 : (u <= 1024) ? 1 : (u <= 4096) ? 2 : 3;
-  const int pos = atomicAdd(&cnt[b], 1);
-  lists[(size_t)b * n + pos] = i;
+  int b = -1;
+  if (i < n) {
+    const int u = ub[i];
+    b = (u <= 256) ? 0 : (u <= 1024) ? 1 : (u <= 4096) ? 2 : 3;
+  }
+  wave_bin_append<4>(b, i, cnt, lists, (size_t)n);
 }
 
 // Wave-level bitonic sort of P (power of two) keys in LDS, then dedup. Returns unique count

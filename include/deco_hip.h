@@ -164,6 +164,14 @@ int deco_csr_filter_proper(deco_csr* A, double tol);
 /* CSR::local_matrix(bs,p,w) (core/CSR.hpp:338-420): rows bs*i+p, columns c%bs==p -> c/bs;
  * w_dev (length n) divides the kept values when not NULL. */
 int deco_csr_local_matrix(const deco_csr* A, int bs, int p, const double* w_dev, deco_csr** out);
+/* In-cell reordering of the unknowns, the step before the solve (AD::block_reorder,
+ * adapter/csr2coo.cpp:19-88, called at cpr.cpp:41). row_order / col_order are the
+ * "prec.row_order" / "prec.col_order" digit strings (1-based, e.g. "213"): in every cell of bs
+ * unknowns, position i receives the unknown numbered by the i-th digit. Rows and columns of A
+ * are permuted, rows stay sorted by column, and b_dev (length n, may be NULL) is permuted like
+ * the rows. In place. bs 1, and bs 2 with both orders "12", leave A unchanged as the reference
+ * does. */
+int deco_block_reorder(deco_csr* A, int bs, const char* row_order, const char* col_order, double* b_dev);
 /* CSR::trans_mat_full (core/CSR.hpp:151-211) as an out-of-place transpose (rows sorted). */
 int deco_csr_transpose(const deco_csr* A, deco_csr** out);
 /* CSR::apply(x,b,bs) -> alg::spmv (core/alg.hpp:51-64): y = A x for bs == 1; for bs > 1

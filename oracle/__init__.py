@@ -46,6 +46,7 @@ def lib():
             "orc_nrm2": (d, [i, vp]),
             "orc_filter": (i, [P(OrcCSR), d]),
             "orc_filter_proper": (i, [P(OrcCSR), d]),
+            "orc_block_reorder": (i, [P(OrcCSR), vp, i, C.c_char_p, C.c_char_p]),
             "orc_local_matrix": (i, [P(OrcCSR), i, i, vp, P(OrcCSR)]),
             "orc_trans_mat_full": (i, [P(OrcCSR), P(OrcCSR)]),
             "orc_spgemm": (i, [P(OrcCSR), P(OrcCSR), P(OrcCSR)]),
@@ -165,6 +166,15 @@ def _inplace(fn, A, *args):
     _check(fn(C.byref(s), *args))
     A.sync_from(s)
     return A
+
+
+def block_reorder(A, b, bs, row_order, col_order):
+    """AD::block_reorder (adapter/csr2coo.cpp:19-88) on copies: returns (A', b')."""
+    B = Mat(A.n, A.m, A.r[:A.n + 1], A.c[:A.nnz], A.v[:A.nnz])
+    bb = np.array(b, np.float64, copy=True) if b is not None else None
+    _check(lib().orc_block_reorder(C.byref(B.view()), _vp(bb) if bb is not None else None, bs,
+                                   row_order.encode(), col_order.encode()))
+    return B, bb
 
 
 def filter(A, tol):

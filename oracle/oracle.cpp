@@ -1294,6 +1294,61 @@ double orc_dot(int n, const double* x, const double* y) { return dot(n, x, y); }
 double orc_nrm2(int n, const double* x) { return nrm2(n, x); }
 
 // in place: arrays keep their allocation, A->nnz shrinks
+// AD::block_reorder (adapter/csr2coo.cpp:19-88): COO row indices, rows and columns mapped
+// inside each cell (:54-66), one sort of the entries by (row, column) (:69-72), row pointers from
+// the sorted rows (:75), b permuted per cell (:78-86). Returns false when A is left unchanged.
+static bool block_reorder(CSR& A, double* b, int bs, const char* row_ord, const char* col_ord) {
+  if (bs < 1 || bs > 9) throw std::invalid_argument("block size out of range");
+  if (A.n % bs || A.m % bs) throw std::invalid_argument("n not divisible by bs");
+  std::vector<int> reord_row(bs), reord_col(bs), ord_row(bs), seen_r(bs, 0), seen_c(bs, 0);
+  if (bs == 1) return false;
+  for (int i = 0; i < bs; ++i) {
+    const int row = row_ord[i] - '1', col = col_ord[i] - '1';
+    if (row < 0 || row >= bs || col < 0 || col >= bs || seen_r[row] || seen_c[col])
+      throw std::invalid_argument("order is not a permutation");
+    seen_r[row] = seen_c[col] = 1;
+    reord_row[row] = i;
+    reord_col[col] = i;
+    ord_row[i] = row;
+  }
+  if (bs == 2 && reord_row[0] == 0 && reord_row[1] == 1 && reord_col[0] == 0 && reord_col[1] == 1) return false;
+  const int nz = A.nnz();
+  std::vector<int> l(nz);
+  for (int i = 0; i < A.n; ++i)
+    for (int k = A.r[i]; k < A.r[i + 1]; ++k) l[k] = i;
+  for (int k = 0; k < nz; ++k) {
+    const int rr = l[k] % bs;
+    l[k] = l[k] - rr + reord_row[rr];
+    const int rc = A.c[k] % bs;
+    A.c[k] = A.c[k] - rc + reord_col[rc];
+  }
+  std::vector<int> idx(nz);
+  for (int k = 0; k < nz; ++k) idx[k] = k;
+  std::sort(idx.begin(), idx.end(), [&](int a, int b2) {
+    if (l[a] != l[b2]) return l[a] < l[b2];
+    return A.c[a] < A.c[b2];
+  });
+  std::vector<int> c2(nz);
+  std::vector<double> v2(nz);
+  std::fill(A.r.begin(), A.r.end(), 0);
+  for (int k = 0; k < nz; ++k) {
+    c2[k] = A.c[idx[k]];
+    v2[k] = A.v[idx[k]];
+    ++A.r[l[idx[k]] + 1];
+  }
+  for (int i = 0; i < A.n; ++i) A.r[i + 1] += A.r[i];
+  A.c.swap(c2);
+  A.v.swap(v2);
+  if (b) {
+    std::vector<double> scratch(bs);
+    for (int s0 = 0; s0 < A.n / bs; ++s0) {
+      for (int q = 0; q < bs; ++q) scratch[q] = b[s0 * bs + q];
+      for (int q = 0; q < bs; ++q) b[s0 * bs + q] = scratch[ord_row[q]];
+    }
+  }
+  return true;
+}
+
 static void write_back(const CSR& B, orc_csr* A) {
   A->nnz = B.nnz();
   std::memcpy(A->r, B.r.data(), sizeof(int) * (B.n + 1));
@@ -1301,6 +1356,9 @@ static void write_back(const CSR& B, orc_csr* A) {
     std::memcpy(A->c, B.c.data(), sizeof(int) * B.nnz());
     std::memcpy(A->v, B.v.data(), sizeof(double) * B.nnz());
   }
+}
+int orc_block_reorder(orc_csr* A, double* b, int bs, const char* row_order, const char* col_order) {
+  return guard([&] { CSR B = from_c(A); if (block_reorder(B, b, bs, row_order, col_order)) write_back(B, A); });
 }
 int orc_filter(orc_csr* A, double tol) {
   return guard([&] { CSR B = from_c(A); filter(B, tol, false); write_back(B, A); });

@@ -38,6 +38,7 @@ double orc_dot(int n, const double* x, const double* y);
 double orc_nrm2(int n, const double* x);
 
 /* core/CSR.hpp */
+int orc_block_reorder(orc_csr* A, double* b, int bs, const char* row_order, const char* col_order);
 int orc_filter(orc_csr* A, double tol);
 int orc_filter_proper(orc_csr* A, double tol);
 int orc_local_matrix(const orc_csr* A, int bs, int p, const double* w, orc_csr* out);

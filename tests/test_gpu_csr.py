@@ -93,3 +93,19 @@ def test_empty_and_single_row_edges(gpu):
     E = deco_amd.CSR(3, 3, [0, 0, 0, 0], np.zeros(0, np.int32), np.zeros(0))
     assert np.array_equal(E.apply(np.ones(3)).numpy(), np.zeros(3))
     assert E.transpose().nz == 0
+
+
+@pytest.mark.parametrize("cfg,bs,ro,co", [(2, 3, "213", "213"), (2, 3, "312", "132"), (1, 2, "21", "21"),
+                                          (1, 2, "12", "12")])
+def test_block_reorder_bitexact(gpu, cfg, bs, ro, co):
+    """deco_block_reorder against the oracle's AD::block_reorder: r, c, v and b bit-identical."""
+    g = gen_system(cfg)
+    A = deco_amd.CSR(g["n"], g["n"], g["r"], g["c"], g["v"])
+    b = deco_amd.DeviceArray.from_numpy(g["b"])
+    A.block_reorder(bs, ro, co, b)
+    Ao, bo = oracle.block_reorder(to_mat(g), g["b"], bs, ro, co)
+    r, c, v = A.download()
+    assert np.array_equal(r, Ao.r) and np.array_equal(c, Ao.c[:Ao.nnz]) and np.array_equal(v, Ao.v[:Ao.nnz])
+    assert np.array_equal(b.numpy(), bo)
+    with pytest.raises(deco_amd._lib.DecoError):
+        A.block_reorder(bs, "1" * bs, co)

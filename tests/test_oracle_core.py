@@ -72,6 +72,51 @@ def test_local_matrix_brute_force():
         assert np.array_equal(L.v[L.r[i]:L.r[i + 1]], Dp[i, cols])
 
 
+def cell_perm(n, bs, order):
+    """new index of every old unknown: position i of a cell receives unknown order[i]-1."""
+    reord = np.empty(bs, np.int64)
+    for i, ch in enumerate(order[:bs]):
+        reord[int(ch) - 1] = i
+    old = np.arange(n)
+    return (old // bs) * bs + reord[old % bs]
+
+
+@pytest.mark.parametrize("bs,ro,co", [(3, "213", "213"), (3, "312", "132"), (2, "21", "12"), (3, "123", "123")])
+def test_block_reorder_is_the_cell_permutation(bs, ro, co):
+    """AD::block_reorder against permutation matrices: A' = P_r A P_c^T, b' = P_r b, rows sorted,
+    explicit zeros kept."""
+    g = gen_system(0, custom=(5, 4, 3, bs))
+    n = g["n"]
+    A = to_mat(g)
+    Ar, br = oracle.block_reorder(A, g["b"], bs, ro, co)
+    pr, pc = cell_perm(n, bs, ro), cell_perm(n, bs, co)
+    S = A.to_scipy().tocoo()
+    E = sp.csr_matrix((S.data, (pr[S.row], pc[S.col])), shape=(n, n))
+    E.sort_indices()
+    assert np.array_equal(Ar.r, E.indptr) and np.array_equal(Ar.c[:Ar.nnz], E.indices)
+    assert np.array_equal(Ar.v[:Ar.nnz], E.data)
+    bexp = np.empty(n)
+    bexp[pr] = g["b"]
+    assert np.array_equal(br, bexp)
+    # the permuted system has the permuted solution: x'[pc] = x*
+    xs = np.empty(n)
+    xs[pc] = g["xstar"]
+    assert np.allclose(E @ xs, bexp, rtol=1e-12, atol=1e-12 * np.abs(bexp).max())
+
+
+def test_block_reorder_identity_and_errors():
+    g = gen_system(0, custom=(4, 3, 2, 2))
+    A = to_mat(g)
+    Ar, br = oracle.block_reorder(A, g["b"], 2, "12", "12")          # csr2coo.cpp:40-41 early return
+    assert np.array_equal(Ar.v, A.v) and np.array_equal(br, g["b"])
+    Ar, br = oracle.block_reorder(A, g["b"], 1, "1", "1")            # bs 1 early return
+    assert np.array_equal(Ar.c, A.c)
+    with pytest.raises(RuntimeError):
+        oracle.block_reorder(A, g["b"], 2, "11", "12")
+    with pytest.raises(RuntimeError):
+        oracle.block_reorder(A, g["b"], 2, "1", "12")
+
+
 def test_transpose_matches_scipy():
     S = rand_csr(50, 37, 0.2, 4, diag=False)
     T = oracle.transpose(oracle.Mat.from_scipy(S))
