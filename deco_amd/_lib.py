@@ -96,6 +96,7 @@ SIGNATURES = {
     "deco_spmv_residual": (i, [vp, vp, vp, vp]),
     "deco_spmv_transpose": (i, [vp, vp, vp]),
     "deco_spmv_f32": (i, [vp, vp, vp]),
+    "deco_spmv_block_f32": (i, [vp, i, i, vp, vp]),
     "deco_spmv_time": (i, [vp, i, i, dp]),
     "deco_spgemm": (i, [vp, vp, pp]),
     "deco_spgemm_tril_pattern": (i, [vp, pp]),
@@ -122,6 +123,7 @@ SIGNATURES = {
     "deco_amg_coarse_inverse": (i, [vp, ip, pp]),
     "deco_amg_interp_host": (i, [C.POINTER(CprParams), i, vp, vp, vp, vp, ip, ip, pp, pp, pp]),
     "deco_host_free": (None, [vp]),
+    "deco_workspace_release": (i, []),
     "deco_solve": (i, [C.POINTER(SolParams), vp, vp, vp, vp, C.POINTER(SolInfo)]),
     "deco_solve_transpose": (i, [C.POINTER(SolParams), vp, vp, vp, vp, C.POINTER(SolInfo)]),
     "deco_grad_A_pattern": (i, [vp, vp, vp, vp]),

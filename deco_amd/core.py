@@ -152,11 +152,15 @@ class CSR:
         check(lib().deco_spmv_transpose(self.h, xd.ptr, y.ptr))
         return y
 
-    def apply_f32(self, x):
+    def apply_f32(self, x, block=None):
+        """y = A x with FP32 values; block=(R, C) uses the FP32 block shadow of A."""
         n, m, _ = self.info()
         xd = as_dev(x, np.float32)
         y = DeviceArray(n, np.float32)
-        check(lib().deco_spmv_f32(self.h, xd.ptr, y.ptr))
+        if block is None:
+            check(lib().deco_spmv_f32(self.h, xd.ptr, y.ptr))
+        else:
+            check(lib().deco_spmv_block_f32(self.h, int(block[0]), int(block[1]), xd.ptr, y.ptr))
         return y
 
     def spmv_time(self, precision=_lib.PREC_FP64, iters=20):

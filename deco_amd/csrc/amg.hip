@@ -103,6 +103,21 @@ struct EpiRelax {
   }
 };
 
+// Restriction b_c = R r fused with the next level's zero-guess Jacobi-type sweep: each coarse
+// row stores b_c and x_c = (0 + (b_c - 0)/d_c) (k_relax_zero's arithmetic).
+template <class T>
+struct EpiRestrictJacobi0 {
+  T* __restrict__ bn;
+  const T* __restrict__ dn;
+  T* __restrict__ xn;
+  template <class AT>
+  __device__ __forceinline__ void operator()(int o, AT acc) const {
+    const T bo = (T)acc;
+    bn[o] = bo;
+    xn[o] = (T)0 + (bo - (T)0) / dn[o];
+  }
+};
+
 // coarse solve x = C b, C row-major with leading dimension nc (the inverse for the forward
 // cycle, its transpose for the transposed one): one wavefront per row, lanes stride the
 // columns, fixed-order butterfly sum; input/output in T, FP64 arithmetic
@@ -266,7 +281,7 @@ struct AMG : deco_prec {
 
   // one V-cycle on level l with right-hand side b, result in x (x != b)
   template <class T>
-  void cycle(size_t l, const T* b, T* x, bool tr) {
+  void cycle(size_t l, const T* b, T* x, bool tr, bool x_ready = false) {
     if (l == L.size()) {
       if (nc == 0) return;
       hipLaunchKernelGGL(k_coarse<T>, dim3((nc + 3) / 4), dim3(256), 0, stream(), nc,
@@ -285,17 +300,22 @@ struct AMG : deco_prec {
       DECO_CHECK_LAUNCH();
       relax_sweep<T>(lv, A, x, b, t, 1);
       relax_sweep<T>(lv, A, t, b, x, 0);
-    } else {
+    } else if (!x_ready) {
       hipLaunchKernelGGL(k_relax_zero<T>, dim3(grid_for(nl, 256)), dim3(256), 0, stream(), nl, b, D<T>(lv), lv.cf.p, -1, x);
       DECO_CHECK_LAUNCH();
     }
-    // residual and restriction
+    // residual and restriction (the restriction also does the next level's zero-guess sweep
+    // for the Jacobi-type smoothers)
     spmv_stream<EPI_RES, T, T, T, T>(A, vals<T>(const_cast<deco_csr*>(A)), x, r, b);
     const bool last = (l + 1 == L.size());
     T* bn = last ? BC<T>() : B<T>(L[l + 1]);
     T* xn = last ? XC<T>() : X<T>(L[l + 1]);
-    spmv_stream<EPI_SET, T, T, T, T>(lv.R, vals<T>(lv.R), r, bn, (const T*)nullptr);
-    cycle<T>(l + 1, bn, xn, tr);
+    const bool fuse = !last && relax != 17;
+    if (fuse)
+      spmv_stream_f(lv.R, vals<T>(lv.R), (const T*)r, EpiRestrictJacobi0<T>{bn, D<T>(L[l + 1]), xn});
+    else
+      spmv_stream<EPI_SET, T, T, T, T>(lv.R, vals<T>(lv.R), r, bn, (const T*)nullptr);
+    cycle<T>(l + 1, bn, xn, tr, fuse);
     // prolongation t = x + P xn, then the up smoother back into x
     spmv_stream<EPI_ADDZ, T, T, T, T>(lv.P, vals<T>(lv.P), xn, t, x);
     if (relax == 17) {

@@ -515,7 +515,13 @@ struct CPR : deco_prec {
     spmv_stream_f(Apr, Apr->v.p, x, EpiStrideRes<double, double>{out, r, bs});
   }
   void pressure_residual(const float* x, const double* r, float* out) {
-    spmv_stream_f(Apr, Apr->values_f32(), x, EpiStrideRes<float, double>{out, r, bs});
+    if (const deco_bsr32* B = Apr->block32(1, bs)) bsr_stream_f(B, x, EpiStrideRes<float, double>{out, r, bs});
+    else spmv_stream_f(Apr, Apr->values_f32(), x, EpiStrideRes<float, double>{out, r, bs});
+  }
+  // out = r - A x with FP32 values of the decoupled A (cell blocks)
+  void residual32(const float* x, const double* r, float* out) {
+    if (const deco_bsr32* B = A->block32(bs, bs)) bsr_stream_f(B, x, EpiStd<EPI_RES, float, double>{out, r});
+    else spmv_stream<EPI_RES, float, float, float, double>(A, A->values_f32(), x, out, r);
   }
 
   void apply(const double* r, double* x) override {
@@ -547,7 +553,6 @@ struct CPR : deco_prec {
 
   // FP32 internals, FP64 in and out
   void apply32_seq(const double* r, double* xo) {
-    const float* A32 = A->values_f32();
     deco::FSAI* F = as_fsai(glo);
     auto glo32_from64 = [&](const double* in, float* out) {
       if (F) fsai_apply32(F, in, out);
@@ -563,7 +568,7 @@ struct CPR : deco_prec {
       DECO_HIP(hipMemsetAsync(x32.p, 0, sizeof(float) * n, stream()));
       hipLaunchKernelGGL((k_set_pressure<float, float>), dim3(gs()), dim3(256), 0, stream(), sn, bs, x32.p,
                          (const float*)xp32.p);
-      spmv_stream<EPI_RES, float, float, float, double>(A, A32, x32.p, rt32.p, r);
+      residual32(x32.p, r, rt32.p);
       if (F) fsai_apply32_f(F, rt32.p, x32.p);
       else glo->apply32(rt32.p, x32.p);
       hipLaunchKernelGGL((k_add_pressure<float, float>), dim3(gs()), dim3(256), 0, stream(), sn, bs, x32.p,
@@ -579,7 +584,7 @@ struct CPR : deco_prec {
                        (const float*)xp32.p);
     DECO_CHECK_LAUNCH();
     if (seq == 0) { convert(x32.p, xo, n); return; }
-    spmv_stream<EPI_RES, float, float, float, double>(A, A32, x32.p, rt32.p, r);
+    residual32(x32.p, r, rt32.p);
     if (F) fsai_apply32_addz(F, rt32.p, x32.p, xo);
     else {
       glo->apply32(rt32.p, t32.p);

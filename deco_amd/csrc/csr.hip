@@ -591,7 +591,111 @@ void to_f32(const double* a, float* b, int64_t n) {
 
 }  // namespace deco
 
-deco_csr::~deco_csr() { delete At; }
+deco_csr::~deco_csr() {
+  delete At;
+  delete b32;
+}
+
+void deco_csr::invalidate() {
+  vf_valid = false;
+  rb_valid = false;
+  delete b32;
+  b32 = nullptr;
+}
+
+namespace deco {
+// number of distinct column blocks over the R rows of block row I (rows sorted by column)
+__global__ void k_bsr_count(int nb, int R, int C, const int* __restrict__ r, const int* __restrict__ c,
+                            int* __restrict__ cnt) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= nb) return;
+  int pos[3], end[3];
+  for (int q = 0; q < R; ++q) { pos[q] = r[I * R + q]; end[q] = r[I * R + q + 1]; }
+  int count = 0;
+  while (true) {
+    int mn = INT_MAX;
+    for (int q = 0; q < R; ++q)
+      if (pos[q] < end[q]) mn = min(mn, c[pos[q]] / C);
+    if (mn == INT_MAX) break;
+    ++count;
+    for (int q = 0; q < R; ++q)
+      while (pos[q] < end[q] && c[pos[q]] / C == mn) ++pos[q];
+  }
+  cnt[I] = count;
+}
+
+// block columns and row-major FP32 block values (bv zeroed beforehand)
+__global__ void k_bsr_fill(int nb, int R, int C, const int* __restrict__ r, const int* __restrict__ c,
+                           const double* __restrict__ v, const int* __restrict__ br, int* __restrict__ bc,
+                           float* __restrict__ bv) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= nb) return;
+  int pos[3], end[3];
+  for (int q = 0; q < R; ++q) { pos[q] = r[I * R + q]; end[q] = r[I * R + q + 1]; }
+  int o = br[I];
+  while (true) {
+    int mn = INT_MAX;
+    for (int q = 0; q < R; ++q)
+      if (pos[q] < end[q]) mn = min(mn, c[pos[q]] / C);
+    if (mn == INT_MAX) break;
+    bc[o] = mn;
+    for (int q = 0; q < R; ++q)
+      for (; pos[q] < end[q] && c[pos[q]] / C == mn; ++pos[q])
+        bv[(size_t)o * R * C + q * C + (c[pos[q]] - mn * C)] = (float)v[pos[q]];
+    ++o;
+  }
+}
+
+// rb[k] = first row i in [0, n] with ptr[i] >= k*cap; rb[nchunks] = n
+__global__ void k_chunks_cap(int n, const int* __restrict__ ptr, int nchunks, int cap, int* __restrict__ rb) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > nchunks) return;
+  if (k == nchunks) { rb[k] = n; return; }
+  const int64_t target = (int64_t)k * cap;
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (ptr[mid] >= target) hi = mid; else lo = mid + 1;
+  }
+  rb[k] = lo;
+}
+}  // namespace deco
+
+const deco_bsr32* deco_csr::block32(int R, int C) const {
+  using namespace deco;
+  if (b32 && b32->R == R && b32->C == C) return b32;
+  delete b32;
+  b32 = nullptr;
+  if (R < 1 || R > 3 || C < 1 || C > 3 || n % R != 0 || m % C != 0) return nullptr;
+  auto* B = new deco_bsr32();
+  B->R = R;
+  B->C = C;
+  B->nb = n / R;
+  const int nb = B->nb;
+  DBuf<int> cnt(nb + 1);
+  B->br.alloc(nb + 1);
+  if (nb > 0) {
+    hipLaunchKernelGGL(k_bsr_count, dim3(grid_for(nb, 256)), dim3(256), 0, stream(), nb, R, C, r.p, c.p, cnt.p);
+    DECO_CHECK_LAUNCH();
+  }
+  const int64_t nnzb = scan_to_ptr(cnt.p, B->br.p, nb);
+  B->nnzb = (int)nnzb;
+  B->bc.alloc(std::max<int64_t>(nnzb, 1));
+  B->bv.alloc(std::max<int64_t>(nnzb, 1) * R * C);
+  DECO_HIP(hipMemsetAsync(B->bv.p, 0, sizeof(float) * std::max<int64_t>(nnzb, 1) * R * C, stream()));
+  if (nb > 0) {
+    hipLaunchKernelGGL(k_bsr_fill, dim3(grid_for(nb, 256)), dim3(256), 0, stream(), nb, R, C, r.p, c.p, v.p,
+                       B->br.p, B->bc.p, B->bv.p);
+    DECO_CHECK_LAUNCH();
+  }
+  B->nchunks = (int)std::max<int64_t>(1, (nnzb + kBsrCap - 1) / kBsrCap);
+  B->rb.alloc(B->nchunks + 1);
+  hipLaunchKernelGGL(k_chunks_cap, dim3(grid_for(B->nchunks + 1, 256)), dim3(256), 0, stream(), nb, B->br.p,
+                     B->nchunks, kBsrCap, B->rb.p);
+  DECO_CHECK_LAUNCH();
+  b32 = B;
+  return B;
+}
 
 namespace deco {
 // rb[k] = first row i in [0, n] with r[i] >= k*CAP; rb[nchunks] = n
@@ -844,6 +948,16 @@ int deco_spmv_f32(deco_csr* A, const float* x, float* y) {
     if (!A) fail(DECO_ERR_ARG, "null CSR");
     const float* vf = A->values_f32();
     spmv_stream<EPI_SET, float, float, float, float>(A, vf, x, y, (const float*)nullptr);
+    sync();
+  });
+}
+
+int deco_spmv_block_f32(deco_csr* A, int R, int C, const float* x, float* y) {
+  return guard([&] {
+    if (!A) fail(DECO_ERR_ARG, "null CSR");
+    const deco_bsr32* B = A->block32(R, C);
+    if (!B) fail(DECO_ERR_ARG, "%d x %d blocks do not tile a %d x %d matrix", R, C, A->n, A->m);
+    bsr_stream_f(B, x, EpiStd<EPI_SET, float, float>{y, nullptr});
     sync();
   });
 }

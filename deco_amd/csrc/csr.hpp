@@ -5,6 +5,16 @@
 #pragma once
 #include "common.hpp"
 
+// FP32 block-CSR shadow of a CSR matrix whose rows and columns come in groups (R rows per block
+// row, C columns per block column): block row I covers rows I*R .. I*R+R-1, block (I, J) holds
+// the R x C values of columns J*C .. J*C+C-1, row-major, zeros where the CSR has no entry.
+// Chunk table: block rows whose first block lies in [k*kBsrCap, (k+1)*kBsrCap).
+struct deco_bsr32 {
+  int R = 1, C = 1, nb = 0, nnzb = 0, nchunks = 0;
+  deco::DBuf<int> br, bc, rb;
+  deco::DBuf<float> bv;
+};
+
 struct deco_csr {
   int n = 0, m = 0, nnz = 0;
   deco::DBuf<int> r, c;
@@ -19,7 +29,11 @@ struct deco_csr {
   ~deco_csr();
   const float* values_f32();  // creates / refreshes the shadow
   const int* row_chunks() const;  // creates the chunk table on first use
-  void invalidate() { vf_valid = false; rb_valid = false; }
+  mutable deco_bsr32* b32 = nullptr;  // FP32 block shadow (valid with vf_valid's lifetime rules)
+  // the FP32 block shadow with R x C blocks (created on first use); null when n or m do not
+  // divide into such blocks
+  const deco_bsr32* block32(int R, int C) const;
+  void invalidate();
 };
 
 namespace deco {
@@ -146,11 +160,19 @@ constexpr int stream_cpb() {
   return 1;
 }
 
-template <class VT, class XT, class E>
+// x access of the stream kernel: a plain vector, or a functor that computes x_j (fused steps)
+template <class XT>
+struct PtrGather {
+  using type = XT;
+  const XT* __restrict__ x;
+  __device__ __forceinline__ XT operator()(int j) const { return x[j]; }
+};
+
+template <class VT, class XG, class E>
 __global__ __launch_bounds__(kStreamBlock) void k_spmv_stream(int nch, const int* __restrict__ rb,
                                                              const int* __restrict__ r, const int* __restrict__ c,
-                                                             const VT* __restrict__ v, const XT* __restrict__ x,
-                                                             E epi) {
+                                                             const VT* __restrict__ v, XG x, E epi) {
+  using XT = typename XG::type;
   using AT = acc_t<VT, XT>;
   constexpr int CPB = stream_cpb<VT, XT>();
   constexpr int CAP = CPB * kStreamCap;
@@ -178,10 +200,10 @@ __global__ __launch_bounds__(kStreamBlock) void k_spmv_stream(int nch, const int
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int t = tid + u * kStreamBlock;
-      if (t < cnt) prod[t] = (AT)vv[u] * (AT)x[cc[u]];
+      if (t < cnt) prod[t] = (AT)vv[u] * (AT)x(cc[u]);
     }
     for (int t = tid + CAP; t < cnt; t += kStreamBlock)
-      prod[t] = (AT)__builtin_nontemporal_load(v + j0 + t) * (AT)x[__builtin_nontemporal_load(c + j0 + t)];
+      prod[t] = (AT)__builtin_nontemporal_load(v + j0 + t) * (AT)x(__builtin_nontemporal_load(c + j0 + t));
     __syncthreads();
     for (int row = r0 + tid; row < r1; row += kStreamBlock) {
       const int b = r[row] - j0, e = r[row + 1] - j0;
@@ -194,26 +216,171 @@ __global__ __launch_bounds__(kStreamBlock) void k_spmv_stream(int nch, const int
     for (int row = r0; row < r1; ++row) {
       const int b = r[row], e = r[row + 1];
       AT acc = AT(0);
-      for (int j = b + tid; j < e; j += kStreamBlock) acc += (AT)v[j] * (AT)x[c[j]];
+      for (int j = b + tid; j < e; j += kStreamBlock) acc += (AT)v[j] * (AT)x(c[j]);
       acc = block_sum<kStreamBlock>(acc, red);
       if (tid == 0) epi(row, acc);
     }
   }
 }
 
-template <class VT, class XT, class E>
-void spmv_stream_f(const deco_csr* A, const VT* v, const XT* x, E epi) {
+template <class VT, class XG, class E>
+void spmv_stream_g(const deco_csr* A, const VT* v, XG x, E epi) {
   if (A->n <= 0) return;
   const int* rb = A->row_chunks();
-  constexpr int CPB = stream_cpb<VT, XT>();
-  hipLaunchKernelGGL((k_spmv_stream<VT, XT, E>), dim3((A->nchunks + CPB - 1) / CPB), dim3(kStreamBlock), 0, stream(),
+  constexpr int CPB = stream_cpb<VT, typename XG::type>();
+  hipLaunchKernelGGL((k_spmv_stream<VT, XG, E>), dim3((A->nchunks + CPB - 1) / CPB), dim3(kStreamBlock), 0, stream(),
                      A->nchunks, rb, A->r.p, A->c.p, v, x, epi);
   DECO_CHECK_LAUNCH();
+}
+
+template <class VT, class XT, class E>
+void spmv_stream_f(const deco_csr* A, const VT* v, const XT* x, E epi) {
+  spmv_stream_g(A, v, PtrGather<XT>{x}, epi);
 }
 
 template <int EPI, class VT, class XT, class YT, class ZT>
 void spmv_stream(const deco_csr* A, const VT* v, const XT* x, YT* y, const ZT* z) {
   spmv_stream_f(A, v, x, EpiStd<EPI, YT, ZT>{y, z});
+}
+
+// ------------------------------------------------------------------ FP32 block SpMV ----
+// Same scheme as the CSR stream kernel on blocks: a workgroup takes the block rows of one chunk,
+// every lane loads kBsrUnroll (column, R x C values) blocks and the C x entries of each, the R
+// block products go to LDS, and each block row's R sums run over its blocks in column order.
+// Per 2 x 2 block: 4 + 16 bytes instead of 4 x (4 + 4) in FP32 CSR.
+constexpr int kBsrCap = 1536;
+constexpr int kBsrBlock = 512;
+constexpr int kBsrUnroll = kBsrCap / kBsrBlock;
+
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+
+template <int N>
+struct BsrVals { float a[N]; };
+
+template <int R, int C>
+__device__ __forceinline__ BsrVals<R * C> bsr_load(const float* __restrict__ bv, int j) {
+  BsrVals<R * C> o;
+  if constexpr (R * C == 4) {
+    const f32x4 t = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(bv) + j);
+    o.a[0] = t.x; o.a[1] = t.y; o.a[2] = t.z; o.a[3] = t.w;
+  } else if constexpr (R * C == 2) {
+    const f32x2 t = __builtin_nontemporal_load(reinterpret_cast<const f32x2*>(bv) + j);
+    o.a[0] = t.x; o.a[1] = t.y;
+  } else {
+#pragma unroll
+    for (int q = 0; q < R * C; ++q) o.a[q] = __builtin_nontemporal_load(bv + (size_t)j * (R * C) + q);
+  }
+  return o;
+}
+
+template <int C, class XT>
+__device__ __forceinline__ void bsr_gather(const XT* __restrict__ x, int col, XT (&xv)[C]) {
+  if constexpr (C == 2 && sizeof(XT) == 4) {
+    const f32x2 t = *(reinterpret_cast<const f32x2*>(x) + col);
+    xv[0] = t.x; xv[1] = t.y;
+  } else if constexpr (C == 2 && sizeof(XT) == 8) {
+    const f64x2 t = *(reinterpret_cast<const f64x2*>(x) + col);
+    xv[0] = t.x; xv[1] = t.y;
+  } else {
+#pragma unroll
+    for (int q = 0; q < C; ++q) xv[q] = x[(size_t)col * C + q];
+  }
+}
+
+template <int R, int C, class XT, class E>
+__global__ __launch_bounds__(kBsrBlock) void k_bsr_stream(const int* __restrict__ rb, const int* __restrict__ br,
+                                                         const int* __restrict__ bc, const float* __restrict__ bv,
+                                                         const XT* __restrict__ x, E epi) {
+  using AT = acc_t<float, XT>;
+  constexpr int LDSB = kBsrCap + 64;          // blocks: a chunk holds <= cap + longest block row - 1
+  __shared__ AT prod[LDSB * R];
+  const int ch = xcd_block(blockIdx.x, gridDim.x);
+  const int r0 = rb[ch], r1 = rb[ch + 1];
+  if (r0 >= r1) return;
+  const int j0 = br[r0], cnt = br[r1] - j0;
+  const int tid = threadIdx.x;
+  if (cnt <= LDSB) {
+    int cc[kBsrUnroll];
+    BsrVals<R * C> vv[kBsrUnroll];
+#pragma unroll
+    for (int u = 0; u < kBsrUnroll; ++u) {
+      const int t = tid + u * kBsrBlock;
+      if (t < cnt) {
+        cc[u] = __builtin_nontemporal_load(bc + j0 + t);
+        vv[u] = bsr_load<R, C>(bv, j0 + t);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kBsrUnroll; ++u) {
+      const int t = tid + u * kBsrBlock;
+      if (t < cnt) {
+        XT xv[C];
+        bsr_gather<C>(x, cc[u], xv);
+#pragma unroll
+        for (int rr = 0; rr < R; ++rr) {
+          AT p = (AT)vv[u].a[rr * C] * (AT)xv[0];
+#pragma unroll
+          for (int q = 1; q < C; ++q) p += (AT)vv[u].a[rr * C + q] * (AT)xv[q];
+          prod[t * R + rr] = p;
+        }
+      }
+    }
+    for (int t = tid + kBsrCap; t < cnt; t += kBsrBlock) {
+      const BsrVals<R * C> w = bsr_load<R, C>(bv, j0 + t);
+      XT xv[C];
+      bsr_gather<C>(x, bc[j0 + t], xv);
+#pragma unroll
+      for (int rr = 0; rr < R; ++rr) {
+        AT p = (AT)w.a[rr * C] * (AT)xv[0];
+#pragma unroll
+        for (int q = 1; q < C; ++q) p += (AT)w.a[rr * C + q] * (AT)xv[q];
+        prod[t * R + rr] = p;
+      }
+    }
+    __syncthreads();
+    for (int row = r0 + tid; row < r1; row += kBsrBlock) {
+      const int b = br[row] - j0, e = br[row + 1] - j0;
+#pragma unroll
+      for (int rr = 0; rr < R; ++rr) {
+        AT acc = AT(0);
+        for (int k = b; k < e; ++k) acc += prod[k * R + rr];
+        epi(row * R + rr, acc);
+      }
+    }
+  } else {
+    __shared__ AT red[kBsrBlock / 64];
+    for (int row = r0; row < r1; ++row) {
+#pragma unroll
+      for (int rr = 0; rr < R; ++rr) {
+        AT acc = AT(0);
+        for (int j = br[row] + tid; j < br[row + 1]; j += kBsrBlock) {
+          XT xv[C];
+          bsr_gather<C>(x, bc[j], xv);
+#pragma unroll
+          for (int q = 0; q < C; ++q) acc += (AT)bv[(size_t)j * (R * C) + rr * C + q] * (AT)xv[q];
+        }
+        acc = block_sum<kBsrBlock>(acc, red);
+        if (tid == 0) epi(row * R + rr, acc);
+      }
+    }
+  }
+}
+
+template <class XT, class E>
+void bsr_stream_f(const deco_bsr32* B, const XT* x, E epi) {
+  if (B->nb <= 0) return;
+#define DECO_BSR_LAUNCH(RR, CC)                                                                            \
+  hipLaunchKernelGGL((k_bsr_stream<RR, CC, XT, E>), dim3(B->nchunks), dim3(kBsrBlock), 0, stream(), B->rb.p, \
+                     B->br.p, B->bc.p, B->bv.p, x, epi)
+  if (B->R == 2 && B->C == 2) DECO_BSR_LAUNCH(2, 2);
+  else if (B->R == 1 && B->C == 2) DECO_BSR_LAUNCH(1, 2);
+  else if (B->R == 3 && B->C == 3) DECO_BSR_LAUNCH(3, 3);
+  else if (B->R == 1 && B->C == 3) DECO_BSR_LAUNCH(1, 3);
+  else fail(DECO_ERR_UNSUPPORTED, "block shape %d x %d", B->R, B->C);
+#undef DECO_BSR_LAUNCH
+  DECO_CHECK_LAUNCH();
 }
 
 // Convenience wrappers on a deco_csr (values f64 or the f32 shadow).

@@ -257,10 +257,12 @@ struct FSAI : deco_prec {
   deco_csr* FLt = nullptr;   // transposed FL (transposed apply), built on demand
   DBuf<double> temp;
   DBuf<float> temp32;
+  int cb = 1;                // cell size of the unknowns: the FP32 factors are stored in cb x cb blocks
 
   FSAI(const deco_cpr_params& p, const deco_csr* A_in) {
     kind = FSAI_KIND;
     n = A_in->n;
+    if ((p.block_size == 2 || p.block_size == 3) && n % p.block_size == 0) cb = p.block_size;
     deco_csr* A = nullptr;
     const deco_csr* Au = A_in;
     if (p.glo_tol_rel != 0.0 || p.glo_tol_abs != 0.0) {
@@ -375,17 +377,24 @@ struct FSAI : deco_prec {
     spmv(FU, r, temp.p);
     spmv_stream<EPI_ACC, double, double, double, double>(FLt, FLt->v.p, temp.p, x, (const double*)nullptr);
   }
+  // y = F x with FP32 factor values (cell blocks when the unknowns come in cells), epilogue e
+  template <class XT, class E>
+  void f32_product(const deco_csr* F, const XT* x, E e) {
+    const deco_bsr32* B = cb > 1 ? F->block32(cb, cb) : nullptr;
+    if (B) bsr_stream_f(B, x, e);
+    else spmv_stream_f(F, const_cast<deco_csr*>(F)->values_f32(), x, e);
+  }
   // FP32 factors; input of any precision, output f32 (or f64 with EPI_ADDZ from the caller)
   template <class TI>
   void apply32_in(const TI* r, float* x) {
-    spmv_stream<EPI_SET, float, TI, float, float>(FL, FL->values_f32(), r, temp32.p, (const float*)nullptr);
-    spmv_stream<EPI_SET, float, float, float, float>(FUt, FUt->values_f32(), temp32.p, x, (const float*)nullptr);
+    f32_product(FL, r, EpiStd<EPI_SET, float, float>{temp32.p, nullptr});
+    f32_product(FUt, (const float*)temp32.p, EpiStd<EPI_SET, float, float>{x, nullptr});
   }
   // xo (f64) = z (f32) + M r
   template <class TI>
   void apply32_addz(const TI* r, const float* z, double* xo) {
-    spmv_stream<EPI_SET, float, TI, float, float>(FL, FL->values_f32(), r, temp32.p, (const float*)nullptr);
-    spmv_stream<EPI_ADDZ, float, float, double, float>(FUt, FUt->values_f32(), temp32.p, xo, z);
+    f32_product(FL, r, EpiStd<EPI_SET, float, float>{temp32.p, nullptr});
+    f32_product(FUt, (const float*)temp32.p, EpiStd<EPI_ADDZ, double, float>{xo, z});
   }
   template <class TI>
   void apply32_t_in(const TI* r, float* x) {

@@ -243,6 +243,21 @@ __global__ void k_axpby2(int n, double a, const double* __restrict__ x, double b
   if (i < n) y[i] = b * y[i] + a * x[i];
 }
 
+// ------------------------------------------------------------------ workspace ----------
+// Work vectors of the Krylov solvers, kept between solves and grown on demand, so that a solve
+// does not allocate and free its basis (1.6 GB at 4M unknowns and restart 50, twice that with
+// flexible directions) every time. One solve at a time uses them (the library runs on one
+// stream); deco_workspace_release() frees them.
+constexpr int kWsSlots = 12;
+static DBuf<double> g_ws[kWsSlots];
+struct WsRef {
+  double* p;
+};
+static WsRef ws(int slot, size_t count) {
+  g_ws[slot].ensure(std::max<size_t>(count, 1));
+  return WsRef{g_ws[slot].p};
+}
+
 // ------------------------------------------------------------------ host helpers -------
 struct Solver {
   int n;
@@ -338,8 +353,9 @@ static void gmres(bool left, const deco_sol_params& sp, const AOp& A, const MOp&
   // update. For that case the right-preconditioned solve keeps Z = [M^{-1} v_j] (flexible GMRES)
   // and updates x += Z y; the FP64 preconditioner keeps the form of rgmres.tpp:106-118.
   const bool flex = !left && M.M && M.M->precision == DECO_PREC_FP32;
-  DBuf<double> r(n), z(n), w(n), V((size_t)n * (m + 1)), gdev(m + 1), hcol(m + 2);
-  DBuf<double> Z(flex ? (size_t)n * m : 0);
+  WsRef r = ws(0, n), z = ws(1, n), w = ws(2, n), V = ws(3, (size_t)n * (m + 1)), gdev = ws(4, m + 1),
+        hcol = ws(5, m + 2);
+  WsRef Z = flex ? ws(6, (size_t)n * m) : WsRef{nullptr};
   std::vector<double> H((size_t)(m + 1) * (m + 3), 0.0);
   double* g = H.data() + (size_t)(m + 1) * m;
   double* cs = g + (m + 1);
@@ -457,9 +473,10 @@ static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, cons
   if (m <= 0) fail(DECO_ERR_ARG, "restart must be positive");
   const bool flex = !left && M.M && M.M->precision == DECO_PREC_FP32;
   const int nbk = (int)std::max<int64_t>(1, ((int64_t)n + kArnTile - 1) / kArnTile);
-  DBuf<double> r(n), z(n), w(n), ua(n), ub(n), V((size_t)n * (m + 1)), gdev(m + 1);
-  DBuf<double> part((size_t)(2 * m + 2) * nbk), redd(2 * m + 2);
-  DBuf<double> Z(flex ? (size_t)n * m : 0);
+  WsRef r = ws(0, n), z = ws(1, n), w = ws(2, n), ua = ws(7, n), ub = ws(8, n), V = ws(3, (size_t)n * (m + 1)),
+        gdev = ws(4, m + 1);
+  WsRef part = ws(9, (size_t)(2 * m + 2) * nbk), redd = ws(10, 2 * m + 2);
+  WsRef Z = flex ? ws(6, (size_t)n * m) : WsRef{nullptr};
   const size_t ld = (size_t)m + 1;
   std::vector<double> Hr(ld * m), Rm(ld * m), T(flex ? ld * m : 0), g(m + 1), cs(m), sn(m), hs(m + 1);
   double* red = ctx().pinned;
@@ -620,7 +637,7 @@ static void bicgstab(bool left, const deco_sol_params& sp, const AOp& A, const M
                      deco_sol_info& info) {
   const int n = A.transpose ? A.A->m : A.A->n;
   Solver S(n);
-  DBuf<double> r(n), rt(n), p(n), y(n), v(n), t(n);
+  WsRef r = ws(0, n), rt = ws(1, n), p = ws(2, n), y = ws(7, n), v = ws(8, n), t = ws(9, n);
   double rho_old, rho = 1, alpha = 1, beta, omega = 1;
   const double reltol = sp.reltol;
   const int itmax = sp.itmax;
@@ -710,7 +727,7 @@ static void pcg(const deco_sol_params& sp, const AOp& A, const MOp& M, const dou
                 deco_sol_info& info) {
   const int n = A.transpose ? A.A->m : A.A->n;
   Solver S(n);
-  DBuf<double> r(n), p(n), z(n), g(n);
+  WsRef r = ws(0, n), p = ws(2, n), z = ws(1, n), g = ws(7, n);
   const double norm_b = S.nrm2(b), tol = norm_b * sp.reltol;
   int it = 0, napp = 0, nspmv = 0;
   if (!sp.use_x0) { DECO_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, stream())); copy_d2d(r.p, b, n); }
@@ -738,6 +755,10 @@ static void pcg(const deco_sol_params& sp, const AOp& A, const MOp& M, const dou
   info.bnorm = norm_b;
   info.n_prec = napp;
   info.n_spmv = nspmv;
+}
+
+static void release_workspace() {
+  for (auto& b : g_ws) b.release();
 }
 
 static void run_solver(const deco_sol_params& sp, const AOp& A, const MOp& M, const double* b, double* x,
@@ -949,6 +970,13 @@ int deco_prec_set_precision(deco_prec* M, int precision) {
     if (precision != DECO_PREC_FP64 && precision != DECO_PREC_FP32) fail(DECO_ERR_ARG, "bad precision %d", precision);
     M->set_precision(precision);
     sync();
+  });
+}
+
+int deco_workspace_release(void) {
+  return guard([&] {
+    sync();
+    release_workspace();
   });
 }
 

@@ -183,6 +183,11 @@ int deco_spmv_residual(const deco_csr* A, const double* x_dev, const double* z_d
 int deco_spmv_transpose(deco_csr* A, const double* x_dev, double* y_dev);
 /* FP32-value shadow of A (created on first use) times an FP32 vector. */
 int deco_spmv_f32(deco_csr* A, const float* x_dev, float* y_dev);
+/* The same product through the FP32 block shadow of A with R x C blocks (R, C in 1..3; rows
+ * and columns of A in groups of R and C), the form the FP32 preconditioner uses for the
+ * cell-structured operators of CPR.hpp (decoupled A, its pressure rows, the FSAI factors).
+ * No reference counterpart (the reference's FP64 CSR product is alg::spmv). */
+int deco_spmv_block_f32(deco_csr* A, int R, int C, const float* x_dev, float* y_dev);
 /* Times `iters` back-to-back SpMVs on the library stream with hipEvents and returns the
  * average kernel duration in milliseconds. precision: DECO_PREC_FP64 (f64 values/vectors)
  * or DECO_PREC_FP32 (f32 values/vectors). */
@@ -254,6 +259,10 @@ void deco_host_free(void* ptr);
  * use_x0). M may be NULL (identity). */
 int deco_solve(const deco_sol_params* sp, const deco_csr* A, deco_prec* M,
                const double* b_dev, double* x_dev, deco_sol_info* info);
+/* The solvers keep their work vectors (Krylov basis, flexible directions, partial sums) between
+ * calls and grow them on demand; this frees them. No reference counterpart (the reference
+ * allocates per solve, solver/rgmres.tpp:18-29). */
+int deco_workspace_release(void);
 /* Solves A^T y = g with the transposed preconditioner M^T (adjoint path). */
 int deco_solve_transpose(const deco_sol_params* sp, deco_csr* A, deco_prec* M,
                          const double* g_dev, double* y_dev, deco_sol_info* info);

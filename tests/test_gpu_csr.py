@@ -109,3 +109,31 @@ def test_block_reorder_bitexact(gpu, cfg, bs, ro, co):
     assert np.array_equal(b.numpy(), bo)
     with pytest.raises(deco_amd._lib.DecoError):
         A.block_reorder(bs, "1" * bs, co)
+
+
+@pytest.mark.parametrize("cfg,shape", [(1, (2, 2)), (2, (3, 3)), (1, "prow"), (1, "tril"), (2, "tril")])
+def test_block_f32_product(gpu, cfg, shape):
+    """FP32 block-shadow products against the FP64 CSR product: full cell blocks, the pressure rows
+    (1 x bs blocks), and lower-triangular patterns whose diagonal blocks are partly empty (the
+    FSAI factor shape)."""
+    g = gen_system(cfg)
+    bs = g["bs"]
+    S = to_mat(g).to_scipy()
+    if shape == "prow":
+        S = S[::bs].tocsr()
+        R, C = 1, bs
+    elif shape == "tril":
+        S = sp.tril(S).tocsr()
+        R, C = bs, bs
+    else:
+        R, C = shape
+    S.sort_indices()
+    A = deco_amd.CSR.from_scipy(S)
+    x = np.random.default_rng(3).uniform(-1, 1, S.shape[1]).astype(np.float32)
+    y = A.apply_f32(x, block=(R, C)).numpy().astype(np.float64)
+    ref = S @ x.astype(np.float64)
+    assert rel(y, ref) <= 2e-6
+    # the CSR FP32 product agrees as well
+    assert rel(A.apply_f32(x).numpy().astype(np.float64), ref) <= 2e-6
+    with pytest.raises(deco_amd._lib.DecoError):
+        A.apply_f32(x, block=(R, 5))
