@@ -516,28 +516,77 @@ static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, cons
     int j = 0, k = 0;
     bool have_w = false;   // step j's operator application is already queued
     double rn1 = -1.0, rn2 = -1.0;   // the last two residual estimates of this cycle
+    // one reduction of step j: s = Q^T u, |u|^2 (and z = Q^T w, u.w with w); the update sweep
+    // is queued behind it when w is there. Leaves the values in red after the event.
+    auto reduce_step = [&](const double* uu, const double* wp, double* un) {
+      const int nv = wp ? 2 * j + 2 : j + 1;
+      hipLaunchKernelGGL(k_arn_dots, dim3(nbk), dim3(kArnBlock), sizeof(double) * 4 * (2 * j + 2), stream(), n, j,
+                         (const double*)V.p, uu, wp, part.p, nbk);
+      DECO_CHECK_LAUNCH();
+      hipLaunchKernelGGL(k_arn_reduce, dim3((nv + 3) / 4), dim3(256), 0, stream(), nv, nbk, (const double*)part.p,
+                         redd.p);
+      DECO_CHECK_LAUNCH();
+      if (wp) {   // queued before the host decision; unused if the cycle ends here
+        hipLaunchKernelGGL(k_arn_update, dim3(nbk), dim3(kArnBlock), 0, stream(), n, j, (const double*)V.p, uu, wp,
+                           (const double*)redd.p, V.p + (size_t)j * n, un);
+        DECO_CHECK_LAUNCH();
+      }
+      DECO_HIP(hipMemcpyAsync(red, redd.p, sizeof(double) * nv, hipMemcpyDeviceToHost, stream()));
+      DECO_HIP(hipEventRecord(ev.e, stream()));
+    };
+    // finalise column j-1 with u_j = Q s + rho v_j and rotate it (rgmres.tpp:80-93); true when
+    // the cycle ends here
+    auto finalize = [&](double rho, bool last) -> bool {
+      for (int i = 0; i < j; ++i) Hr[i + (size_t)(j - 1) * ld] += red[i];
+      Hr[j + (size_t)(j - 1) * ld] = rho;
+      double* col = &Rm[(size_t)(j - 1) * ld];
+      for (int i = 0; i <= j; ++i) col[i] = Hr[i + (size_t)(j - 1) * ld];
+      for (int i = 0; i <= j - 2; ++i) apply_givens(cs[i], sn[i], col[i], col[i + 1]);
+      comp_givens(col[j - 1], col[j], cs[j - 1], sn[j - 1]);
+      apply_givens(cs[j - 1], sn[j - 1], col[j - 1], col[j]);
+      g[j] = 0.0;
+      apply_givens(cs[j - 1], sn[j - 1], g[j - 1], g[j]);
+      ++iter;
+      k = j;
+      rnorm = std::fabs(g[j]);
+      rn2 = rn1;
+      rn1 = rnorm;
+      if (rnorm < absTol) { flag = 0; return true; }
+      return last || !(rho > 0.0);
+    };
+    auto rho_of = [&](bool with_w, double& gam) {
+      double rho;
+      gam = 0.0;
+      if (with_w) arn_scalars(j, red, rho, gam);
+      else {
+        double ss = 0.0;
+        for (int i = 0; i < j; ++i) ss += red[i] * red[i];
+        const double d = red[j] - ss;
+        rho = std::sqrt(d > 0.0 ? d : 0.0);
+      }
+      return rho;
+    };
     while (true) {
       const bool last = (j == m) || (it0 + j >= itmax);
+      double* un = bufs[j & 1];
+      double rho, gam;
+      // When the last two estimates extrapolate below the tolerance at this step, column j-1 is
+      // finalised from Q^T u_j and |u_j|^2 alone first, and the operator is applied to u_j only
+      // if the solve goes on (the values of s and |u_j|^2 do not depend on w).
+      bool finalized = false;
+      if (!last && !have_w && j > 0 && rn1 > 0.0 && rn2 > 0.0 && rn1 * (rn1 / rn2) < absTol) {
+        reduce_step(u, nullptr, un);
+        DECO_HIP(hipEventSynchronize(ev.e));
+        rho = rho_of(false, gam);
+        if (finalize(rho, false)) break;
+        finalized = true;
+      }
       const double* wp = nullptr;
       if (!last) {
         if (!have_w) apply_op(j, u);
         wp = w.p;
       }
-      const int nv = wp ? 2 * j + 2 : j + 1;
-      hipLaunchKernelGGL(k_arn_dots, dim3(nbk), dim3(kArnBlock), sizeof(double) * 4 * (2 * j + 2), stream(), n, j,
-                         (const double*)V.p, (const double*)u, wp, part.p, nbk);
-      DECO_CHECK_LAUNCH();
-      hipLaunchKernelGGL(k_arn_reduce, dim3((nv + 3) / 4), dim3(256), 0, stream(), nv, nbk, (const double*)part.p,
-                         redd.p);
-      DECO_CHECK_LAUNCH();
-      double* un = bufs[j & 1];
-      if (wp) {   // queued before the host decision; unused if the cycle ends here
-        hipLaunchKernelGGL(k_arn_update, dim3(nbk), dim3(kArnBlock), 0, stream(), n, j, (const double*)V.p,
-                           (const double*)u, wp, (const double*)redd.p, V.p + (size_t)j * n, un);
-        DECO_CHECK_LAUNCH();
-      }
-      DECO_HIP(hipMemcpyAsync(red, redd.p, sizeof(double) * nv, hipMemcpyDeviceToHost, stream()));
-      DECO_HIP(hipEventRecord(ev.e, stream()));
+      reduce_step(u, wp, un);
       // Queue step j+1's operator application on u_{j+1} before the host reads step j, so the
       // device works while the host rotates the Hessenberg column; if the solve stops at step j
       // this application is not used (stream order keeps every later use of w, z after it).
@@ -547,38 +596,15 @@ static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, cons
       have_w = wp && !next_last && !likely_stop;
       if (have_w) apply_op(j + 1, un);
       DECO_HIP(hipEventSynchronize(ev.e));
-      double rho, gam = 0.0;
-      if (wp) arn_scalars(j, red, rho, gam);
-      else {
-        double ss = 0.0;
-        for (int i = 0; i < j; ++i) ss += red[i] * red[i];
-        const double d = red[j] - ss;
-        rho = std::sqrt(d > 0.0 ? d : 0.0);
-      }
+      rho = rho_of(wp != nullptr, gam);
       if (j == 0) {
         g[0] = rho;
         rnorm = std::fabs(g[0]);
         rn1 = rnorm;
         if (rnorm < absTol) { flag = 0; break; }
         if (last || !(rho > 0.0)) break;
-      } else {
-        // finalise column j-1 and rotate it (rgmres.tpp:80-93)
-        for (int i = 0; i < j; ++i) Hr[i + (size_t)(j - 1) * ld] += red[i];
-        Hr[j + (size_t)(j - 1) * ld] = rho;
-        double* col = &Rm[(size_t)(j - 1) * ld];
-        for (int i = 0; i <= j; ++i) col[i] = Hr[i + (size_t)(j - 1) * ld];
-        for (int i = 0; i <= j - 2; ++i) apply_givens(cs[i], sn[i], col[i], col[i + 1]);
-        comp_givens(col[j - 1], col[j], cs[j - 1], sn[j - 1]);
-        apply_givens(cs[j - 1], sn[j - 1], col[j - 1], col[j]);
-        g[j] = 0.0;
-        apply_givens(cs[j - 1], sn[j - 1], g[j - 1], g[j]);
-        ++iter;
-        k = j;
-        rnorm = std::fabs(g[j]);
-        rn2 = rn1;
-        rn1 = rnorm;
-        if (rnorm < absTol) { flag = 0; break; }
-        if (last || !(rho > 0.0)) break;
+      } else if (!finalized) {
+        if (finalize(rho, last)) break;
       }
       // u_j = Q s + rho v_j: column j of T, provisional Hessenberg column j
       if (flex) {

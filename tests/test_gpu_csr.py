@@ -137,3 +137,25 @@ def test_block_f32_product(gpu, cfg, shape):
     assert rel(A.apply_f32(x).numpy().astype(np.float64), ref) <= 2e-6
     with pytest.raises(deco_amd._lib.DecoError):
         A.apply_f32(x, block=(R, 5))
+
+
+def test_spmv_wide_column_offsets(gpu):
+    """Rows coupling to columns far from the diagonal (60000 rows away) and near ones only: the
+    FP64 product equals the oracle bit for bit, the FP32 one to FP32 accuracy."""
+    n = 70000
+    rng = np.random.default_rng(11)
+    for far in (False, True):
+        rows = np.repeat(np.arange(n), 3)
+        i = np.arange(n)
+        cols = np.stack([i, np.minimum(i + 1, n - 1), np.minimum(i + 7, n - 1)], 1).ravel()
+        if far:
+            cols[3 * 5 + 2] = 60005            # row 5 couples to a column 60000 rows away
+        S = sp.csr_matrix((rng.uniform(-1, 1, 3 * n), (rows, cols)), shape=(n, n))
+        S.sum_duplicates()
+        S.sort_indices()
+        A = deco_amd.CSR.from_scipy(S)
+        M = oracle.Mat.from_scipy(S)
+        x = rng.uniform(-1, 1, n)
+        assert np.array_equal(A.apply(x).numpy(), oracle.spmv(M, x))
+        y32 = A.apply_f32(x.astype(np.float32)).numpy().astype(np.float64)
+        assert rel(y32, S @ x) <= 2e-6
