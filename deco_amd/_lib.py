@@ -124,6 +124,12 @@ SIGNATURES = {
     "deco_amg_interp_host": (i, [C.POINTER(CprParams), i, vp, vp, vp, vp, ip, ip, pp, pp, pp]),
     "deco_host_free": (None, [vp]),
     "deco_workspace_release": (i, []),
+    "deco_profile_enable": (i, [i]),
+    "deco_profile_reset": (i, []),
+    "deco_profile_start": (i, [C.c_char_p]),
+    "deco_profile_stop": (i, []),
+    "deco_profile_time": (i, [C.c_char_p, dp]),
+    "deco_profile_report": (i, [C.c_char_p, sz, C.POINTER(C.c_size_t)]),
     "deco_solve": (i, [C.POINTER(SolParams), vp, vp, vp, vp, C.POINTER(SolInfo)]),
     "deco_solve_transpose": (i, [C.POINTER(SolParams), vp, vp, vp, vp, C.POINTER(SolInfo)]),
     "deco_grad_A_pattern": (i, [vp, vp, vp, vp]),

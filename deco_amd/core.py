@@ -537,6 +537,39 @@ class SOL:
         return self.x
 
 
+# ---------------------------------------------------------------------------- profiler ---
+def profile_enable(on=True):
+    """Phase timer tree under the reference's labels (PROFILE, core/profiler.hpp); device waits
+    at every phase boundary while enabled."""
+    check(lib().deco_profile_enable(1 if on else 0))
+
+
+def profile_reset():
+    check(lib().deco_profile_reset())
+
+
+def profile_start(name):
+    check(lib().deco_profile_start(name.encode()))
+
+
+def profile_stop():
+    check(lib().deco_profile_stop())
+
+
+def profile_time(name):
+    t = C.c_double()
+    check(lib().deco_profile_time(name.encode(), C.byref(t)))
+    return t.value
+
+
+def profile_report():
+    need = C.c_size_t()
+    check(lib().deco_profile_report(None, 0, C.byref(need)))
+    buf = C.create_string_buffer(need.value)
+    check(lib().deco_profile_report(buf, need.value, None))
+    return buf.value.decode()
+
+
 def grad_A_pattern(A, lam, x):
     """dL/dA_ij = -lambda_i x_j on A's pattern (device)."""
     ld, xd = as_dev(lam), as_dev(x)

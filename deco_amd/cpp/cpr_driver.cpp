@@ -23,6 +23,7 @@ int main(int argc, char* argv[]) {
   double* b = nullptr;
   std::string src = argc > 2 ? argv[2] : "";
   auto t0 = Clock::now();
+  profiler.start("IO");                                                       // cpr.cpp:28-39
   if (src.rfind("synthetic:", 0) == 0) {
     deco_gen* g = nullptr;
     deco_facade::check(deco_gen_create(std::atoi(src.c_str() + 10), 0, &g), "deco_gen_create");
@@ -44,17 +45,22 @@ int main(int argc, char* argv[]) {
     A = IO::cpp_read_binary_matrix<int, int, double>(param.get<std::string>("IO.file_A"));
     b = IO::cpp_read_binary_rhs<int, int, double>(param.get<std::string>("IO.file_b"), A);
   }
+  profiler.stop();
   AD::block_reorder(A, b, param);   // cpr.cpp:41
   auto t1 = Clock::now();
   std::printf("matrix size:\tn = %d, m = %d, nz = %d\n", A->n, A->m, A->nz);
   const std::string glo = param.get<std::string>("prec.prec_glo.type");
+  profiler.start("setup & solve");                                            // cpr.cpp:57-79
+  profiler.start("Prec");
   std::shared_ptr<CPRBase<double>> P;
   if (glo == "Jacobi") P = std::make_shared<CPR<int, int, double, Jacobi, hypre_AMG>>(param.get_child("prec"), A, b);
   else if (glo == "FSAI") P = std::make_shared<CPR<int, int, double, FSAI, hypre_AMG>>(param.get_child("prec"), A, b);
   else throw std::invalid_argument("Unknown global prec.prec_glo.type: " + glo);
+  profiler.stop();
   auto t2 = Clock::now();
   int iter, flag;
   double res;
+  profiler.start("Solve");
   {
     SOL<int, int, double> S(param.get_child("sol"), A, P->apply, b, nullptr);
     iter = S.iter;
@@ -62,10 +68,13 @@ int main(int argc, char* argv[]) {
     res = S.res_rel;
     std::printf("solve (device) %.3f ms\n", S.info.time_ms);
   }   // SOL frees b and x, as in the reference
+  profiler.stop();
+  profiler.stop();
   auto t3 = Clock::now();
   std::printf("\n--------------------------------\nProfile of Execution Times (seconds):\n");
   std::printf("└── IO                  %.4fs\n└── Prec                %.4fs\n└── Solve               %.4fs\n",
               secs(t0, t1), secs(t1, t2), secs(t2, t3));
   std::printf("iteration count =  %d\nresidual = %.2e\nflag = %d\n", iter, res, flag);
+  profiler.report();
   return flag;
 }

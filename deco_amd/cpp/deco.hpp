@@ -421,3 +421,31 @@ void block_reorder(std::shared_ptr<CSR<R, C, V>> A, V*& b, const deco_ptree& par
   A->pull();
 }
 }  // namespace AD
+
+// The reference's global profiler (PROFILE profiler(PROFILE::MPI), core/profiler.hpp:178): the
+// same start/stop/report/getTime calls, recorded in the library's phase tree so the caller's
+// phases ("IO", "setup & solve", "Prec", "Solve") and the library's own ("decoupling",
+// "Global precond", "AMG apply", ...) form one tree. The first start() turns timing on.
+namespace deco_facade {
+struct PROFILE {
+  void start(const std::string& name) {
+    deco_facade::check(deco_profile_enable(1), "profile");
+    deco_facade::check(deco_profile_start(name.c_str()), "profile");
+  }
+  void stop() { deco_facade::check(deco_profile_stop(), "profile"); }
+  double getTime(const std::string& name) const {
+    double t = 0.0;
+    deco_facade::check(deco_profile_time(name.c_str(), &t), "profile");
+    return t;
+  }
+  void reset() { deco_facade::check(deco_profile_reset(), "profile"); }
+  void report() const {
+    size_t need = 0;
+    deco_facade::check(deco_profile_report(nullptr, 0, &need), "profile");
+    std::string s(need, '\0');
+    deco_facade::check(deco_profile_report(&s[0], need, nullptr), "profile");
+    std::fputs(s.c_str(), stdout);
+  }
+};
+}  // namespace deco_facade
+inline deco_facade::PROFILE profiler;

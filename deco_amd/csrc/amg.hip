@@ -181,25 +181,38 @@ struct AMG : deco_prec {
         H.r.resize(Al->n + 1);
         H.c.resize(Al->nnz);
         H.v.resize(Al->nnz);
+        ProfPhase ph_dl("download");
         Al->r.download(H.r.data(), Al->n + 1);
         Al->c.download(H.c.data(), Al->nnz);
         Al->v.download(H.v.data(), Al->nnz);
+        ph_dl.end();
         std::vector<char> strong;
+        ProfPhase ph_st("strength");
         amg::strength(H, theta, mrs, strong);
+        ph_st.end();
         std::vector<int> cf;
+        ProfPhase ph_co("coarsening");
         amg::coarsen(H, strong, p.amg_coarse_type, 0x5eed0000ull + (uint64_t)lev, cf);
+        ph_co.end();
         int ncl = 0;
+        ProfPhase ph_in("interpolation");
         amg::HostCSR Ph = amg::interp_extpi(H, strong, cf, p.amg_prol_trunc_fac, p.amg_P_max_elmts, ncl);
+        ph_in.end();
         if (ncl == 0 || ncl == Al->n) break;
         Level lv;
         lv.A = Al;
+        ProfPhase ph_up("upload P");
         lv.cf.upload(cf.data(), cf.size());
         lv.P = csr_new(Ph.n, Ph.m);
         lv.P->nnz = (int)Ph.c.size();
         lv.P->r.upload(Ph.r.data(), Ph.r.size());
         lv.P->c.upload(Ph.c.data(), Ph.c.size());
         lv.P->v.upload(Ph.v.data(), Ph.v.size());
+        ph_up.end();
+        ProfPhase ph_t("Transpose");
         lv.R = csr_transpose(lv.P);
+        ph_t.end();
+        ProfPhase ph_rap("Galerkin RAP");
         deco_csr* AP = spgemm(Al, lv.P);
         deco_csr* RAP = spgemm(lv.R, AP);
         delete AP;
@@ -209,6 +222,7 @@ struct AMG : deco_prec {
         } else {
           Al = RAP;
         }
+        ph_rap.end();
         L.push_back(std::move(lv));
       }
       Ac = Al;
@@ -232,6 +246,7 @@ struct AMG : deco_prec {
       lv.xf.alloc(nl); lv.bf.alloc(nl); lv.tf.alloc(nl); lv.rf.alloc(nl);
     }
     // coarsest: dense FP64 inverse
+    ProfPhase ph_ci("coarse inverse");
     nc = Ac->n;
     amg::HostCSR H;
     H.n = nc;
@@ -344,9 +359,15 @@ struct AMG : deco_prec {
     }
     cycle<T>(0, r, x, tr);
   }
-  void apply(const double* r, double* x) override { run<double>(r, x, false); }
+  void apply(const double* r, double* x) override {
+    ProfPhase ph("AMG apply");                                           // hypre_AMG.hpp:27-35
+    run<double>(r, x, false);
+  }
   void apply_t(const double* r, double* x) override { run<double>(r, x, true); }
-  void apply32(const float* r, float* x) override { run<float>(r, x, false); }
+  void apply32(const float* r, float* x) override {
+    ProfPhase ph("AMG apply");
+    run<float>(r, x, false);
+  }
   void apply32_t(const float* r, float* x) override { run<float>(r, x, true); }
 };
 

@@ -63,6 +63,23 @@ void ensure_init();
 inline hipStream_t stream() { ensure_init(); return ctx().stream; }
 void sync();
 
+// Phase timer tree (profile.hip): no-ops unless enabled with deco_profile_enable.
+void prof_start(const char* name);
+void prof_stop();
+// a phase that ends at end() or, at the latest, when it goes out of scope (also on a throw)
+struct ProfPhase {
+  bool open = true;
+  explicit ProfPhase(const char* name) { prof_start(name); }
+  void end() {
+    if (open) { open = false; prof_stop(); }
+  }
+  ~ProfPhase() {
+    try { end(); } catch (...) {}
+  }
+  ProfPhase(const ProfPhase&) = delete;
+  ProfPhase& operator=(const ProfPhase&) = delete;
+};
+
 // ------------------------------------------------------------------ device buffer ------
 template <class T>
 struct DBuf {

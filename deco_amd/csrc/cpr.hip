@@ -429,10 +429,12 @@ struct CPR : deco_prec {
     flag.zero();
     swapped.zero();
     err.zero();
+    ProfPhase ph_dec("decoupling");                                    // CPR.hpp:47-138
     hipLaunchKernelGGL(k_fix_diag, dim3(gc), dim3(128), 0, stream(), sn, bs, A->r.p, A->c.p, A->v.p, b, flag.p,
                        swapped.p, err.p + 1);
     DECO_CHECK_LAUNCH();
     DBuf<double> w;
+    ProfPhase ph_w("quasiimpes weights");
     if (p.weights != DECO_W_NONE) {
       w.alloc(n);
       if (p.weights == DECO_W_QUASIIMPES_GE)
@@ -444,7 +446,9 @@ struct CPR : deco_prec {
       hipLaunchKernelGGL(k_row_scale, dim3(gr), dim3(256), 0, stream(), n, A->r.p, A->v.p, (const double*)w.p, b);
       DECO_CHECK_LAUNCH();
     }
+    ph_w.end();
     if (p.use_drs) {
+      ProfPhase ph("DRS weights");
       hipLaunchKernelGGL(k_drs, dim3(gc), dim3(128), 0, stream(), sn, bs, A->r.p, A->c.p, A->v.p, p.drs_e_dd,
                          p.drs_e_ps, wd.p);
       DECO_CHECK_LAUNCH();
@@ -455,15 +459,20 @@ struct CPR : deco_prec {
       }
     }
     const unsigned char* wdp = p.use_drs ? wd.p : nullptr;
+    ProfPhase ph_impes("IMPES");
     hipLaunchKernelGGL(k_impes, dim3(gc), dim3(128), 0, stream(), sn, bs, A->r.p, A->v.p, b, wdp, err.p);
     DECO_CHECK_LAUNCH();
     check_err(err.p);
+    ph_impes.end();
     DBuf<double> J;
     if (p.cpd) {
       J.alloc((size_t)sn * bs * bs);
+      ProfPhase ph_cd("CPD decoup");
       hipLaunchKernelGGL(k_cpd_build, dim3(gc), dim3(128), 0, stream(), sn, bs, A->r.p, A->c.p, A->v.p, J.p, err.p);
       DECO_CHECK_LAUNCH();
       check_err(err.p);
+      ph_cd.end();
+      ProfPhase ph_ca("CPD apply");
       hipLaunchKernelGGL(k_cpd_apply, dim3(gc), dim3(128), 0, stream(), sn, bs, A->r.p, A->v.p, b,
                          (const double*)J.p, err.p);
       DECO_CHECK_LAUNCH();
@@ -476,13 +485,33 @@ struct CPR : deco_prec {
     DECO_CHECK_LAUNCH();
     A->invalidate();
     if (A->At) { delete A->At; A->At = nullptr; }
-    if (p.tol_zero != 0.0) csr_filter(A, p.tol_zero);
-    Ap = local_matrix(A, bs, 0, nullptr);
-    if (p.loc_tol_rel > 1e-20) csr_filter_proper(Ap, p.loc_tol_rel);
-    if (p.glo_type == DECO_GLO_FSAI) glo = make_fsai(p, A);
-    else glo = new Jacobi(A, p.glo_block_size);
-    loc = make_amg(p, Ap);
-    if (p.seq == 2) jac = new Jacobi(A, p.glo_block_size);
+    if (p.tol_zero != 0.0) {
+      ProfPhase ph("filter A");
+      csr_filter(A, p.tol_zero);
+    }
+    {
+      ProfPhase ph("get local matrix");
+      Ap = local_matrix(A, bs, 0, nullptr);
+    }
+    if (p.loc_tol_rel > 1e-20) {
+      ProfPhase ph("filter Ap");
+      csr_filter_proper(Ap, p.loc_tol_rel);
+    }
+    ph_dec.end();
+    {
+      ProfPhase ph("Global precond");                                 // CPR.hpp:141-150
+      if (p.glo_type == DECO_GLO_FSAI) glo = make_fsai(p, A);
+      else glo = new Jacobi(A, p.glo_block_size);
+    }
+    {
+      ProfPhase ph("Local  precond");
+      loc = make_amg(p, Ap);
+    }
+    ProfPhase ph_dec2("decoupling");                                  // CPR.hpp:152-170
+    if (p.seq == 2) {
+      ProfPhase ph("Jacobi precond");
+      jac = new Jacobi(A, p.glo_block_size);
+    }
     hipLaunchKernelGGL(k_undo_fix, dim3(gr), dim3(256), 0, stream(), n, A->r.p, A->c.p, A->v.p,
                        (const unsigned char*)flag.p);
     DECO_CHECK_LAUNCH();
@@ -541,13 +570,18 @@ struct CPR : deco_prec {
       return;
     }
     if (seq == 2) jac->apply(r, x); else glo_apply(r, x);
-    pressure_residual((const double*)x, r, rt.p);
+    {
+      ProfPhase ph("CPR_rest");                                          // CPR.hpp:240-296
+      pressure_residual((const double*)x, r, rt.p);
+    }
     loc->apply(rt.p, xp.p);
+    ProfPhase ph_rest("CPR_rest");
     hipLaunchKernelGGL((k_add_pressure<double, double>), dim3(gs()), dim3(256), 0, stream(), sn, bs, x,
                        (const double*)xp.p);
     DECO_CHECK_LAUNCH();
     if (seq == 0) return;
     spmv_res(A, x, r, rt.p);
+    ph_rest.end();
     glo_apply_add(rt.p, x);
   }
 
@@ -578,13 +612,18 @@ struct CPR : deco_prec {
     }
     if (seq == 2) { convert(r, t32.p, n); jac->apply32(t32.p, x32.p); }
     else glo32_from64(r, x32.p);
-    pressure_residual((const float*)x32.p, r, rt32.p);
+    {
+      ProfPhase ph("CPR_rest");
+      pressure_residual((const float*)x32.p, r, rt32.p);
+    }
     loc->apply32(rt32.p, xp32.p);
+    ProfPhase ph_rest("CPR_rest");
     hipLaunchKernelGGL((k_add_pressure<float, float>), dim3(gs()), dim3(256), 0, stream(), sn, bs, x32.p,
                        (const float*)xp32.p);
     DECO_CHECK_LAUNCH();
     if (seq == 0) { convert(x32.p, xo, n); return; }
     residual32(x32.p, r, rt32.p);
+    ph_rest.end();
     if (F) fsai_apply32_addz(F, rt32.p, x32.p, xo);
     else {
       glo->apply32(rt32.p, t32.p);

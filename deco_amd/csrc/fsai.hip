@@ -266,6 +266,7 @@ struct FSAI : deco_prec {
     deco_csr* A = nullptr;
     const deco_csr* Au = A_in;
     if (p.glo_tol_rel != 0.0 || p.glo_tol_abs != 0.0) {
+      ProfPhase ph("filter A");                                          // FSAI.hpp:23-27
       A = csr_clone(A_in);
       if (p.glo_tol_rel != 0.0) csr_filter_proper(A, p.glo_tol_rel);
       if (p.glo_tol_abs != 0.0) csr_filter(A, p.glo_tol_abs);
@@ -291,6 +292,7 @@ struct FSAI : deco_prec {
   ~FSAI() override { cleanup(); }
 
   void build(const deco_cpr_params& p, const deco_csr* A) {
+    ProfPhase ph_sym("symbolic");                                        // FSAI.tpp:13-15
     if (p.glo_power == 1) {
       FL = csr_new(n, n);
       DBuf<int> cnt(n + 1);
@@ -305,6 +307,8 @@ struct FSAI : deco_prec {
     } else {
       fail(DECO_ERR_ARG, "k > 2 is not implemented, k = %d", p.glo_power);
     }
+    ph_sym.end();
+    ProfPhase ph_num("numeric");                                         // FSAI.tpp:28-30
     // FU shares FL's pattern (FSAI.tpp:21-23)
     FU = csr_new(n, n);
     FU->nnz = FL->nnz;
@@ -318,6 +322,7 @@ struct FSAI : deco_prec {
       DECO_HIP(hipMemsetAsync(FU->v.p, 0, sizeof(double) * FL->nnz, stream()));
     }
     // batched local systems, binned by size
+    ProfPhase ph_bin("sort");
     DBuf<int> cnt(4), lists((size_t)4 * n), flags(2);
     cnt.zero();
     int init[2] = {0, INT_MAX};
@@ -329,6 +334,8 @@ struct FSAI : deco_prec {
     copy_d2h(hc, cnt.p, 4);
     copy_d2h(hf, flags.p, 1);
     if (hf[0] > 0) fail(DECO_ERR_UNSUPPORTED, "FSAI local system of size %d exceeds 64", hf[0]);
+    ph_bin.end();
+    ProfPhase ph_ls("linsolve");                                         // gather + LU + scatter
     const int* L = lists.p;
     if (hc[0]) launch<8>(L, hc[0], A, flags.p + 1);
     if (hc[1]) launch<16>(L + n, hc[1], A, flags.p + 1);
@@ -336,14 +343,21 @@ struct FSAI : deco_prec {
     if (hc[3]) launch<64>(L + 3 * (size_t)n, hc[3], A, flags.p + 1);
     copy_d2h(hf + 1, flags.p + 1, 1);
     if (hf[1] != INT_MAX) fail(DECO_ERR_SINGULAR, "singular FSAI local system at row %d", hf[1]);
+    ph_ls.end();
+    ProfPhase ph_dg("diagonal");
     hipLaunchKernelGGL(k_fsai_unit_diag, dim3(grid_for(n, 256)), dim3(256), 0, stream(), n, (const int*)FL->r.p,
                        FL->v.p, FU->v.p);
     DECO_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_fsai_diag_scale, dim3(grid_for(n, 128)), dim3(128), 0, stream(), n, A->r.p, A->c.p, A->v.p,
-                       (const int*)FL->r.p, (const int*)FL->c.p, FL->v.p, FU->v.p);
-    DECO_CHECK_LAUNCH();
+    ph_dg.end();
+    {
+      ProfPhase ph("Jacobi_scale");
+      hipLaunchKernelGGL(k_fsai_diag_scale, dim3(grid_for(n, 128)), dim3(128), 0, stream(), n, A->r.p, A->c.p,
+                         A->v.p, (const int*)FL->r.p, (const int*)FL->c.p, FL->v.p, FU->v.p);
+      DECO_CHECK_LAUNCH();
+    }
     FL->invalidate();
     FU->invalidate();
+    ProfPhase ph_t("Transpose");
     FUt = csr_transpose(FU);
   }
 
@@ -356,11 +370,13 @@ struct FSAI : deco_prec {
   }
 
   void apply(const double* r, double* x) override {
+    ProfPhase ph("FSAI apply");                                          // FSAI.hpp:42-45
     spmv(FL, r, temp.p);
     spmv(FUt, temp.p, x);
   }
   // x += M r
   void apply_add(const double* r, double* x) {
+    ProfPhase ph("FSAI apply");
     spmv(FL, r, temp.p);
     spmv_stream<EPI_ACC, double, double, double, double>(FUt, FUt->v.p, temp.p, x, (const double*)nullptr);
   }
@@ -387,12 +403,14 @@ struct FSAI : deco_prec {
   // FP32 factors; input of any precision, output f32 (or f64 with EPI_ADDZ from the caller)
   template <class TI>
   void apply32_in(const TI* r, float* x) {
+    ProfPhase ph("FSAI apply");
     f32_product(FL, r, EpiStd<EPI_SET, float, float>{temp32.p, nullptr});
     f32_product(FUt, (const float*)temp32.p, EpiStd<EPI_SET, float, float>{x, nullptr});
   }
   // xo (f64) = z (f32) + M r
   template <class TI>
   void apply32_addz(const TI* r, const float* z, double* xo) {
+    ProfPhase ph("FSAI apply");
     f32_product(FL, r, EpiStd<EPI_SET, float, float>{temp32.p, nullptr});
     f32_product(FUt, (const float*)temp32.p, EpiStd<EPI_ADDZ, double, float>{xo, z});
   }

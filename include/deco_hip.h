@@ -259,6 +259,19 @@ void deco_host_free(void* ptr);
  * use_x0). M may be NULL (identity). */
 int deco_solve(const deco_sol_params* sp, const deco_csr* A, deco_prec* M,
                const double* b_dev, double* x_dev, deco_sol_info* info);
+/* Phase timer tree in the reference's format (PROFILE, core/profiler.hpp:1-178): off by default.
+ * When enabled, the library times its phases under the reference's labels ("decoupling",
+ * "quasiimpes weights", "IMPES", "CPD decoup", "CPD apply", "filter A", "get local matrix",
+ * "Global precond" with "symbolic" / "numeric", "Local  precond", "FSAI apply", "AMG apply",
+ * "CPR_rest", ...) and waits for the device at every boundary, so profiled runs are slower.
+ * Callers add their own phases with start/stop (cpr.cpp:28,57-58,75). report writes the tree as
+ * PROFILE::report prints it (needed = bytes including the terminator). */
+int deco_profile_enable(int on);
+int deco_profile_reset(void);
+int deco_profile_start(const char* name);
+int deco_profile_stop(void);
+int deco_profile_time(const char* name, double* seconds);
+int deco_profile_report(char* buf, size_t len, size_t* needed);
 /* The solvers keep their work vectors (Krylov basis, flexible directions, partial sums) between
  * calls and grow them on demand; this frees them. No reference counterpart (the reference
  * allocates per solve, solver/rgmres.tpp:18-29). */

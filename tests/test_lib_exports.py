@@ -1,6 +1,8 @@
 """The C-ABI library loads on a machine without a GPU and exports every declared symbol."""
 import ctypes
 
+import pytest
+
 from deco_amd import _lib
 
 
@@ -42,3 +44,34 @@ def test_struct_layouts_match_c(tmp_path):
         want.append(ctypes.sizeof(py))
         want += [getattr(py, f).offset for f, _ in py._fields_]
     assert out == want
+
+
+def test_profile_tree_report_format():
+    """The phase tree re-enters phases by name and prints PROFILE::report's layout
+    (core/profiler.hpp:89-112); no device is touched while none is initialised."""
+    import deco_amd
+    deco_amd.profile_reset()
+    deco_amd.profile_enable(True)
+    for _ in range(2):
+        deco_amd.profile_start("setup & solve")
+        deco_amd.profile_start("Prec")
+        deco_amd.profile_stop()
+        deco_amd.profile_start("Solve")
+        deco_amd.profile_start("AMG apply")
+        deco_amd.profile_stop()
+        deco_amd.profile_stop()
+        deco_amd.profile_stop()
+    deco_amd.profile_enable(False)
+    rep = deco_amd.profile_report()
+    lines = [l for l in rep.splitlines() if "└" in l]
+    assert rep.startswith("\n--------------------------------\nProfile of Execution Times (seconds):\n")
+    assert [l.strip().split()[1] for l in lines] == ["setup", "Prec", "Solve", "AMG"]
+    assert lines[0].startswith("└── setup & solve") and "100.00%" in lines[0]
+    assert lines[1].startswith("    └── Prec") and lines[3].startswith("        └── AMG apply")
+    t = deco_amd.profile_time("Solve")
+    assert t >= deco_amd.profile_time("AMG apply") >= 0.0 and deco_amd.profile_time("nothing") == 0.0
+    deco_amd.profile_reset()
+    with pytest.raises(deco_amd._lib.DecoError):
+        deco_amd.profile_enable(True)
+        deco_amd.profile_stop()
+    deco_amd.profile_enable(False)
