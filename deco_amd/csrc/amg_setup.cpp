@@ -45,55 +45,75 @@ void strength(const HostCSR& A, double theta, double max_row_sum, std::vector<ch
   }
 }
 
-// strong dependencies S (rows) and strong influences S^T as CSR adjacency
+// strong dependencies S (rows) and strong influences S^T as CSR adjacency; the S^T lists are
+// filled in parallel and then sorted, which gives them the ascending order of a serial fill
 static void adjacency(const HostCSR& A, const std::vector<char>& strong, std::vector<int>& sr, std::vector<int>& sc,
                       std::vector<int>& tr, std::vector<int>& tc) {
   const int n = A.n;
   sr.assign(n + 1, 0);
   tr.assign(n + 1, 0);
-  for (int i = 0; i < n; ++i)
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; ++i) {
+    int cnt = 0;
     for (int k = A.r[i]; k < A.r[i + 1]; ++k)
-      if (strong[k]) { sr[i + 1]++; tr[A.c[k] + 1]++; }
+      if (strong[k]) {
+        ++cnt;
+#pragma omp atomic
+        tr[A.c[k] + 1]++;
+      }
+    sr[i + 1] = cnt;
+  }
   for (int i = 0; i < n; ++i) { sr[i + 1] += sr[i]; tr[i + 1] += tr[i]; }
   sc.resize(sr[n]);
   tc.resize(tr[n]);
   std::vector<int> cur(tr.begin(), tr.end() - 1);
-  int o = 0;
-  for (int i = 0; i < n; ++i)
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; ++i) {
+    int o = sr[i];
     for (int k = A.r[i]; k < A.r[i + 1]; ++k)
       if (strong[k]) {
         sc[o++] = A.c[k];
-        tc[cur[A.c[k]]++] = i;
+        int slot;
+#pragma omp atomic capture
+        slot = cur[A.c[k]]++;
+        tc[slot] = i;
       }
+  }
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < n; ++j) std::sort(tc.begin() + tr[j], tc.begin() + tr[j + 1]);
 }
 
-// Ruge-Stueben first pass with bucket lists (max measure first, LIFO inside a bucket).
+// Ruge-Stueben first pass with bucket lists (max measure first, LIFO inside a bucket). The
+// measure, the list links and the C/F state of a point share one 16-byte record: the pass is
+// bound by random accesses to the neighbours of each new C point, one cache line each.
 static void coarsen_rs(int n, const std::vector<int>& sr, const std::vector<int>& sc, const std::vector<int>& tr,
-                       const std::vector<int>& tc, std::vector<int>& cf) {
-  cf.assign(n, 0);
-  std::vector<int> lam(n);
+                       const std::vector<int>& tc, std::vector<int>& cf_out) {
+  struct Pt {
+    int lam, nxt, prv, cf;
+  };
+  std::vector<Pt> P(n);
   int maxdeg = 0;
   for (int i = 0; i < n; ++i) {
-    lam[i] = tr[i + 1] - tr[i];
-    maxdeg = std::max(maxdeg, lam[i]);
+    P[i] = Pt{tr[i + 1] - tr[i], -1, -1, 0};
+    maxdeg = std::max(maxdeg, P[i].lam);
   }
   const int nb = 2 * maxdeg + 2;
-  std::vector<int> head(nb, -1), nxt(n, -1), prv(n, -1);
+  std::vector<int> head(nb, -1);
   auto insert = [&](int i) {
-    const int b = lam[i];
-    nxt[i] = head[b];
-    prv[i] = -1;
-    if (head[b] >= 0) prv[head[b]] = i;
-    head[b] = i;
+    Pt& p = P[i];
+    p.nxt = head[p.lam];
+    p.prv = -1;
+    if (p.nxt >= 0) P[p.nxt].prv = i;
+    head[p.lam] = i;
   };
   auto remove = [&](int i) {
-    const int b = lam[i];
-    if (prv[i] >= 0) nxt[prv[i]] = nxt[i]; else head[b] = nxt[i];
-    if (nxt[i] >= 0) prv[nxt[i]] = prv[i];
-    nxt[i] = prv[i] = -1;
+    Pt& p = P[i];
+    if (p.prv >= 0) P[p.prv].nxt = p.nxt; else head[p.lam] = p.nxt;
+    if (p.nxt >= 0) P[p.nxt].prv = p.prv;
+    p.nxt = p.prv = -1;
   };
   for (int i = 0; i < n; ++i) {
-    if (lam[i] == 0) cf[i] = -1;  // influences nobody (or isolated): F
+    if (P[i].lam == 0) P[i].cf = -1;  // influences nobody (or isolated): F
     else insert(i);
   }
   int top = nb - 1;
@@ -102,31 +122,31 @@ static void coarsen_rs(int n, const std::vector<int>& sr, const std::vector<int>
     if (top <= 0) break;
     const int i = head[top];
     remove(i);
-    cf[i] = 1;
+    P[i].cf = 1;
     for (int k = tr[i]; k < tr[i + 1]; ++k) {  // points depending on i become F
       const int j = tc[k];
-      if (cf[j] != 0) continue;
+      if (P[j].cf != 0) continue;
       remove(j);
-      cf[j] = -1;
+      P[j].cf = -1;
       for (int q = sr[j]; q < sr[j + 1]; ++q) {  // their influencers gain weight
         const int m = sc[q];
-        if (cf[m] != 0) continue;
+        if (P[m].cf != 0) continue;
         remove(m);
-        lam[m] = std::min(lam[m] + 1, nb - 1);
+        P[m].lam = std::min(P[m].lam + 1, nb - 1);
         insert(m);
-        if (lam[m] > top) top = lam[m];
+        if (P[m].lam > top) top = P[m].lam;
       }
     }
     for (int q = sr[i]; q < sr[i + 1]; ++q) {  // points i depends on lose weight
       const int m = sc[q];
-      if (cf[m] != 0) continue;
+      if (P[m].cf != 0) continue;
       remove(m);
-      lam[m] = std::max(lam[m] - 1, 0);
+      P[m].lam = std::max(P[m].lam - 1, 0);
       insert(m);
     }
   }
-  for (int i = 0; i < n; ++i)
-    if (cf[i] == 0) cf[i] = -1;
+  cf_out.resize(n);
+  for (int i = 0; i < n; ++i) cf_out[i] = P[i].cf == 0 ? -1 : P[i].cf;
 }
 
 // PMIS: measure |S^T_i| + U[0,1) from a seeded hash; independent sets of local maxima.
