@@ -113,13 +113,18 @@ void launch_spmv(int G, int nout, int rs, const int* r, const int* c, const VT* 
 
 // ------------------------------------------------------------------ CSR-stream SpMV ------
 // Chunk k holds the rows whose first entry lies in [k*kStreamCap, (k+1)*kStreamCap). A block
-// streams the contiguous entry range of its chunk(s) into LDS: every lane issues 6 (FP64) or 12
-// (FP32) independent (column, value) loads and then the x gathers, so many bytes are in flight
-// per wave; the products are then summed per row in column order (the same order as alg::spmv,
+// streams the contiguous entry range of its chunk into LDS: every lane issues 12 (FP64, 256
+// lanes) or 6 (FP32, 512 lanes) independent (column, value) loads, then all of its x gathers,
+// then the products. The tile has no per-entry branches (indices past the range are clamped
+// and their products dropped), so the gathers are in flight together instead of one at a
+// time. The products are summed per row in column order (the same order as alg::spmv,
 // core/alg.hpp:59-61). Ranges that exceed the LDS tile (rows over 129 entries) fall back to a
 // block reduction per row.
 constexpr int kStreamCap = 3072;                 // target entries per chunk
-constexpr int kStreamBlock = 512;
+template <class VT>
+constexpr int stream_block() {
+  return sizeof(VT) == 8 ? 256 : 512;
+}
 
 // Epilogue functors: called once per row with the row sum (accumulator type AT).
 template <int EPI, class YT, class ZT>
@@ -153,13 +158,6 @@ struct EpiStrideRes {
   __device__ __forceinline__ void operator()(int o, AT acc) const { y[o] = (YT)((AT)z[(size_t)o * zs] - acc); }
 };
 
-// Chunks per block (one: two chunks per block for FP32, 12 loads per lane, measured 3-7% slower
-// on the config-3 operators and slower still on small AMG levels with half the blocks).
-template <class VT, class XT>
-constexpr int stream_cpb() {
-  return 1;
-}
-
 // x access of the stream kernel: a plain vector, or a functor that computes x_j (fused steps)
 template <class XT>
 struct PtrGather {
@@ -169,55 +167,58 @@ struct PtrGather {
 };
 
 template <class VT, class XG, class E>
-__global__ __launch_bounds__(kStreamBlock) void k_spmv_stream(int nch, const int* __restrict__ rb,
-                                                             const int* __restrict__ r, const int* __restrict__ c,
-                                                             const VT* __restrict__ v, XG x, E epi) {
+__global__ __launch_bounds__(stream_block<VT>()) void k_spmv_stream(int nch, const int* __restrict__ rb,
+                                                                   const int* __restrict__ r,
+                                                                   const int* __restrict__ c,
+                                                                   const VT* __restrict__ v, XG x, E epi) {
   using XT = typename XG::type;
   using AT = acc_t<VT, XT>;
-  constexpr int CPB = stream_cpb<VT, XT>();
-  constexpr int CAP = CPB * kStreamCap;
-  constexpr int LDSN = CAP + 128;      // a group of CPB chunks holds <= CAP + longest row - 1
-  constexpr int U = CAP / kStreamBlock;
+  constexpr int BLK = stream_block<VT>();
+  constexpr int LDSN = kStreamCap + 128;   // a chunk holds <= cap + longest row - 1
+  constexpr int U = kStreamCap / BLK;
   __shared__ AT prod[LDSN];
-  const int grp = xcd_block(blockIdx.x, gridDim.x);
-  const int r0 = rb[grp * CPB], r1 = rb[min(grp * CPB + CPB, nch)];
+  const int ch = xcd_block(blockIdx.x, gridDim.x);
+  const int r0 = rb[ch], r1 = rb[min(ch + 1, nch)];
   if (r0 >= r1) return;
   const int j0 = r[r0], j1 = r[r1];
   const int cnt = j1 - j0;
   const int tid = threadIdx.x;
   if (cnt <= LDSN) {
-    // first tile: U independent loads per lane
-    int cc[U];
-    VT vv[U];
+    if (cnt > 0) {
+      const int jl = j1 - 1;
+      int cc[U];
+      VT vv[U];
+      XT xx[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int t = tid + u * kStreamBlock;
-      if (t < cnt) {
-        cc[u] = __builtin_nontemporal_load(c + j0 + t);
-        vv[u] = __builtin_nontemporal_load(v + j0 + t);
+      for (int u = 0; u < U; ++u) {
+        const int j = min(j0 + tid + u * BLK, jl);
+        cc[u] = __builtin_nontemporal_load(c + j);
+        vv[u] = __builtin_nontemporal_load(v + j);
       }
-    }
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int t = tid + u * kStreamBlock;
-      if (t < cnt) prod[t] = (AT)vv[u] * (AT)x(cc[u]);
+      for (int u = 0; u < U; ++u) xx[u] = x(cc[u]);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int t = tid + u * BLK;
+        if (t < cnt) prod[t] = (AT)vv[u] * (AT)xx[u];
+      }
+      for (int t = tid + kStreamCap; t < cnt; t += BLK)
+        prod[t] = (AT)__builtin_nontemporal_load(v + j0 + t) * (AT)x(__builtin_nontemporal_load(c + j0 + t));
     }
-    for (int t = tid + CAP; t < cnt; t += kStreamBlock)
-      prod[t] = (AT)__builtin_nontemporal_load(v + j0 + t) * (AT)x(__builtin_nontemporal_load(c + j0 + t));
     __syncthreads();
-    for (int row = r0 + tid; row < r1; row += kStreamBlock) {
+    for (int row = r0 + tid; row < r1; row += BLK) {
       const int b = r[row] - j0, e = r[row + 1] - j0;
       AT acc = AT(0);
       for (int k = b; k < e; ++k) acc += prod[k];
       epi(row, acc);
     }
   } else {
-    __shared__ AT red[kStreamBlock / 64];
+    __shared__ AT red[BLK / 64];
     for (int row = r0; row < r1; ++row) {
       const int b = r[row], e = r[row + 1];
       AT acc = AT(0);
-      for (int j = b + tid; j < e; j += kStreamBlock) acc += (AT)v[j] * (AT)x(c[j]);
-      acc = block_sum<kStreamBlock>(acc, red);
+      for (int j = b + tid; j < e; j += BLK) acc += (AT)v[j] * (AT)x(c[j]);
+      acc = block_sum<BLK>(acc, red);
       if (tid == 0) epi(row, acc);
     }
   }
@@ -227,9 +228,8 @@ template <class VT, class XG, class E>
 void spmv_stream_g(const deco_csr* A, const VT* v, XG x, E epi) {
   if (A->n <= 0) return;
   const int* rb = A->row_chunks();
-  constexpr int CPB = stream_cpb<VT, typename XG::type>();
-  hipLaunchKernelGGL((k_spmv_stream<VT, XG, E>), dim3((A->nchunks + CPB - 1) / CPB), dim3(kStreamBlock), 0, stream(),
-                     A->nchunks, rb, A->r.p, A->c.p, v, x, epi);
+  hipLaunchKernelGGL((k_spmv_stream<VT, XG, E>), dim3(A->nchunks), dim3(stream_block<VT>()), 0, stream(), A->nchunks,
+                     rb, A->r.p, A->c.p, v, x, epi);
   DECO_CHECK_LAUNCH();
 }
 
@@ -302,28 +302,30 @@ __global__ __launch_bounds__(kBsrBlock) void k_bsr_stream(const int* __restrict_
   const int j0 = br[r0], cnt = br[r1] - j0;
   const int tid = threadIdx.x;
   if (cnt <= LDSB) {
-    int cc[kBsrUnroll];
-    BsrVals<R * C> vv[kBsrUnroll];
+    if (cnt > 0) {   // branch-free tile: clamped indices, every gather in flight before the products
+      const int jl = j0 + cnt - 1;
+      int cc[kBsrUnroll];
+      BsrVals<R * C> vv[kBsrUnroll];
+      XT xv[kBsrUnroll][C];
 #pragma unroll
-    for (int u = 0; u < kBsrUnroll; ++u) {
-      const int t = tid + u * kBsrBlock;
-      if (t < cnt) {
-        cc[u] = __builtin_nontemporal_load(bc + j0 + t);
-        vv[u] = bsr_load<R, C>(bv, j0 + t);
+      for (int u = 0; u < kBsrUnroll; ++u) {
+        const int j = min(j0 + tid + u * kBsrBlock, jl);
+        cc[u] = __builtin_nontemporal_load(bc + j);
+        vv[u] = bsr_load<R, C>(bv, j);
       }
-    }
 #pragma unroll
-    for (int u = 0; u < kBsrUnroll; ++u) {
-      const int t = tid + u * kBsrBlock;
-      if (t < cnt) {
-        XT xv[C];
-        bsr_gather<C>(x, cc[u], xv);
+      for (int u = 0; u < kBsrUnroll; ++u) bsr_gather<C>(x, cc[u], xv[u]);
 #pragma unroll
-        for (int rr = 0; rr < R; ++rr) {
-          AT p = (AT)vv[u].a[rr * C] * (AT)xv[0];
+      for (int u = 0; u < kBsrUnroll; ++u) {
+        const int t = tid + u * kBsrBlock;
+        if (t < cnt) {
 #pragma unroll
-          for (int q = 1; q < C; ++q) p += (AT)vv[u].a[rr * C + q] * (AT)xv[q];
-          prod[t * R + rr] = p;
+          for (int rr = 0; rr < R; ++rr) {
+            AT p = (AT)vv[u].a[rr * C] * (AT)xv[u][0];
+#pragma unroll
+            for (int q = 1; q < C; ++q) p += (AT)vv[u].a[rr * C + q] * (AT)xv[u][q];
+            prod[t * R + rr] = p;
+          }
         }
       }
     }
