@@ -257,35 +257,40 @@ def main():
     })
 
     # ---------------------------------------------------------------- adjoint
+    # transposed CPR solve (W A)^T mu = w, lambda = W^T mu, dL/dA on A's pattern; and, because
+    # iteration counts depend on the right-hand side (w ~ N(0,1) is harder than b = A x*), the
+    # forward solve with the same w as the like-for-like cost. Both preconditioner precisions.
     if not args.no_adjoint and method == "rgmres":
         w = deco_amd.DeviceArray.from_numpy(np.random.default_rng(1004).standard_normal(n))
-        St = deco_amd.SOL(sol, A, M, w, transpose=True)          # warm-up (builds A^T and the transposes)
-        deco_amd.lib().deco_sync()
-        ts = []
-        for _ in range(args.steps):
-            t1 = time.perf_counter()
-            St = deco_amd.SOL(sol, A, M, w, transpose=True)      # (W A)^T mu = w
-            lam = M.transform_transpose(St.x)                    # lambda = W^T mu
-            dA = deco_amd.grad_A_pattern(A, lam, S64.x)          # dL/dA_ij = -lambda_i x_j
+        fwd = {deco_amd.PREC_FP64: r64, deco_amd.PREC_FP32: r32}
+        result["adjoint"] = {}
+        for prec, key in ((deco_amd.PREC_FP64, "fp64"), (deco_amd.PREC_FP32, "fp32_precond")):
+            M.set_precision(prec)
+            St = deco_amd.SOL(sol, A, M, w, transpose=True)      # warm-up (builds A^T and the transposes)
             deco_amd.lib().deco_sync()
-            ts.append(time.perf_counter() - t1)
-        adj_ms = float(np.median(ts)) * 1e3
-        del dA
-        # the same right-hand side through the forward solve: iteration counts depend on the
-        # right-hand side (w ~ N(0,1) is harder than b = A x*), so this is the like-for-like cost
-        tf = []
-        for _ in range(args.steps):
-            t1 = time.perf_counter()
-            Sf = deco_amd.SOL(sol, A, M, w)
-            deco_amd.lib().deco_sync()
-            tf.append(time.perf_counter() - t1)
-        fw_ms = float(np.median(tf)) * 1e3
-        result["adjoint"] = {"wall_ms": adj_ms, "iter": St.iter, "flag": St.flag, "res_rel": St.res_rel,
-                             "ratio_to_fp64_forward": adj_ms / r64["wall_ms"],
-                             "forward_same_rhs_ms": fw_ms, "forward_same_rhs_iter": Sf.iter,
-                             "ratio_to_forward_same_rhs": adj_ms / fw_ms,
-                             "per_iteration_ratio": (adj_ms / max(St.iter, 1)) / (fw_ms / max(Sf.iter, 1))}
-        log(f"adjoint: {result['adjoint']}")
+            ts, tf = [], []
+            for _ in range(args.steps):
+                t1 = time.perf_counter()
+                St = deco_amd.SOL(sol, A, M, w, transpose=True)
+                lam = M.transform_transpose(St.x)
+                dA = deco_amd.grad_A_pattern(A, lam, S64.x)
+                deco_amd.lib().deco_sync()
+                ts.append(time.perf_counter() - t1)
+            del dA
+            for _ in range(args.steps):
+                t1 = time.perf_counter()
+                Sf = deco_amd.SOL(sol, A, M, w)
+                deco_amd.lib().deco_sync()
+                tf.append(time.perf_counter() - t1)
+            adj_ms, fw_ms = float(np.median(ts)) * 1e3, float(np.median(tf)) * 1e3
+            result["adjoint"][key] = {
+                "wall_ms": adj_ms, "iter": St.iter, "flag": St.flag, "res_rel": St.res_rel,
+                "ratio_to_forward_b": adj_ms / fwd[prec]["wall_ms"],
+                "forward_same_rhs_ms": fw_ms, "forward_same_rhs_iter": Sf.iter,
+                "ratio_to_forward_same_rhs": adj_ms / fw_ms,
+                "per_iteration_ratio": (adj_ms / max(St.iter, 1)) / (fw_ms / max(Sf.iter, 1))}
+            log(f"adjoint {key}: {result['adjoint'][key]}")
+        M.set_precision(deco_amd.PREC_FP64)
 
     # ---------------------------------------------------------------- CPU baseline
     if not args.no_cpu:

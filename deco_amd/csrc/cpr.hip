@@ -28,6 +28,8 @@ void fsai_apply_t_add(FSAI* F, const double* r, double* x);
 void fsai_apply32(FSAI* F, const double* r, float* x);
 void fsai_apply32_f(FSAI* F, const float* r, float* x);
 void fsai_apply32_addz(FSAI* F, const float* r, const float* z, double* x);
+void fsai_apply32_t(FSAI* F, const double* r, float* x);
+void fsai_apply32_t_addz(FSAI* F, const float* r, const float* z, double* x);
 
 enum CprErr { ERR_DET = 1, ERR_ROWLEN = 2 };
 
@@ -371,6 +373,10 @@ __global__ void k_addz(int n, const TA* __restrict__ z, const TB* __restrict__ a
   const int i = blockIdx.x * blockDim.x + threadIdx.x;  // x = z + a (z, a of any precision)
   if (i < n) x[i] = (double)z[i] + (double)a[i];
 }
+__global__ void k_add_f(int n, const float* __restrict__ a, float* __restrict__ e) {   // e += a
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) e[i] = a[i] + e[i];
+}
 __global__ void k_add3(int n, const double* __restrict__ a, const double* __restrict__ e, double* __restrict__ x) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;  // x += a + e
   if (i < n) x[i] += a[i] + e[i];
@@ -386,7 +392,7 @@ struct CPR : deco_prec {
   deco_prec* jac = nullptr;
   int bs = 1, sn = 0;
   DBuf<double> W, rt, xp, tmp, tmp2, x2;
-  DBuf<float> x32, rt32, xp32, t32;
+  DBuf<float> x32, rt32, xp32, t32, s32, e32;
 
   CPR(const deco_cpr_params& pp, deco_csr* A_, double* b) : p(pp), A(A_) {
     kind = deco_prec::CPR;
@@ -633,10 +639,47 @@ struct CPR : deco_prec {
     }
   }
 
+  // out = r - A^T x with FP32 values of the decoupled A^T (cell blocks)
+  template <class TR>
+  void residual32_t(const float* x, const TR* r, float* out) {
+    const deco_csr* At = A->At;
+    if (const deco_bsr32* B = At->block32(bs, bs)) bsr_stream_f(B, x, EpiStd<EPI_RES, float, TR>{out, r});
+    else spmv_stream<EPI_RES, float, float, float, TR>(At, const_cast<deco_csr*>(At)->values_f32(), x, out, r);
+  }
+
+  // M^T u for seq 3 with FP32 internals, FP64 in and out (the sequence of apply_t)
+  void apply32_t_seq(const double* u, double* xo) {
+    deco::FSAI* F = as_fsai(glo);
+    if (!F) fail(DECO_ERR_UNSUPPORTED, "the FP32 transposed CPR needs the FSAI global stage");
+    s32.ensure(n);
+    e32.ensure(n);
+    float* a = t32.p;
+    fsai_apply32_t(F, u, a);                                           // a = G^T u
+    {
+      ProfPhase ph("CPR_rest");
+      residual32_t(a, u, s32.p);                                       // s = u - A^T a
+      hipLaunchKernelGGL((k_gather_pressure<float, float>), dim3(gs()), dim3(256), 0, stream(), sn, bs,
+                         (const float*)s32.p, rt32.p);
+      DECO_CHECK_LAUNCH();
+    }
+    loc->apply32_t(rt32.p, xp32.p);                                    // e = P L^T R s
+    ProfPhase ph_rest("CPR_rest");
+    DECO_HIP(hipMemsetAsync(e32.p, 0, sizeof(float) * n, stream()));
+    hipLaunchKernelGGL((k_set_pressure<float, float>), dim3(gs()), dim3(256), 0, stream(), sn, bs, e32.p,
+                       (const float*)xp32.p);
+    DECO_CHECK_LAUNCH();
+    residual32_t(e32.p, (const float*)s32.p, x32.p);                    // s - A^T e
+    hipLaunchKernelGGL(k_add_f, dim3(gn()), dim3(256), 0, stream(), n, (const float*)a, e32.p);   // e = a + e
+    DECO_CHECK_LAUNCH();
+    ph_rest.end();
+    fsai_apply32_t_addz(F, x32.p, e32.p, xo);                          // out = a + e + G^T (s - A^T e)
+  }
+
   // M^T u for seq 3 (FP64): a = G^T u, s = u - A^T a, e = P L^T R s, out = a + e + G^T (s - A^T e)
   void apply_t(const double* u, double* x) override {
     if (p.seq != 3) fail(DECO_ERR_UNSUPPORTED, "transposed CPR is implemented for seq 3");
     if (!A->At) A->At = csr_transpose(A);
+    if (precision == DECO_PREC_FP32) { apply32_t_seq(u, x); return; }
     deco_csr* At = A->At;
     double* a = tmp.p;
     double* s = tmp2.p;

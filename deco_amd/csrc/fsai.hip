@@ -416,15 +416,17 @@ struct FSAI : deco_prec {
   }
   template <class TI>
   void apply32_t_in(const TI* r, float* x) {
+    ProfPhase ph("FSAI apply");
     ensure_t();
-    spmv_stream<EPI_SET, float, TI, float, float>(FU, FU->values_f32(), r, temp32.p, (const float*)nullptr);
-    spmv_stream<EPI_SET, float, float, float, float>(FLt, FLt->values_f32(), temp32.p, x, (const float*)nullptr);
+    f32_product(FU, r, EpiStd<EPI_SET, float, float>{temp32.p, nullptr});
+    f32_product(FLt, (const float*)temp32.p, EpiStd<EPI_SET, float, float>{x, nullptr});
   }
   template <class TI>
   void apply32_t_addz(const TI* r, const float* z, double* xo) {
+    ProfPhase ph("FSAI apply");
     ensure_t();
-    spmv_stream<EPI_SET, float, TI, float, float>(FU, FU->values_f32(), r, temp32.p, (const float*)nullptr);
-    spmv_stream<EPI_ADDZ, float, float, double, float>(FLt, FLt->values_f32(), temp32.p, xo, z);
+    f32_product(FU, r, EpiStd<EPI_SET, float, float>{temp32.p, nullptr});
+    f32_product(FLt, (const float*)temp32.p, EpiStd<EPI_ADDZ, double, float>{xo, z});
   }
   void apply32(const float* r, float* x) override { apply32_in(r, x); }
   void apply32_t(const float* r, float* x) override { apply32_t_in(r, x); }
@@ -438,6 +440,8 @@ void fsai_apply_t_add(FSAI* F, const double* r, double* x) { F->apply_t_add(r, x
 void fsai_apply32(FSAI* F, const double* r, float* x) { F->apply32_in<double>(r, x); }
 void fsai_apply32_f(FSAI* F, const float* r, float* x) { F->apply32_in<float>(r, x); }
 void fsai_apply32_addz(FSAI* F, const float* r, const float* z, double* x) { F->apply32_addz<float>(r, z, x); }
+void fsai_apply32_t(FSAI* F, const double* r, float* x) { F->apply32_t_in<double>(r, x); }
+void fsai_apply32_t_addz(FSAI* F, const float* r, const float* z, double* x) { F->apply32_t_addz<float>(r, z, x); }
 
 }  // namespace deco
 

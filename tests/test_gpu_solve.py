@@ -147,3 +147,23 @@ def test_adjoint_transposed_solve_and_gradient(gpu):
     dA = deco_amd.grad_A_pattern(Aorig, lam, x).numpy()
     rows = np.repeat(np.arange(g["n"]), np.diff(g["r"]))
     assert np.allclose(dA, -lam[rows] * x[g["c"]], rtol=0, atol=0)
+
+
+def test_fp32_transposed_cpr_and_adjoint_solve(gpu):
+    """The FP32 transposed CPR (seq 3) against the FP64 one, and the FP32-preconditioned
+    transposed solve against the FP64 one at the same tolerance."""
+    cfg = 4
+    tree = small_params(2)
+    g, A, b, M, Ao, bo, Mo = setup_pair(cfg, tree)
+    sol = {"type": "rgmres", "reltol": "1e-10", "restart": "50", "itmax": "500", "verbosity": "0"}
+    w = np.random.default_rng(8).standard_normal(g["n"])
+    u = np.random.default_rng(9).standard_normal(g["n"])
+    y64 = M.apply_t(u).numpy()
+    S64 = deco_amd.SOL(sol, A, M, w, transpose=True)
+    M.set_precision(deco_amd.PREC_FP32)
+    y32 = M.apply_t(u).numpy()
+    S32 = deco_amd.SOL(sol, A, M, w, transpose=True)
+    M.set_precision(deco_amd.PREC_FP64)
+    assert rel(y32, y64) <= 2e-4
+    assert S32.flag == 0 and S64.flag == 0 and abs(S32.iter - S64.iter) <= 2
+    assert rel(S32.x.numpy(), S64.x.numpy()) <= 1e-6
