@@ -130,10 +130,15 @@ class CSR:
         n, m, _ = self.info()
         return sp.csr_matrix((v, c, r), shape=(n, m))
 
-    def apply(self, x, bs=1):
-        """CSR::apply(x, b, bs) (core/CSR.hpp:132-136)."""
+    def apply(self, x, bs=1, block=None):
+        """CSR::apply(x, b, bs) (core/CSR.hpp:132-136); block=(R, C) runs the product on the
+        FP64 block shadow of A."""
         n, m, _ = self.info()
         xd = as_dev(x)
+        if block is not None:
+            y = DeviceArray(n)
+            check(lib().deco_spmv_block(self.h, int(block[0]), int(block[1]), xd.ptr, y.ptr))
+            return y
         y = DeviceArray(n // bs)
         check(lib().deco_spmv(self.h, xd.ptr, y.ptr, bs))
         return y
@@ -163,10 +168,17 @@ class CSR:
             check(lib().deco_spmv_block_f32(self.h, int(block[0]), int(block[1]), xd.ptr, y.ptr))
         return y
 
-    def spmv_time(self, precision=_lib.PREC_FP64, iters=20):
+    def spmv_time(self, precision=_lib.PREC_FP64, iters=20, block=None):
+        """Average product duration in ms (hipEvents); with block=(R, C), (ms, blocks) of the
+        block-shadow product."""
         ms = C.c_double()
-        check(lib().deco_spmv_time(self.h, precision, iters, C.byref(ms)))
-        return ms.value
+        if block is None:
+            check(lib().deco_spmv_time(self.h, precision, iters, C.byref(ms)))
+            return ms.value
+        nb = C.c_int()
+        check(lib().deco_spmv_block_time(self.h, int(block[0]), int(block[1]), precision, iters, C.byref(ms),
+                                         C.byref(nb)))
+        return ms.value, nb.value
 
     def copy(self):
         h = C.c_void_p()

@@ -96,10 +96,13 @@ struct EpiRelax {
   const int* __restrict__ cf;
   int subset;
   T* __restrict__ y;
+  struct Pre { T x, b, d; bool on; };
+  __device__ __forceinline__ Pre pre(int o) const {
+    return Pre{x[o], b[o], d[o], subset < 0 || (subset == 1 ? cf[o] == 1 : cf[o] != 1)};
+  }
   template <class AT>
-  __device__ __forceinline__ void operator()(int o, AT acc) const {
-    const bool on = subset < 0 || (subset == 1 ? cf[o] == 1 : cf[o] != 1);
-    y[o] = on ? (T)(x[o] + (b[o] - (T)acc) / d[o]) : x[o];
+  __device__ __forceinline__ void operator()(int o, AT acc, const Pre& p) const {
+    y[o] = p.on ? (T)(p.x + (p.b - (T)acc) / p.d) : p.x;
   }
 };
 
@@ -110,11 +113,13 @@ struct EpiRestrictJacobi0 {
   T* __restrict__ bn;
   const T* __restrict__ dn;
   T* __restrict__ xn;
+  struct Pre { T d; };
+  __device__ __forceinline__ Pre pre(int o) const { return Pre{dn[o]}; }
   template <class AT>
-  __device__ __forceinline__ void operator()(int o, AT acc) const {
+  __device__ __forceinline__ void operator()(int o, AT acc, const Pre& p) const {
     const T bo = (T)acc;
     bn[o] = bo;
-    xn[o] = (T)0 + (bo - (T)0) / dn[o];
+    xn[o] = (T)0 + (bo - (T)0) / p.d;
   }
 };
 

@@ -522,6 +522,9 @@ struct CPR : deco_prec {
                        (const unsigned char*)flag.p);
     DECO_CHECK_LAUNCH();
     A->invalidate();
+    // the Krylov product with the decoupled A runs on its cell-block shadow (FP64 values)
+    A->cell = bs;
+    A->block64(bs, bs);
     // the strided pressure residual of CPR.hpp:241-246 runs as a contiguous stream SpMV over
     // a copy of the pressure rows (same entries, same per-row summation order)
     if (p.seq != 1) Apr = csr_row_subset(A, bs, 0);
@@ -550,12 +553,12 @@ struct CPR : deco_prec {
     spmv_stream_f(Apr, Apr->v.p, x, EpiStrideRes<double, double>{out, r, bs});
   }
   void pressure_residual(const float* x, const double* r, float* out) {
-    if (const deco_bsr32* B = Apr->block32(1, bs)) bsr_stream_f(B, x, EpiStrideRes<float, double>{out, r, bs});
+    if (const deco_bsr* B = Apr->block32(1, bs)) bsr_stream_f(B, x, EpiStrideRes<float, double>{out, r, bs});
     else spmv_stream_f(Apr, Apr->values_f32(), x, EpiStrideRes<float, double>{out, r, bs});
   }
   // out = r - A x with FP32 values of the decoupled A (cell blocks)
   void residual32(const float* x, const double* r, float* out) {
-    if (const deco_bsr32* B = A->block32(bs, bs)) bsr_stream_f(B, x, EpiStd<EPI_RES, float, double>{out, r});
+    if (const deco_bsr* B = A->block32(bs, bs)) bsr_stream_f(B, x, EpiStd<EPI_RES, float, double>{out, r});
     else spmv_stream<EPI_RES, float, float, float, double>(A, A->values_f32(), x, out, r);
   }
 
@@ -643,7 +646,7 @@ struct CPR : deco_prec {
   template <class TR>
   void residual32_t(const float* x, const TR* r, float* out) {
     const deco_csr* At = A->At;
-    if (const deco_bsr32* B = At->block32(bs, bs)) bsr_stream_f(B, x, EpiStd<EPI_RES, float, TR>{out, r});
+    if (const deco_bsr* B = At->block32(bs, bs)) bsr_stream_f(B, x, EpiStd<EPI_RES, float, TR>{out, r});
     else spmv_stream<EPI_RES, float, float, float, TR>(At, const_cast<deco_csr*>(At)->values_f32(), x, out, r);
   }
 

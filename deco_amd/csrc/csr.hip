@@ -624,10 +624,12 @@ __global__ void k_bsr_count(int nb, int R, int C, const int* __restrict__ r, con
   cnt[I] = count;
 }
 
-// block columns and row-major FP32 block values (bv zeroed beforehand)
+// block columns and block values (bv zeroed beforehand): row-major blocks one after another
+// (ps == 0), or row-planar (element (rr, q) of block o at rr*ps + o*C + q)
+template <class VT>
 __global__ void k_bsr_fill(int nb, int R, int C, const int* __restrict__ r, const int* __restrict__ c,
                            const double* __restrict__ v, const int* __restrict__ br, int* __restrict__ bc,
-                           float* __restrict__ bv) {
+                           VT* __restrict__ bv, size_t ps) {
   const int I = blockIdx.x * blockDim.x + threadIdx.x;
   if (I >= nb) return;
   int pos[3], end[3];
@@ -641,33 +643,34 @@ __global__ void k_bsr_fill(int nb, int R, int C, const int* __restrict__ r, cons
     bc[o] = mn;
     for (int q = 0; q < R; ++q)
       for (; pos[q] < end[q] && c[pos[q]] / C == mn; ++pos[q])
-        bv[(size_t)o * R * C + q * C + (c[pos[q]] - mn * C)] = (float)v[pos[q]];
+        bv[ps ? q * ps + (size_t)o * C + (c[pos[q]] - mn * C) : (size_t)o * R * C + q * C + (c[pos[q]] - mn * C)] =
+            (VT)v[pos[q]];
     ++o;
   }
 }
 
-// rb[k] = first row i in [0, n] with ptr[i] >= k*cap; rb[nchunks] = n
-__global__ void k_chunks_cap(int n, const int* __restrict__ ptr, int nchunks, int cap, int* __restrict__ rb) {
+// rb[k] = (i, ptr[i]) for the first row i in [0, n] with ptr[i] >= k*cap; rb[nchunks] = (n, ptr[n])
+__global__ void k_chunks_cap(int n, const int* __restrict__ ptr, int nchunks, int cap, int2* __restrict__ rb) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k > nchunks) return;
-  if (k == nchunks) { rb[k] = n; return; }
+  if (k == nchunks) { rb[k] = make_int2(n, ptr[n]); return; }
   const int64_t target = (int64_t)k * cap;
   int lo = 0, hi = n;
   while (lo < hi) {
     int mid = (lo + hi) >> 1;
     if (ptr[mid] >= target) hi = mid; else lo = mid + 1;
   }
-  rb[k] = lo;
+  rb[k] = make_int2(lo, ptr[lo]);
 }
 }  // namespace deco
 
-const deco_bsr32* deco_csr::block32(int R, int C) const {
+const deco_bsr* deco_csr::block32(int R, int C) const {
   using namespace deco;
   if (b32 && b32->R == R && b32->C == C) return b32;
   delete b32;
   b32 = nullptr;
   if (R < 1 || R > 3 || C < 1 || C > 3 || n % R != 0 || m % C != 0) return nullptr;
-  auto* B = new deco_bsr32();
+  auto* B = new deco_bsr();
   B->R = R;
   B->C = C;
   B->nb = n / R;
@@ -684,8 +687,8 @@ const deco_bsr32* deco_csr::block32(int R, int C) const {
   B->bv.alloc(std::max<int64_t>(nnzb, 1) * R * C);
   DECO_HIP(hipMemsetAsync(B->bv.p, 0, sizeof(float) * std::max<int64_t>(nnzb, 1) * R * C, stream()));
   if (nb > 0) {
-    hipLaunchKernelGGL(k_bsr_fill, dim3(grid_for(nb, 256)), dim3(256), 0, stream(), nb, R, C, r.p, c.p, v.p,
-                       B->br.p, B->bc.p, B->bv.p);
+    hipLaunchKernelGGL(k_bsr_fill<float>, dim3(grid_for(nb, 256)), dim3(256), 0, stream(), nb, R, C, r.p, c.p, v.p,
+                       B->br.p, B->bc.p, B->bv.p, bsr_planar_rt(4, R, C) ? (size_t)nnzb * C : (size_t)0);
     DECO_CHECK_LAUNCH();
   }
   B->nchunks = (int)std::max<int64_t>(1, (nnzb + kBsrCap - 1) / kBsrCap);
@@ -697,28 +700,30 @@ const deco_bsr32* deco_csr::block32(int R, int C) const {
   return B;
 }
 
-namespace deco {
-// rb[k] = first row i in [0, n] with r[i] >= k*CAP; rb[nchunks] = n
-__global__ void k_row_chunks(int n, const int* __restrict__ r, int nchunks, int* __restrict__ rb) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k > nchunks) return;
-  if (k == nchunks) { rb[k] = n; return; }
-  const int target = k * kStreamCap;
-  int lo = 0, hi = n;  // answer in [0, n]
-  while (lo < hi) {
-    int mid = (lo + hi) >> 1;
-    if (r[mid] >= target) hi = mid; else lo = mid + 1;
+const deco_bsr* deco_csr::block64(int R, int C) const {
+  using namespace deco;
+  const deco_bsr* B0 = block32(R, C);
+  if (!B0 || B0->has64) return B0;
+  deco_bsr* B = b32;
+  const size_t nv = (size_t)std::max(B->nnzb, 1) * R * C;
+  B->bv64.alloc(nv);
+  DECO_HIP(hipMemsetAsync(B->bv64.p, 0, sizeof(double) * nv, stream()));
+  if (B->nb > 0) {
+    // rewrites the same block columns, adds the FP64 values
+    hipLaunchKernelGGL(k_bsr_fill<double>, dim3(grid_for(B->nb, 256)), dim3(256), 0, stream(), B->nb, R, C, r.p,
+                       c.p, v.p, B->br.p, B->bc.p, B->bv64.p, bsr_planar_rt(8, R, C) ? (size_t)B->nnzb * C : (size_t)0);
+    DECO_CHECK_LAUNCH();
   }
-  rb[k] = lo;
+  B->has64 = true;
+  return B;
 }
-}  // namespace deco
 
-const int* deco_csr::row_chunks() const {
+const int2* deco_csr::row_chunks() const {
   if (!rb_valid) {
     nchunks = (int)std::max<int64_t>(1, ((int64_t)nnz + deco::kStreamCap - 1) / deco::kStreamCap);
     rb.ensure(nchunks + 1);
-    hipLaunchKernelGGL(deco::k_row_chunks, dim3(deco::grid_for(nchunks + 1, 256)), dim3(256), 0, deco::stream(), n,
-                       r.p, nchunks, rb.p);
+    hipLaunchKernelGGL(deco::k_chunks_cap, dim3(deco::grid_for(nchunks + 1, 256)), dim3(256), 0, deco::stream(), n,
+                       r.p, nchunks, deco::kStreamCap, rb.p);
     DECO_CHECK_LAUNCH();
     rb_valid = true;
   }
@@ -955,10 +960,53 @@ int deco_spmv_f32(deco_csr* A, const float* x, float* y) {
 int deco_spmv_block_f32(deco_csr* A, int R, int C, const float* x, float* y) {
   return guard([&] {
     if (!A) fail(DECO_ERR_ARG, "null CSR");
-    const deco_bsr32* B = A->block32(R, C);
+    const deco_bsr* B = A->block32(R, C);
     if (!B) fail(DECO_ERR_ARG, "%d x %d blocks do not tile a %d x %d matrix", R, C, A->n, A->m);
     bsr_stream_f(B, x, EpiStd<EPI_SET, float, float>{y, nullptr});
     sync();
+  });
+}
+
+int deco_spmv_block(deco_csr* A, int R, int C, const double* x, double* y) {
+  return guard([&] {
+    if (!A) fail(DECO_ERR_ARG, "null CSR");
+    const deco_bsr* B = A->block64(R, C);
+    if (!B) fail(DECO_ERR_ARG, "%d x %d blocks do not tile a %d x %d matrix", R, C, A->n, A->m);
+    bsr_stream_d(B, x, EpiOuter{y});
+    sync();
+  });
+}
+
+int deco_spmv_block_time(deco_csr* A, int R, int C, int precision, int iters, double* avg_ms, int* nnzb) {
+  return guard([&] {
+    if (!A || !avg_ms || iters <= 0) fail(DECO_ERR_ARG, "bad argument");
+    const deco_bsr* B = precision == DECO_PREC_FP32 ? A->block32(R, C) : A->block64(R, C);
+    if (!B) fail(DECO_ERR_ARG, "%d x %d blocks do not tile a %d x %d matrix", R, C, A->n, A->m);
+    if (nnzb) *nnzb = B->nnzb;
+    hipEvent_t e0, e1;
+    DECO_HIP(hipEventCreate(&e0));
+    DECO_HIP(hipEventCreate(&e1));
+    if (precision == DECO_PREC_FP32) {
+      DBuf<float> x(A->m), y(A->n);
+      x.zero();
+      bsr_stream_f(B, x.p, EpiStd<EPI_SET, float, float>{y.p, nullptr});
+      DECO_HIP(hipEventRecord(e0, stream()));
+      for (int k = 0; k < iters; ++k) bsr_stream_f(B, x.p, EpiStd<EPI_SET, float, float>{y.p, nullptr});
+      DECO_HIP(hipEventRecord(e1, stream()));
+    } else {
+      DBuf<double> x(A->m), y(A->n);
+      x.zero();
+      bsr_stream_d(B, x.p, EpiOuter{y.p});
+      DECO_HIP(hipEventRecord(e0, stream()));
+      for (int k = 0; k < iters; ++k) bsr_stream_d(B, x.p, EpiOuter{y.p});
+      DECO_HIP(hipEventRecord(e1, stream()));
+    }
+    DECO_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    DECO_HIP(hipEventElapsedTime(&ms, e0, e1));
+    *avg_ms = (double)ms / iters;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
   });
 }
 

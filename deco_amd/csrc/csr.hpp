@@ -5,14 +5,19 @@
 #pragma once
 #include "common.hpp"
 
-// FP32 block-CSR shadow of a CSR matrix whose rows and columns come in groups (R rows per block
+// Block-CSR shadow of a CSR matrix whose rows and columns come in groups (R rows per block
 // row, C columns per block column): block row I covers rows I*R .. I*R+R-1, block (I, J) holds
-// the R x C values of columns J*C .. J*C+C-1, row-major, zeros where the CSR has no entry.
-// Chunk table: block rows whose first block lies in [k*kBsrCap, (k+1)*kBsrCap).
-struct deco_bsr32 {
+// the R x C values of columns J*C .. J*C+C-1, row-major, zeros where the CSR has no entry. The
+// FP32 values are made with the structure; the FP64 values on first FP64 use.
+// Chunk table: block rows whose first block lies in [k*kBsrCap, (k+1)*kBsrCap), as
+// (first block row, its first block) pairs.
+struct deco_bsr {
   int R = 1, C = 1, nb = 0, nnzb = 0, nchunks = 0;
-  deco::DBuf<int> br, bc, rb;
+  deco::DBuf<int> br, bc;
+  deco::DBuf<int2> rb;
   deco::DBuf<float> bv;
+  deco::DBuf<double> bv64;
+  bool has64 = false;
 };
 
 struct deco_csr {
@@ -23,16 +28,22 @@ struct deco_csr {
   bool vf_valid = false;
   deco_csr* At = nullptr;    // cached transpose (adjoint SpMV)
   // SpMV row chunks: chunk k holds the rows whose first entry lies in [k*CAP, (k+1)*CAP)
-  mutable deco::DBuf<int> rb;
+  // as (first row, its first entry) pairs, nchunks + 1 of them
+  mutable deco::DBuf<int2> rb;
   mutable int nchunks = 0;
   mutable bool rb_valid = false;
   ~deco_csr();
   const float* values_f32();  // creates / refreshes the shadow
-  const int* row_chunks() const;  // creates the chunk table on first use
-  mutable deco_bsr32* b32 = nullptr;  // FP32 block shadow (valid with vf_valid's lifetime rules)
-  // the FP32 block shadow with R x C blocks (created on first use); null when n or m do not
-  // divide into such blocks
-  const deco_bsr32* block32(int R, int C) const;
+  const int2* row_chunks() const;  // creates the chunk table on first use
+  mutable deco_bsr* b32 = nullptr;  // block shadow (dropped with the FP32 shadow by invalidate())
+  // the block shadow with R x C blocks and FP32 values (created on first use); null when n or
+  // m do not divide into such blocks
+  const deco_bsr* block32(int R, int C) const;
+  // the same with its FP64 values filled
+  const deco_bsr* block64(int R, int C) const;
+  // unknowns per cell of the system this matrix belongs to (set by the CPR setup): above 1 the
+  // solvers' FP64 product with the matrix runs on the cell x cell block shadow
+  int cell = 1;
   void invalidate();
 };
 
@@ -117,26 +128,34 @@ void launch_spmv(int G, int nout, int rs, const int* r, const int* c, const VT* 
 // lanes) or 6 (FP32, 512 lanes) independent (column, value) loads, then all of its x gathers,
 // then the products. The tile has no per-entry branches (indices past the range are clamped
 // and their products dropped), so the gathers are in flight together instead of one at a
-// time. The products are summed per row in column order (the same order as alg::spmv,
-// core/alg.hpp:59-61). Ranges that exceed the LDS tile (rows over 129 entries) fall back to a
-// block reduction per row.
+// time. Rows shorter than 8 entries on average are summed in column order (the same order as
+// alg::spmv, core/alg.hpp:59-61); longer rows by a lane group and a fixed xor tree. Ranges that
+// exceed the LDS tile (rows over 129 entries) fall back to a block reduction per row.
 constexpr int kStreamCap = 3072;                 // target entries per chunk
 template <class VT>
 constexpr int stream_block() {
   return sizeof(VT) == 8 ? 256 : 512;
 }
 
-// Epilogue functors: called once per row with the row sum (accumulator type AT).
+// Epilogue functors. pre(o) loads the operands row o needs besides its sum; the stream kernels
+// issue it together with the tile's column/value loads, so its latency is hidden behind them.
+// operator()(o, acc, pre) writes row o from its sum (accumulator type AT).
 template <int EPI, class YT, class ZT>
 struct EpiStd {
   YT* __restrict__ y;
   const ZT* __restrict__ z;
+  using PT = typename std::conditional<EPI == EPI_ACC, YT, ZT>::type;
+  struct Pre { PT a; };
+  __device__ __forceinline__ Pre pre(int o) const {
+    if constexpr (EPI == EPI_SET) return Pre{PT(0)};
+    else if constexpr (EPI == EPI_ACC) return Pre{y[o]};
+    else return Pre{z[o]};
+  }
   template <class AT>
-  __device__ __forceinline__ void operator()(int o, AT acc) const {
+  __device__ __forceinline__ void operator()(int o, AT acc, const Pre& p) const {
     if constexpr (EPI == EPI_SET) y[o] = (YT)acc;
-    else if constexpr (EPI == EPI_RES) y[o] = (YT)((AT)z[o] - acc);
-    else if constexpr (EPI == EPI_ACC) y[o] = (YT)((AT)y[o] + acc);
-    else y[o] = (YT)((AT)z[o] + acc);
+    else if constexpr (EPI == EPI_RES) y[o] = (YT)((AT)p.a - acc);
+    else y[o] = (YT)((AT)p.a + acc);                    // EPI_ACC, EPI_ADDZ
   }
 };
 
@@ -144,8 +163,10 @@ struct EpiStd {
 // instantiation, so a profile separates it from the preconditioner's products
 struct EpiOuter {
   double* __restrict__ y;
+  struct Pre {};
+  __device__ __forceinline__ Pre pre(int) const { return Pre{}; }
   template <class AT>
-  __device__ __forceinline__ void operator()(int o, AT acc) const { y[o] = (double)acc; }
+  __device__ __forceinline__ void operator()(int o, AT acc, const Pre&) const { y[o] = (double)acc; }
 };
 
 // y[o] = z[o*zs] - sum: residual of a row subset (rows zs*o of the full matrix)
@@ -154,8 +175,10 @@ struct EpiStrideRes {
   YT* __restrict__ y;
   const ZT* __restrict__ z;
   int zs;
+  struct Pre { ZT a; };
+  __device__ __forceinline__ Pre pre(int o) const { return Pre{z[(size_t)o * zs]}; }
   template <class AT>
-  __device__ __forceinline__ void operator()(int o, AT acc) const { y[o] = (YT)((AT)z[(size_t)o * zs] - acc); }
+  __device__ __forceinline__ void operator()(int o, AT acc, const Pre& p) const { y[o] = (YT)((AT)p.a - acc); }
 };
 
 // x access of the stream kernel: a plain vector, or a functor that computes x_j (fused steps)
@@ -167,7 +190,7 @@ struct PtrGather {
 };
 
 template <class VT, class XG, class E>
-__global__ __launch_bounds__(stream_block<VT>()) void k_spmv_stream(int nch, const int* __restrict__ rb,
+__global__ __launch_bounds__(stream_block<VT>()) void k_spmv_stream(const int2* __restrict__ desc,
                                                                    const int* __restrict__ r,
                                                                    const int* __restrict__ c,
                                                                    const VT* __restrict__ v, XG x, E epi) {
@@ -178,11 +201,29 @@ __global__ __launch_bounds__(stream_block<VT>()) void k_spmv_stream(int nch, con
   constexpr int U = kStreamCap / BLK;
   __shared__ AT prod[LDSN];
   const int ch = xcd_block(blockIdx.x, gridDim.x);
-  const int r0 = rb[ch], r1 = rb[min(ch + 1, nch)];
+  const int2 d0 = desc[ch], d1 = desc[ch + 1];   // (first row, its first entry) of chunks ch, ch+1
+  const int r0 = d0.x, r1 = d1.x;
   if (r0 >= r1) return;
-  const int j0 = r[r0], j1 = r[r1];
+  const int j0 = d0.y, j1 = d1.y;
   const int cnt = j1 - j0;
   const int tid = threadIdx.x;
+  // Lanes per row, 2^lg: 1 (the row summed in column order) unless the rows average 8 or more
+  // entries, then the widest group (at most a wavefront) that gives every row of the chunk its
+  // own group, so a long row's sum is not one lane's chain of LDS reads.
+  const int nr = r1 - r0;
+  int lg = 0;
+  if (cnt >= 8 * nr)
+    while (lg < 6 && (nr << (lg + 1)) <= BLK) ++lg;
+  const int sub = tid & ((1 << lg) - 1);
+  // this lane's first row: bounds and epilogue operands load together with the tile
+  const int row = r0 + (tid >> lg);
+  int rbeg = 0, rend = 0;
+  typename E::Pre pre{};
+  if (row < r1) {
+    rbeg = r[row];
+    rend = r[row + 1];
+    pre = epi.pre(row);
+  }
   if (cnt <= LDSN) {
     if (cnt > 0) {
       const int jl = j1 - 1;
@@ -206,20 +247,28 @@ __global__ __launch_bounds__(stream_block<VT>()) void k_spmv_stream(int nch, con
         prod[t] = (AT)__builtin_nontemporal_load(v + j0 + t) * (AT)x(__builtin_nontemporal_load(c + j0 + t));
     }
     __syncthreads();
-    for (int row = r0 + tid; row < r1; row += BLK) {
-      const int b = r[row] - j0, e = r[row + 1] - j0;
+    if (row < r1) {   // a lane group is active or inactive as a whole
+      const int G = 1 << lg;
       AT acc = AT(0);
-      for (int k = b; k < e; ++k) acc += prod[k];
-      epi(row, acc);
+      for (int k = rbeg - j0 + sub; k < rend - j0; k += G) acc += prod[k];
+      for (int off = G >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+      if (sub == 0) epi(row, acc, pre);
+      for (int rw = row + (BLK >> lg); rw < r1; rw += BLK >> lg) {
+        const int b = r[rw] - j0, e = r[rw + 1] - j0;
+        AT a2 = AT(0);
+        for (int k = b + sub; k < e; k += G) a2 += prod[k];
+        for (int off = G >> 1; off > 0; off >>= 1) a2 += __shfl_xor(a2, off);
+        if (sub == 0) epi(rw, a2, epi.pre(rw));
+      }
     }
   } else {
     __shared__ AT red[BLK / 64];
-    for (int row = r0; row < r1; ++row) {
-      const int b = r[row], e = r[row + 1];
+    for (int rw = r0; rw < r1; ++rw) {
+      const int b = r[rw], e = r[rw + 1];
       AT acc = AT(0);
       for (int j = b + tid; j < e; j += BLK) acc += (AT)v[j] * (AT)x(c[j]);
       acc = block_sum<BLK>(acc, red);
-      if (tid == 0) epi(row, acc);
+      if (tid == 0) epi(rw, acc, epi.pre(rw));
     }
   }
 }
@@ -227,9 +276,9 @@ __global__ __launch_bounds__(stream_block<VT>()) void k_spmv_stream(int nch, con
 template <class VT, class XG, class E>
 void spmv_stream_g(const deco_csr* A, const VT* v, XG x, E epi) {
   if (A->n <= 0) return;
-  const int* rb = A->row_chunks();
-  hipLaunchKernelGGL((k_spmv_stream<VT, XG, E>), dim3(A->nchunks), dim3(stream_block<VT>()), 0, stream(), A->nchunks,
-                     rb, A->r.p, A->c.p, v, x, epi);
+  const int2* desc = A->row_chunks();
+  hipLaunchKernelGGL((k_spmv_stream<VT, XG, E>), dim3(A->nchunks), dim3(stream_block<VT>()), 0, stream(), desc,
+                     A->r.p, A->c.p, v, x, epi);
   DECO_CHECK_LAUNCH();
 }
 
@@ -256,21 +305,46 @@ typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x2 __attribute__((ext_vector_type(2)));
 
-template <int N>
-struct BsrVals { float a[N]; };
+template <class VT, int N>
+struct BsrVals { VT a[N]; };
 
-template <int R, int C>
-__device__ __forceinline__ BsrVals<R * C> bsr_load(const float* __restrict__ bv, int j) {
-  BsrVals<R * C> o;
-  if constexpr (R * C == 4) {
-    const f32x4 t = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(bv) + j);
-    o.a[0] = t.x; o.a[1] = t.y; o.a[2] = t.z; o.a[3] = t.w;
-  } else if constexpr (R * C == 2) {
-    const f32x2 t = __builtin_nontemporal_load(reinterpret_cast<const f32x2*>(bv) + j);
-    o.a[0] = t.x; o.a[1] = t.y;
+// Value layouts of the block shadow. Interleaved: block j's R*C values at [j*R*C, (j+1)*R*C).
+// Row-planar: plane rr (stride ps elements) holds row rr of every block, C values per block, so a
+// wavefront's load of one row of 64 consecutive blocks is one contiguous range.
+template <class VT, int R, int C>
+constexpr bool bsr_planar() {
+  return !(sizeof(VT) == 4 && (R * C == 2 || R * C == 4));
+}
+inline bool bsr_planar_rt(int value_size, int R, int C) { return !(value_size == 4 && (R * C == 2 || R * C == 4)); }
+
+template <class VT, int R, int C>
+__device__ __forceinline__ BsrVals<VT, R * C> bsr_load(const VT* __restrict__ bv, int j, size_t ps) {
+  BsrVals<VT, R * C> o;
+  if constexpr (!bsr_planar<VT, R, C>()) {
+    if constexpr (R * C == 4) {
+      const f32x4 t = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(bv) + j);
+      o.a[0] = t.x; o.a[1] = t.y; o.a[2] = t.z; o.a[3] = t.w;
+    } else {
+      const f32x2 t = __builtin_nontemporal_load(reinterpret_cast<const f32x2*>(bv) + j);
+      o.a[0] = t.x; o.a[1] = t.y;
+    }
+  } else if constexpr (sizeof(VT) == 8 && C == 2) {
+#pragma unroll
+    for (int rr = 0; rr < R; ++rr) {
+      const f64x2 t = __builtin_nontemporal_load(reinterpret_cast<const f64x2*>(bv + rr * ps) + j);
+      o.a[rr * 2] = t.x; o.a[rr * 2 + 1] = t.y;
+    }
+  } else if constexpr (sizeof(VT) == 4 && C == 2) {
+#pragma unroll
+    for (int rr = 0; rr < R; ++rr) {
+      const f32x2 t = __builtin_nontemporal_load(reinterpret_cast<const f32x2*>(bv + rr * ps) + j);
+      o.a[rr * 2] = t.x; o.a[rr * 2 + 1] = t.y;
+    }
   } else {
 #pragma unroll
-    for (int q = 0; q < R * C; ++q) o.a[q] = __builtin_nontemporal_load(bv + (size_t)j * (R * C) + q);
+    for (int rr = 0; rr < R; ++rr)
+#pragma unroll
+      for (int q = 0; q < C; ++q) o.a[rr * C + q] = __builtin_nontemporal_load(bv + rr * ps + (size_t)j * C + q);
   }
   return o;
 }
@@ -289,35 +363,47 @@ __device__ __forceinline__ void bsr_gather(const XT* __restrict__ x, int col, XT
   }
 }
 
-template <int R, int C, class XT, class E>
-__global__ __launch_bounds__(kBsrBlock) void k_bsr_stream(const int* __restrict__ rb, const int* __restrict__ br,
-                                                         const int* __restrict__ bc, const float* __restrict__ bv,
-                                                         const XT* __restrict__ x, E epi) {
-  using AT = acc_t<float, XT>;
-  constexpr int LDSB = kBsrCap + 64;          // blocks: a chunk holds <= cap + longest block row - 1
+template <class VT, int R, int C, class XT, class E, int BLK = kBsrBlock, int UNR = kBsrUnroll>
+__global__ __launch_bounds__(BLK) void k_bsr_stream(const int2* __restrict__ desc, const int* __restrict__ br,
+                                                   const int* __restrict__ bc, const VT* __restrict__ bv, size_t ps,
+                                                   const XT* __restrict__ x, E epi) {
+  using AT = acc_t<VT, XT>;
+  constexpr int CAP = BLK * UNR;
+  constexpr int LDSB = CAP + 64;              // blocks: a chunk holds <= cap + longest block row - 1
   __shared__ AT prod[LDSB * R];
   const int ch = xcd_block(blockIdx.x, gridDim.x);
-  const int r0 = rb[ch], r1 = rb[ch + 1];
+  const int2 d0 = desc[ch], d1 = desc[ch + 1];   // (first block row, its first block) of chunks ch, ch+1
+  const int r0 = d0.x, r1 = d1.x;
   if (r0 >= r1) return;
-  const int j0 = br[r0], cnt = br[r1] - j0;
+  const int j0 = d0.y, cnt = d1.y - j0;
   const int tid = threadIdx.x;
+  // this lane's first block row: bounds and epilogue operands load together with the tile
+  const int row = r0 + tid;
+  int rbeg = 0, rend = 0;
+  typename E::Pre pre[R];
+  if (row < r1) {
+    rbeg = br[row];
+    rend = br[row + 1];
+#pragma unroll
+    for (int rr = 0; rr < R; ++rr) pre[rr] = epi.pre(row * R + rr);
+  }
   if (cnt <= LDSB) {
     if (cnt > 0) {   // branch-free tile: clamped indices, every gather in flight before the products
       const int jl = j0 + cnt - 1;
-      int cc[kBsrUnroll];
-      BsrVals<R * C> vv[kBsrUnroll];
-      XT xv[kBsrUnroll][C];
+      int cc[UNR];
+      BsrVals<VT, R * C> vv[UNR];
+      XT xv[UNR][C];
 #pragma unroll
-      for (int u = 0; u < kBsrUnroll; ++u) {
-        const int j = min(j0 + tid + u * kBsrBlock, jl);
+      for (int u = 0; u < UNR; ++u) {
+        const int j = min(j0 + tid + u * BLK, jl);
         cc[u] = __builtin_nontemporal_load(bc + j);
-        vv[u] = bsr_load<R, C>(bv, j);
+        vv[u] = bsr_load<VT, R, C>(bv, j, ps);
       }
 #pragma unroll
-      for (int u = 0; u < kBsrUnroll; ++u) bsr_gather<C>(x, cc[u], xv[u]);
+      for (int u = 0; u < UNR; ++u) bsr_gather<C>(x, cc[u], xv[u]);
 #pragma unroll
-      for (int u = 0; u < kBsrUnroll; ++u) {
-        const int t = tid + u * kBsrBlock;
+      for (int u = 0; u < UNR; ++u) {
+        const int t = tid + u * BLK;
         if (t < cnt) {
 #pragma unroll
           for (int rr = 0; rr < R; ++rr) {
@@ -329,8 +415,8 @@ __global__ __launch_bounds__(kBsrBlock) void k_bsr_stream(const int* __restrict_
         }
       }
     }
-    for (int t = tid + kBsrCap; t < cnt; t += kBsrBlock) {
-      const BsrVals<R * C> w = bsr_load<R, C>(bv, j0 + t);
+    for (int t = tid + CAP; t < cnt; t += BLK) {
+      const BsrVals<VT, R * C> w = bsr_load<VT, R, C>(bv, j0 + t, ps);
       XT xv[C];
       bsr_gather<C>(x, bc[j0 + t], xv);
 #pragma unroll
@@ -342,40 +428,49 @@ __global__ __launch_bounds__(kBsrBlock) void k_bsr_stream(const int* __restrict_
       }
     }
     __syncthreads();
-    for (int row = r0 + tid; row < r1; row += kBsrBlock) {
-      const int b = br[row] - j0, e = br[row + 1] - j0;
+    if (row < r1) {
 #pragma unroll
       for (int rr = 0; rr < R; ++rr) {
         AT acc = AT(0);
-        for (int k = b; k < e; ++k) acc += prod[k * R + rr];
-        epi(row * R + rr, acc);
+        for (int k = rbeg - j0; k < rend - j0; ++k) acc += prod[k * R + rr];
+        epi(row * R + rr, acc, pre[rr]);
+      }
+      for (int rw = row + BLK; rw < r1; rw += BLK) {
+        const int b = br[rw] - j0, e = br[rw + 1] - j0;
+#pragma unroll
+        for (int rr = 0; rr < R; ++rr) {
+          AT acc = AT(0);
+          for (int k = b; k < e; ++k) acc += prod[k * R + rr];
+          epi(rw * R + rr, acc, epi.pre(rw * R + rr));
+        }
       }
     }
   } else {
-    __shared__ AT red[kBsrBlock / 64];
-    for (int row = r0; row < r1; ++row) {
+    __shared__ AT red[BLK / 64];
+    for (int rw = r0; rw < r1; ++rw) {
 #pragma unroll
       for (int rr = 0; rr < R; ++rr) {
         AT acc = AT(0);
-        for (int j = br[row] + tid; j < br[row + 1]; j += kBsrBlock) {
+        for (int j = br[rw] + tid; j < br[rw + 1]; j += BLK) {
           XT xv[C];
           bsr_gather<C>(x, bc[j], xv);
+          const BsrVals<VT, R * C> w = bsr_load<VT, R, C>(bv, j, ps);
 #pragma unroll
-          for (int q = 0; q < C; ++q) acc += (AT)bv[(size_t)j * (R * C) + rr * C + q] * (AT)xv[q];
+          for (int q = 0; q < C; ++q) acc += (AT)w.a[rr * C + q] * (AT)xv[q];
         }
-        acc = block_sum<kBsrBlock>(acc, red);
-        if (tid == 0) epi(row * R + rr, acc);
+        acc = block_sum<BLK>(acc, red);
+        if (tid == 0) epi(rw * R + rr, acc, epi.pre(rw * R + rr));
       }
     }
   }
 }
 
-template <class XT, class E>
-void bsr_stream_f(const deco_bsr32* B, const XT* x, E epi) {
+template <class VT, class XT, class E>
+void bsr_stream_v(const deco_bsr* B, const VT* bv, const XT* x, E epi) {
   if (B->nb <= 0) return;
-#define DECO_BSR_LAUNCH(RR, CC)                                                                            \
-  hipLaunchKernelGGL((k_bsr_stream<RR, CC, XT, E>), dim3(B->nchunks), dim3(kBsrBlock), 0, stream(), B->rb.p, \
-                     B->br.p, B->bc.p, B->bv.p, x, epi)
+#define DECO_BSR_LAUNCH(RR, CC)                                                                                  \
+  hipLaunchKernelGGL((k_bsr_stream<VT, RR, CC, XT, E>), dim3(B->nchunks), dim3(kBsrBlock), 0, stream(), B->rb.p, \
+                     B->br.p, B->bc.p, bv, (size_t)B->nnzb * CC, x, epi)
   if (B->R == 2 && B->C == 2) DECO_BSR_LAUNCH(2, 2);
   else if (B->R == 1 && B->C == 2) DECO_BSR_LAUNCH(1, 2);
   else if (B->R == 3 && B->C == 3) DECO_BSR_LAUNCH(3, 3);
@@ -383,6 +478,19 @@ void bsr_stream_f(const deco_bsr32* B, const XT* x, E epi) {
   else fail(DECO_ERR_UNSUPPORTED, "block shape %d x %d", B->R, B->C);
 #undef DECO_BSR_LAUNCH
   DECO_CHECK_LAUNCH();
+}
+
+// product with the FP32 block values
+template <class XT, class E>
+void bsr_stream_f(const deco_bsr* B, const XT* x, E epi) {
+  bsr_stream_v<float>(B, B->bv.p, x, epi);
+}
+
+// product with the FP64 block values (block64() filled them)
+template <class E>
+void bsr_stream_d(const deco_bsr* B, const double* x, E epi) {
+  if (!B->has64) fail(DECO_ERR_STATE, "FP64 block values missing");
+  bsr_stream_v<double>(B, B->bv64.p, x, epi);
 }
 
 // Convenience wrappers on a deco_csr (values f64 or the f32 shadow).

@@ -396,7 +396,7 @@ struct FSAI : deco_prec {
   // y = F x with FP32 factor values (cell blocks when the unknowns come in cells), epilogue e
   template <class XT, class E>
   void f32_product(const deco_csr* F, const XT* x, E e) {
-    const deco_bsr32* B = cb > 1 ? F->block32(cb, cb) : nullptr;
+    const deco_bsr* B = cb > 1 ? F->block32(cb, cb) : nullptr;
     if (B) bsr_stream_f(B, x, e);
     else spmv_stream_f(F, const_cast<deco_csr*>(F)->values_f32(), x, e);
   }

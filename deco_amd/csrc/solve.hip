@@ -321,8 +321,13 @@ struct AOp {
   const deco_csr* A;
   bool transpose;
   void apply(const double* x, double* y) const {
-    if (!transpose) spmv_stream_f(A, A->v.p, x, EpiOuter{y});
-    else spmv(A->At, x, y);
+    const deco_csr* M = transpose ? A->At : A;
+    if (A->cell > 1)
+      if (const deco_bsr* B = M->block64(A->cell, A->cell)) {
+        bsr_stream_d(B, x, EpiOuter{y});
+        return;
+      }
+    spmv_stream_f(M, M->v.p, x, EpiOuter{y});
   }
   void residual(const double* x, const double* b, double* r) const {  // r = b - A x
     spmv_res(transpose ? A->At : A, x, b, r);

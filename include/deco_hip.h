@@ -188,6 +188,13 @@ int deco_spmv_f32(deco_csr* A, const float* x_dev, float* y_dev);
  * cell-structured operators of CPR.hpp (decoupled A, its pressure rows, the FSAI factors).
  * No reference counterpart (the reference's FP64 CSR product is alg::spmv). */
 int deco_spmv_block_f32(deco_csr* A, int R, int C, const float* x_dev, float* y_dev);
+/* y = A x through the same block shadow with FP64 values: the form of the solvers' product with
+ * the decoupled CPR matrix (CSR::apply, core/CSR.hpp:132-136, via alg::spmv, core/alg.hpp:52-63).
+ * Each row's sum runs over its blocks in column order, two products per block. */
+int deco_spmv_block(deco_csr* A, int R, int C, const double* x_dev, double* y_dev);
+/* deco_spmv_time for the block-shadow product (precision as there); nnzb (may be NULL)
+ * receives the number of R x C blocks. */
+int deco_spmv_block_time(deco_csr* A, int R, int C, int precision, int iters, double* avg_ms, int* nnzb);
 /* Times `iters` back-to-back SpMVs on the library stream with hipEvents and returns the
  * average kernel duration in milliseconds. precision: DECO_PREC_FP64 (f64 values/vectors)
  * or DECO_PREC_FP32 (f32 values/vectors). */

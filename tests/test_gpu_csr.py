@@ -137,6 +137,13 @@ def test_block_f32_product(gpu, cfg, shape):
     assert rel(A.apply_f32(x).numpy().astype(np.float64), ref) <= 2e-6
     with pytest.raises(deco_amd._lib.DecoError):
         A.apply_f32(x, block=(R, 5))
+    # FP64 values on the same block structure: the oracle's CSR product to FP64 rounding (the
+    # sums pair the two products of a block before adding it to the row)
+    xd = np.random.default_rng(4).uniform(-1, 1, S.shape[1])
+    yo = oracle.spmv(oracle.Mat.from_scipy(S), xd)
+    assert rel(A.apply(xd, block=(R, C)).numpy(), yo) <= 1e-14
+    with pytest.raises(deco_amd._lib.DecoError):
+        A.apply(xd, block=(R, 5))
 
 
 def test_spmv_wide_column_offsets(gpu):
