@@ -11,6 +11,7 @@ is measured with hipEvents on the library stream.
 Prints one JSON line.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -66,8 +67,8 @@ WORKLOAD_KEYS = {1: "two_phase_32x32x16", 11: "two_phase_32x32x32", 2: "black_oi
 
 def cpu_solve_baseline(g, M, prec_tree, sol, gpu_iters, args, log):
     """The oracle's OpenMP CPR-GMRES on the host cores, on the same system and the same AMG
-    hierarchy, timed over a bounded number of iterations and scaled to the GPU's iteration
-    count."""
+    hierarchy: the whole solve to the same tolerance (median of --cpu-reps runs), or, with
+    --cpu-iters k > 0, k iterations scaled to the GPU's iteration count."""
     import deco_amd
     import oracle
     n = g["n"]
@@ -80,18 +81,50 @@ def cpu_solve_baseline(g, M, prec_tree, sol, gpu_iters, args, log):
                R=oracle.Mat(l["nc"], l["n"], *l["R"]), d=l["d"], cf=l["cf"]) for l in levels]
     Mo.set_local(oracle.amg(lv, 0, cinv))
     setup = time.perf_counter() - t0
-    k = args.cpu_iters
-    sp = deco_amd.sol_params(dict(sol, itmax=str(k)))
-    t1 = time.perf_counter()
-    _, info = oracle.solve(sp, Ao, Mo, bo)
-    dt = time.perf_counter() - t1
-    per_it = dt / max(info.iter, 1)
     threads = oracle.num_threads()
-    log(f"CPU oracle: {info.iter} iterations in {dt:.2f}s ({threads} threads), setup+import {setup:.1f}s")
-    return {"value": per_it * gpu_iters, "unit": "s", "cores": threads, "kind": "port",
-            "sample": f"oracle CPR-{sol['type']}({sol['restart']}) on the same decoupled system and AMG hierarchy, {info.iter} "
-                      f"iterations timed ({dt:.2f}s, OpenMP {threads} threads), scaled to the {gpu_iters} "
-                      f"iterations of the GPU FP64 solve; per-iteration {per_it * 1e3:.1f} ms"}
+    k = args.cpu_iters
+    desc = f"oracle CPR-{sol['type']}({sol['restart']}) on the same decoupled system and AMG hierarchy"
+    if k > 0:
+        sp = deco_amd.sol_params(dict(sol, itmax=str(k)))
+        t1 = time.perf_counter()
+        _, info = oracle.solve(sp, Ao, Mo, bo)
+        dt = time.perf_counter() - t1
+        per_it = dt / max(info.iter, 1)
+        log(f"CPU oracle: {info.iter} iterations in {dt:.2f}s ({threads} threads), setup+import {setup:.1f}s")
+        return {"value": per_it * gpu_iters, "unit": "s", "cores": threads, "kind": "port",
+                "sample": f"{desc}, {info.iter} iterations timed ({dt:.2f}s, OpenMP {threads} threads), scaled to "
+                          f"the {gpu_iters} iterations of the GPU FP64 solve; per-iteration {per_it * 1e3:.1f} ms"}
+    sp = deco_amd.sol_params(sol)
+    times = []
+    for _ in range(max(args.cpu_reps, 1)):
+        t1 = time.perf_counter()
+        _, info = oracle.solve(sp, Ao, Mo, bo)
+        times.append(time.perf_counter() - t1)
+    dt = float(np.median(times))
+    log(f"CPU oracle: solve to rtol {sol['reltol']} in {info.iter} iterations, median {dt:.2f}s of {len(times)} "
+        f"({threads} threads), setup+import {setup:.1f}s")
+    # the same solve on 4 threads (the reference's run script uses 4 OpenMP threads)
+    oracle.set_threads(4)
+    t1 = time.perf_counter()
+    _, info4 = oracle.solve(sp, Ao, Mo, bo)
+    dt4 = time.perf_counter() - t1
+    oracle.set_threads(threads)
+    log(f"CPU oracle on 4 threads: {info4.iter} iterations in {dt4:.2f}s")
+    return {"value": dt, "unit": "s", "cores": threads, "kind": "port", "value_4_threads": dt4,
+            "host": cpu_model(),
+            "sample": f"{desc}: the whole solve to rtol {sol['reltol']} ({info.iter} iterations; the GPU FP64 solve "
+                      f"takes {gpu_iters}), median of {len(times)} runs, OpenMP {threads} threads; "
+                      f"per-iteration {dt / max(info.iter, 1) * 1e3:.1f} ms; one run on 4 threads {dt4:.2f}s"}
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return f"{line.split(':', 1)[1].strip()}, {os.cpu_count()} logical CPUs"
+    except OSError:
+        pass
+    return f"{os.cpu_count()} logical CPUs"
 
 
 def bsr_bytes(nb, mb, nnzb, R, C, sv, sx, sy):
@@ -134,6 +167,21 @@ def copy_calibration(deco_amd, nbytes=512 << 20, reps=5):
     return nbytes
 
 
+def copy_rate(deco_amd, nbytes=512 << 20, reps=20):
+    """Measured copy bandwidth on this GPU (bytes read + written per second, GB/s): back-to-back
+    512 MiB device-to-device copies between two device syncs."""
+    a = deco_amd.DeviceArray(nbytes // 8)
+    b = deco_amd.DeviceArray(nbytes // 8)
+    L = deco_amd.lib()
+    deco_amd.check(L.deco_memcpy_d2d(b.ptr, a.ptr, nbytes))
+    deco_amd.check(L.deco_sync())
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        deco_amd.check(L.deco_memcpy_d2d(b.ptr, a.ptr, nbytes))
+    deco_amd.check(L.deco_sync())
+    return 2 * nbytes * reps / (time.perf_counter() - t0) / 1e9
+
+
 def traffic_from_profiles(kind, workload_key):
     """HBM bytes per launch of an FP64 product with A measured by the FETCH_SIZE and WRITE_SIZE
     passes, calibrated against a plain copy (profiles/<kind>_<workload>_traffic.md), or None."""
@@ -159,7 +207,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cpu-solve", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--cpu-iters", type=int, default=5, help="GMRES iterations in the CPU baseline sample")
+    ap.add_argument("--cpu-iters", type=int, default=0,
+                    help="GMRES iterations in the CPU baseline sample (0: the whole solve)")
+    ap.add_argument("--cpu-reps", type=int, default=3, help="repetitions of the whole CPU solve (median)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (every GPU box here has one GPU); "
@@ -203,6 +253,11 @@ def main():
         "spmv": spmv,
         "upload_s": upload_s,
     }
+    copy_GBps = copy_rate(deco_amd)
+    rd = ctypes.c_double()
+    deco_amd.check(deco_amd.lib().deco_read_bandwidth(4 << 30, 10, ctypes.byref(rd)))   # 4 GiB: HBM, not cache
+    read_GBps = rd.value
+    log(f"device copy {copy_GBps:.0f} GB/s (read + write); streaming read {read_GBps:.0f} GB/s")
     # the dominant kernel: the solvers' FP64 product with A (cell-block form when A has cells)
     if "blocks" in spmv:
         bs = g["bs"]
@@ -211,12 +266,16 @@ def main():
             "frac": spmv["block_fp64_GBps"] / HBM_PEAK_GBS, "traffic": traffic_from_profiles("spmv_block_fp64", key),
             "kernel": f"k_bsr_stream<double,{bs},{bs},double,EpiOuter> (FP64 {bs}x{bs} cell-block product with A)",
             "bytes_per_launch": spmv["bytes_block_fp64"]}
+        result["roofline"].update(copy_GBps=copy_GBps, read_GBps=read_GBps,
+                                  frac_of_read=spmv["block_fp64_GBps"] / read_GBps)
     else:
         result["roofline"] = {
             "bound": "hbm", "achieved": spmv["fp64_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": spmv["fp64_GBps"] / HBM_PEAK_GBS, "traffic": traffic_from_profiles("spmv_fp64", key),
             "kernel": "k_spmv_stream<double,double,EpiOuter> (FP64 CSR SpMV with A)",
             "bytes_per_launch": spmv["bytes_fp64"]}
+        result["roofline"].update(copy_GBps=copy_GBps, read_GBps=read_GBps,
+                                  frac_of_read=spmv["fp64_GBps"] / read_GBps)
 
     # ---------------------------------------------------------------- in-cell reorder
     # AD::block_reorder, the driver's step before CPR (cpr.cpp:41), on a device copy of A

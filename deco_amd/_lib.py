@@ -99,6 +99,7 @@ SIGNATURES = {
     "deco_spmv_block_f32": (i, [vp, i, i, vp, vp]),
     "deco_spmv_time": (i, [vp, i, i, dp]),
     "deco_spmv_block": (i, [vp, i, i, vp, vp]),
+    "deco_read_bandwidth": (i, [C.c_size_t, i, dp]),
     "deco_spmv_block_time": (i, [vp, i, i, i, i, dp, ip]),
     "deco_spgemm": (i, [vp, vp, pp]),
     "deco_spgemm_tril_pattern": (i, [vp, pp]),

@@ -700,6 +700,32 @@ const deco_bsr* deco_csr::block32(int R, int C) const {
   return B;
 }
 
+namespace deco {
+// streaming read of n2 16-byte vectors (eight non-temporal loads in flight per lane), one
+// partial sum per workgroup: the read bandwidth the products' read-dominated traffic is compared
+// with (64 workgroups per CU and eight loads per lane measured fastest on MI355X)
+__global__ __launch_bounds__(256) void k_read_stream(const f64x2* __restrict__ p, size_t n2, double* __restrict__ part) {
+  constexpr int K = 8;
+  __shared__ double red[4];
+  double s = 0.0;
+  const size_t stride = (size_t)gridDim.x * 256;
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  for (; i + (K - 1) * stride < n2; i += K * stride) {
+    f64x2 t[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) t[k] = __builtin_nontemporal_load(p + i + k * stride);
+#pragma unroll
+    for (int k = 0; k < K; ++k) s += t[k].x + t[k].y;
+  }
+  for (; i < n2; i += stride) {
+    const f64x2 a = __builtin_nontemporal_load(p + i);
+    s += a.x + a.y;
+  }
+  s = block_sum<256>(s, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+}  // namespace deco
+
 const deco_bsr* deco_csr::block64(int R, int C) const {
   using namespace deco;
   const deco_bsr* B0 = block32(R, C);
@@ -964,6 +990,35 @@ int deco_spmv_block_f32(deco_csr* A, int R, int C, const float* x, float* y) {
     if (!B) fail(DECO_ERR_ARG, "%d x %d blocks do not tile a %d x %d matrix", R, C, A->n, A->m);
     bsr_stream_f(B, x, EpiStd<EPI_SET, float, float>{y, nullptr});
     sync();
+  });
+}
+
+int deco_read_bandwidth(size_t bytes, int iters, double* gbps) {
+  return guard([&] {
+    if (!gbps || iters <= 0 || bytes < ((size_t)1 << 20)) fail(DECO_ERR_ARG, "bad argument");
+    const size_t n2 = bytes / 16;
+    int dev = 0, cus = 0;
+    DECO_HIP(hipGetDevice(&dev));
+    DECO_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int G = std::max(cus, 1) * 64;
+    DBuf<double> buf(2 * n2), part(G);
+    buf.zero();
+    hipLaunchKernelGGL(k_read_stream, dim3(G), dim3(256), 0, stream(), (const f64x2*)buf.p, n2, part.p);
+    DECO_CHECK_LAUNCH();
+    hipEvent_t e0, e1;
+    DECO_HIP(hipEventCreate(&e0));
+    DECO_HIP(hipEventCreate(&e1));
+    DECO_HIP(hipEventRecord(e0, stream()));
+    for (int k = 0; k < iters; ++k)
+      hipLaunchKernelGGL(k_read_stream, dim3(G), dim3(256), 0, stream(), (const f64x2*)buf.p, n2, part.p);
+    DECO_CHECK_LAUNCH();
+    DECO_HIP(hipEventRecord(e1, stream()));
+    DECO_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    DECO_HIP(hipEventElapsedTime(&ms, e0, e1));
+    *gbps = (double)(n2 * 16) * iters / ((double)ms * 1e-3) / 1e9;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
   });
 }
 

@@ -192,6 +192,11 @@ int deco_spmv_block_f32(deco_csr* A, int R, int C, const float* x_dev, float* y_
  * the decoupled CPR matrix (CSR::apply, core/CSR.hpp:132-136, via alg::spmv, core/alg.hpp:52-63).
  * Each row's sum runs over its blocks in column order, two products per block. */
 int deco_spmv_block(deco_csr* A, int R, int C, const double* x_dev, double* y_dev);
+/* Read bandwidth of the current GPU in GB/s: a streaming read of `bytes` (at least 1 MiB) with
+ * 16-byte non-temporal loads, 64 workgroups per CU and one partial sum per workgroup, `iters`
+ * back-to-back launches timed with hipEvents (reads well past the 256 MB cache measure HBM). The reference rate for the products' read-dominated traffic; no reference
+ * counterpart. */
+int deco_read_bandwidth(size_t bytes, int iters, double* gbps);
 /* deco_spmv_time for the block-shadow product (precision as there); nnzb (may be NULL)
  * receives the number of R x C blocks. */
 int deco_spmv_block_time(deco_csr* A, int R, int C, int precision, int iters, double* avg_ms, int* nnzb);

@@ -61,6 +61,18 @@ def test_blas_dot_nrm2_axpby(gpu):
     assert deco_amd.dot(x, y) == deco_amd.dot(x, y)
 
 
+def test_read_bandwidth_probe(gpu):
+    """The read-bandwidth measurement the bench reports beside the roofline: a positive rate
+    below the published HBM peak, and argument errors reported as errors."""
+    import ctypes
+    L = deco_amd.lib()
+    g = ctypes.c_double()
+    deco_amd.check(L.deco_read_bandwidth(1 << 30, 3, ctypes.byref(g)))   # 1 GiB: past the 256 MB cache
+    assert 0.0 < g.value < 8000.0
+    assert L.deco_read_bandwidth(1024, 3, ctypes.byref(g)) != 0
+    assert L.deco_read_bandwidth(64 << 20, 0, ctypes.byref(g)) != 0
+
+
 def test_transpose_filters_local_matrix_bitexact(gpu):
     g = gen_system(2)
     bs = g["bs"]
