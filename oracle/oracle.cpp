@@ -5,7 +5,7 @@
 // tests, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it; the product library
 // never does.
 //
-// Parity status of each part (see DESIGN.md for the checks that pin them):
+// Parity status of each part (tests/test_oracle_*.py hold the checks that pin them):
 //   spmv/BLAS, filters, local_matrix, transpose, SpGEMM, FSAI pattern     pinned (closed forms,
 //       scipy.sparse, brute force)
 //   CPR decoupling (fix_diag, weights, DRS, IMPES, CPD)                   pinned (invariants)
