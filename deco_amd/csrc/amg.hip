@@ -87,7 +87,8 @@ __global__ void k_relax_zero(int n, const T* __restrict__ b, const T* __restrict
   x[i] = on ? (T)0 + (b[i] - (T)0) / d[i] : (T)0;
 }
 
-// y = x + (b - A x)/d on the subset rows, y = x elsewhere
+// y = x + (b - A x)/d on the subset rows, y = x elsewhere; with add_to set, also
+// add_to[o*stride] += y (k_add_pressure's update, from the finest level's last sweep)
 template <class T>
 struct EpiRelax {
   const T* __restrict__ x;
@@ -96,13 +97,18 @@ struct EpiRelax {
   const int* __restrict__ cf;
   int subset;
   T* __restrict__ y;
-  struct Pre { T x, b, d; bool on; };
+  T* __restrict__ add_to = nullptr;
+  int stride = 0;
+  struct Pre { T x, b, d, a; bool on; };
   __device__ __forceinline__ Pre pre(int o) const {
-    return Pre{x[o], b[o], d[o], subset < 0 || (subset == 1 ? cf[o] == 1 : cf[o] != 1)};
+    return Pre{x[o], b[o], d[o], add_to ? add_to[(size_t)o * stride] : (T)0,
+               subset < 0 || (subset == 1 ? cf[o] == 1 : cf[o] != 1)};
   }
   template <class AT>
   __device__ __forceinline__ void operator()(int o, AT acc, const Pre& p) const {
-    y[o] = p.on ? (T)(p.x + (p.b - (T)acc) / p.d) : p.x;
+    const T yo = p.on ? (T)(p.x + (p.b - (T)acc) / p.d) : p.x;
+    y[o] = yo;
+    if (add_to) add_to[(size_t)o * stride] = p.a + yo;
   }
 };
 
@@ -295,13 +301,16 @@ struct AMG : deco_prec {
   template <class T> T* BC();
 
   template <class T>
-  void relax_sweep(Level& lv, const deco_csr* A, const T* x, const T* b, T* y, int subset) {
-    spmv_stream_f(A, vals<T>(const_cast<deco_csr*>(A)), x, EpiRelax<T>{x, b, D<T>(lv), lv.cf.p, subset, y});
+  void relax_sweep(Level& lv, const deco_csr* A, const T* x, const T* b, T* y, int subset, T* add_to = nullptr,
+                   int stride = 0) {
+    spmv_stream_f(A, vals<T>(const_cast<deco_csr*>(A)), x,
+                  EpiRelax<T>{x, b, D<T>(lv), lv.cf.p, subset, y, add_to, stride});
   }
 
-  // one V-cycle on level l with right-hand side b, result in x (x != b)
+  // one V-cycle on level l with right-hand side b, result in x (x != b); x_ready: x already
+  // holds the zero-guess sweep; add_to: the last sweep also adds the result to add_to[stride*i]
   template <class T>
-  void cycle(size_t l, const T* b, T* x, bool tr, bool x_ready = false) {
+  void cycle(size_t l, const T* b, T* x, bool tr, bool x_ready = false, T* add_to = nullptr, int stride = 0) {
     if (l == L.size()) {
       if (nc == 0) return;
       hipLaunchKernelGGL(k_coarse<T>, dim3((nc + 3) / 4), dim3(256), 0, stream(), nc,
@@ -341,9 +350,9 @@ struct AMG : deco_prec {
     if (relax == 17) {
       relax_sweep<T>(lv, A, t, b, x, 0);
       relax_sweep<T>(lv, A, x, b, t, 1);
-      relax_sweep<T>(lv, A, t, b, x, 0);
+      relax_sweep<T>(lv, A, t, b, x, 0, add_to, stride);
     } else {
-      relax_sweep<T>(lv, A, t, b, x, -1);
+      relax_sweep<T>(lv, A, t, b, x, -1, add_to, stride);
     }
   }
 
@@ -374,6 +383,28 @@ struct AMG : deco_prec {
     run<float>(r, x, false);
   }
   void apply32_t(const float* r, float* x) override { run<float>(r, x, true); }
+
+  // fused CPR pressure stage: the caller's residual epilogue performs the finest level's
+  // zero-guess Jacobi sweep (x = b/d with this diagonal), the last sweep adds the result to the
+  // caller's iterate (Jacobi-type smoothers with at least one level below the finest)
+  bool stage_diag(const double** d0) override {
+    if (relax == 17 || L.empty()) return false;
+    *d0 = L[0].d.p;
+    return true;
+  }
+  bool stage_diag32(const float** d0) override {
+    if (relax == 17 || L.empty()) return false;
+    *d0 = L[0].df.p;
+    return true;
+  }
+  void stage(const double* b, double* x, double* add_to, int stride) override {
+    ProfPhase ph("AMG apply");
+    cycle<double>(0, b, x, false, true, add_to, stride);
+  }
+  void stage32(const float* b, float* x, float* add_to, int stride) override {
+    ProfPhase ph("AMG apply");
+    cycle<float>(0, b, x, false, true, add_to, stride);
+  }
 };
 
 template <> double* AMG::X<double>(Level& l) { return l.x.p; }

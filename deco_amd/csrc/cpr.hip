@@ -556,6 +556,15 @@ struct CPR : deco_prec {
     if (const deco_bsr* B = Apr->block32(1, bs)) bsr_stream_f(B, x, EpiStrideRes<float, double>{out, r, bs});
     else spmv_stream_f(Apr, Apr->values_f32(), x, EpiStrideRes<float, double>{out, r, bs});
   }
+  // the same, also writing the AMG's first zero-guess sweep x0 = rt/d0
+  void pressure_residual_j0(const double* x, const double* r, double* out, const double* d0, double* x0) {
+    spmv_stream_f(Apr, Apr->v.p, x, EpiStrideResJ0<double, double>{out, r, bs, d0, x0});
+  }
+  void pressure_residual_j0(const float* x, const double* r, float* out, const float* d0, float* x0) {
+    if (const deco_bsr* B = Apr->block32(1, bs))
+      bsr_stream_f(B, x, EpiStrideResJ0<float, double>{out, r, bs, d0, x0});
+    else spmv_stream_f(Apr, Apr->values_f32(), x, EpiStrideResJ0<float, double>{out, r, bs, d0, x0});
+  }
   // out = r - A x with FP32 values of the decoupled A (cell blocks)
   void residual32(const float* x, const double* r, float* out) {
     if (const deco_bsr* B = A->block32(bs, bs)) bsr_stream_f(B, x, EpiStd<EPI_RES, float, double>{out, r});
@@ -579,15 +588,24 @@ struct CPR : deco_prec {
       return;
     }
     if (seq == 2) jac->apply(r, x); else glo_apply(r, x);
+    // With an AMG of Jacobi-type smoothers the residual epilogue also performs the finest level's
+    // zero-guess sweep, and the cycle's last sweep adds xp into x (same arithmetic as the
+    // separate k_relax_zero and k_add_pressure).
+    const double* d0 = nullptr;
+    const bool fused = loc->stage_diag(&d0);
     {
       ProfPhase ph("CPR_rest");                                          // CPR.hpp:240-296
-      pressure_residual((const double*)x, r, rt.p);
+      if (fused) pressure_residual_j0((const double*)x, r, rt.p, d0, xp.p);
+      else pressure_residual((const double*)x, r, rt.p);
     }
-    loc->apply(rt.p, xp.p);
+    if (fused) loc->stage(rt.p, xp.p, x, bs);
+    else loc->apply(rt.p, xp.p);
     ProfPhase ph_rest("CPR_rest");
-    hipLaunchKernelGGL((k_add_pressure<double, double>), dim3(gs()), dim3(256), 0, stream(), sn, bs, x,
-                       (const double*)xp.p);
-    DECO_CHECK_LAUNCH();
+    if (!fused) {
+      hipLaunchKernelGGL((k_add_pressure<double, double>), dim3(gs()), dim3(256), 0, stream(), sn, bs, x,
+                         (const double*)xp.p);
+      DECO_CHECK_LAUNCH();
+    }
     if (seq == 0) return;
     spmv_res(A, x, r, rt.p);
     ph_rest.end();
@@ -621,15 +639,21 @@ struct CPR : deco_prec {
     }
     if (seq == 2) { convert(r, t32.p, n); jac->apply32(t32.p, x32.p); }
     else glo32_from64(r, x32.p);
+    const float* d0 = nullptr;
+    const bool fused = loc->stage_diag32(&d0);
     {
       ProfPhase ph("CPR_rest");
-      pressure_residual((const float*)x32.p, r, rt32.p);
+      if (fused) pressure_residual_j0((const float*)x32.p, r, rt32.p, d0, xp32.p);
+      else pressure_residual((const float*)x32.p, r, rt32.p);
     }
-    loc->apply32(rt32.p, xp32.p);
+    if (fused) loc->stage32(rt32.p, xp32.p, x32.p, bs);
+    else loc->apply32(rt32.p, xp32.p);
     ProfPhase ph_rest("CPR_rest");
-    hipLaunchKernelGGL((k_add_pressure<float, float>), dim3(gs()), dim3(256), 0, stream(), sn, bs, x32.p,
-                       (const float*)xp32.p);
-    DECO_CHECK_LAUNCH();
+    if (!fused) {
+      hipLaunchKernelGGL((k_add_pressure<float, float>), dim3(gs()), dim3(256), 0, stream(), sn, bs, x32.p,
+                         (const float*)xp32.p);
+      DECO_CHECK_LAUNCH();
+    }
     if (seq == 0) { convert(x32.p, xo, n); return; }
     residual32(x32.p, r, rt32.p);
     ph_rest.end();

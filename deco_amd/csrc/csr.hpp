@@ -181,6 +181,25 @@ struct EpiStrideRes {
   __device__ __forceinline__ void operator()(int o, AT acc, const Pre& p) const { y[o] = (YT)((AT)p.a - acc); }
 };
 
+// EpiStrideRes that also writes x0[o] = 0 + (y[o] - 0)/d[o]: the zero-guess Jacobi-type sweep on
+// the residual it produces (the AMG's k_relax_zero arithmetic), so the cycle starts one launch later
+template <class YT, class ZT>
+struct EpiStrideResJ0 {
+  YT* __restrict__ y;
+  const ZT* __restrict__ z;
+  int zs;
+  const YT* __restrict__ d;
+  YT* __restrict__ x0;
+  struct Pre { ZT a; YT d; };
+  __device__ __forceinline__ Pre pre(int o) const { return Pre{z[(size_t)o * zs], d[o]}; }
+  template <class AT>
+  __device__ __forceinline__ void operator()(int o, AT acc, const Pre& p) const {
+    const YT bo = (YT)((AT)p.a - acc);
+    y[o] = bo;
+    x0[o] = (YT)0 + (bo - (YT)0) / p.d;
+  }
+};
+
 // x access of the stream kernel: a plain vector, or a functor that computes x_j (fused steps)
 template <class XT>
 struct PtrGather {

@@ -25,6 +25,20 @@ struct deco_prec {
     deco::fail(DECO_ERR_UNSUPPORTED, "FP32 transposed apply not available for this preconditioner");
   }
   virtual void set_precision(int p) { precision = p; }
+  // Pressure stage of CPR with two steps folded in (AMG only; false: not available, nothing
+  // done). d0 receives the diagonal of the first zero-guess sweep x = b/d, which the caller's
+  // residual epilogue then applies itself; stage() runs the rest of the cycle on (b, x) and adds
+  // the result to add_to[stride*i] from its last kernel.
+  virtual bool stage_diag(const double** d0) { (void)d0; return false; }
+  virtual bool stage_diag32(const float** d0) { (void)d0; return false; }
+  virtual void stage(const double* b, double* x, double* add_to, int stride) {
+    (void)b; (void)x; (void)add_to; (void)stride;
+    deco::fail(DECO_ERR_STATE, "fused pressure stage not available");
+  }
+  virtual void stage32(const float* b, float* x, float* add_to, int stride) {
+    (void)b; (void)x; (void)add_to; (void)stride;
+    deco::fail(DECO_ERR_STATE, "fused pressure stage not available");
+  }
 };
 
 namespace deco {

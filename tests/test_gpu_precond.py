@@ -126,18 +126,26 @@ def test_cpr_setup_bitexact_and_apply_parity(gpu, cfg):
     M.set_precision(deco_amd.PREC_FP64)
 
 
-@pytest.mark.parametrize("seq", [0, 1, 2])
-def test_cpr_sequences(gpu, seq):
+@pytest.mark.parametrize("seq,relax", [(0, 18), (1, 18), (2, 18), (3, 17), (0, 17)])
+def test_cpr_sequences(gpu, seq, relax):
+    """CPR sequences 0-3 against the oracle. With a Jacobi-type smoother (18) the pressure stage
+    runs fused (the residual epilogue does the first sweep, the last sweep adds into x); with
+    F-C-F half-sweeps (17) it runs as separate kernels. FP32 against FP64 on the device."""
     g = gen_system(4)
-    tree = small_params(2, seq=str(seq))
+    tree = small_params(2, seq=str(seq), **{"prec_loc.amg_relax_type": relax})
     A = dev_csr(g)
     M = deco_amd.CPR(tree, A, None)
     Mo_A = to_mat(g)
     Mo = oracle.CPR(deco_amd.cpr_params(tree), Mo_A, None)
-    Go, _, _ = oracle_amg(M.amg(), 18)
+    Go, _, _ = oracle_amg(M.amg(), relax)
     Mo.set_local(Go)
     x = np.random.default_rng(seq).uniform(-1, 1, g["n"])
-    assert rel(M.apply(x).numpy(), Mo.apply(x)) <= 1e-12
+    y = M.apply(x).numpy()
+    assert rel(y, Mo.apply(x)) <= 1e-12
+    M.set_precision(deco_amd.PREC_FP32)
+    y32 = M.apply(x).numpy()
+    M.set_precision(deco_amd.PREC_FP64)
+    assert rel(y32, y) <= 2e-4
 
 
 @pytest.mark.parametrize("relax,coarse", [(17, 10), (18, 8), (0, 10)])
