@@ -129,6 +129,77 @@ struct EpiRestrictJacobi0 {
   }
 };
 
+// Up-sweep of a level with a Jacobi-type smoother as one product: with t = x + P e and the
+// residual r = b - A x kept from the down-sweep, t + (b - A t)/d = x + r/d + S e for
+// S = P - D^{-1} (A P). Writes x (and add_to[o*stride] += x when set).
+template <class T>
+struct EpiUp {
+  T* __restrict__ x;
+  const T* __restrict__ r;
+  const T* __restrict__ d;
+  T* __restrict__ add_to;
+  int stride;
+  struct Pre { T x, r, d, a; };
+  __device__ __forceinline__ Pre pre(int o) const {
+    return Pre{x[o], r[o], d[o], add_to ? add_to[(size_t)o * stride] : (T)0};
+  }
+  template <class AT>
+  __device__ __forceinline__ void operator()(int o, AT acc, const Pre& p) const {
+    const T y = (T)(p.x + p.r / p.d) + (T)acc;
+    x[o] = y;
+    if (add_to) add_to[(size_t)o * stride] = p.a + y;
+  }
+};
+
+// S = P - D^{-1} Q (Q = A P) on the union of the two row patterns (both sorted by column)
+__global__ void k_upop_count(int n, const int* __restrict__ pr, const int* __restrict__ pc, const int* __restrict__ qr,
+                             const int* __restrict__ qc, int* __restrict__ cnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int a = pr[i], b = qr[i], k = 0;
+  const int ae = pr[i + 1], be = qr[i + 1];
+  while (a < ae || b < be) {
+    if (b >= be || (a < ae && pc[a] < qc[b])) ++a;
+    else if (a >= ae || qc[b] < pc[a]) ++b;
+    else { ++a; ++b; }
+    ++k;
+  }
+  cnt[i] = k;
+}
+__global__ void k_upop_fill(int n, const int* __restrict__ pr, const int* __restrict__ pc, const double* __restrict__ pv,
+                            const int* __restrict__ qr, const int* __restrict__ qc, const double* __restrict__ qv,
+                            const double* __restrict__ d, const int* __restrict__ sr, int* __restrict__ sc,
+                            double* __restrict__ sv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int a = pr[i], b = qr[i], o = sr[i];
+  const int ae = pr[i + 1], be = qr[i + 1];
+  const double di = d[i];
+  while (a < ae || b < be) {
+    if (b >= be || (a < ae && pc[a] < qc[b])) { sc[o] = pc[a]; sv[o] = pv[a]; ++a; }
+    else if (a >= ae || qc[b] < pc[a]) { sc[o] = qc[b]; sv[o] = -(qv[b] / di); ++b; }
+    else { sc[o] = pc[a]; sv[o] = pv[a] - qv[b] / di; ++a; ++b; }
+    ++o;
+  }
+}
+
+static deco_csr* up_operator(const deco_csr* P, const deco_csr* Q, const double* d) {
+  deco_csr* S = csr_new(P->n, P->m);
+  DBuf<int> cnt(P->n + 1);
+  if (P->n) {
+    hipLaunchKernelGGL(k_upop_count, dim3(grid_for(P->n, 256)), dim3(256), 0, stream(), P->n, P->r.p, P->c.p, Q->r.p,
+                       Q->c.p, cnt.p);
+    DECO_CHECK_LAUNCH();
+  }
+  build_row_ptr(S, cnt.p, P->n);
+  if (P->n) {
+    hipLaunchKernelGGL(k_upop_fill, dim3(grid_for(P->n, 256)), dim3(256), 0, stream(), P->n, P->r.p, P->c.p, P->v.p,
+                       Q->r.p, Q->c.p, Q->v.p, d, (const int*)S->r.p, S->c.p, S->v.p);
+    DECO_CHECK_LAUNCH();
+  }
+  return S;
+}
+
 // coarse solve x = C b, C row-major with leading dimension nc (the inverse for the forward
 // cycle, its transpose for the transposed one): one wavefront per row, lanes stride the
 // columns, fixed-order butterfly sum; input/output in T, FP64 arithmetic
@@ -151,6 +222,9 @@ struct AMG : deco_prec {
     deco_csr* At = nullptr;   // transposed level matrix (transposed apply)
     deco_csr* P = nullptr;
     deco_csr* R = nullptr;
+    deco_csr* AP = nullptr;   // A P from the Galerkin product, until S is formed
+    deco_csr* S = nullptr;    // up-sweep operator P - D^{-1} A P (Jacobi-type smoothers)
+    deco_csr* St = nullptr;   // the same with A^T, for the transposed cycle
     DBuf<double> d;
     DBuf<float> df;
     DBuf<int> cf;
@@ -225,8 +299,8 @@ struct AMG : deco_prec {
         ph_t.end();
         ProfPhase ph_rap("Galerkin RAP");
         deco_csr* AP = spgemm(Al, lv.P);
+        lv.AP = AP;
         deco_csr* RAP = spgemm(lv.R, AP);
-        delete AP;
         if (p.amg_A_drop_tol > 0.0) {
           Al = drop_small(RAP, p.amg_A_drop_tol);
           delete RAP;
@@ -255,6 +329,10 @@ struct AMG : deco_prec {
       convert(lv.d.p, lv.df.p, nl);
       lv.x.alloc(nl); lv.b.alloc(nl); lv.t.alloc(nl); lv.r.alloc(nl);
       lv.xf.alloc(nl); lv.bf.alloc(nl); lv.tf.alloc(nl); lv.rf.alloc(nl);
+      // the up-sweep's prolongation and Jacobi-type sweep as one operator
+      if (relax != 17) lv.S = up_operator(lv.P, lv.AP, lv.d.p);
+      delete lv.AP;
+      lv.AP = nullptr;
     }
     // coarsest: dense FP64 inverse
     ProfPhase ph_ci("coarse inverse");
@@ -283,7 +361,10 @@ struct AMG : deco_prec {
   }
 
   void release() {
-    for (auto& lv : L) { delete lv.A; delete lv.At; delete lv.P; delete lv.R; lv.A = lv.At = lv.P = lv.R = nullptr; }
+    for (auto& lv : L) {
+      delete lv.A; delete lv.At; delete lv.P; delete lv.R; delete lv.AP; delete lv.S; delete lv.St;
+      lv.A = lv.At = lv.P = lv.R = lv.AP = lv.S = lv.St = nullptr;
+    }
     L.clear();
     delete Ac;
     Ac = nullptr;
@@ -345,6 +426,10 @@ struct AMG : deco_prec {
     else
       spmv_stream<EPI_SET, T, T, T, T>(lv.R, vals<T>(lv.R), r, bn, (const T*)nullptr);
     cycle<T>(l + 1, bn, xn, tr, fuse);
+    if (deco_csr* S = tr ? lv.St : lv.S) {   // x <- x + r/d + S xn: prolongation and sweep in one product
+      spmv_stream_f(S, vals<T>(S), (const T*)xn, EpiUp<T>{x, (const T*)r, D<T>(lv), add_to, stride});
+      return;
+    }
     // prolongation t = x + P xn, then the up smoother back into x
     spmv_stream<EPI_ADDZ, T, T, T, T>(lv.P, vals<T>(lv.P), xn, t, x);
     if (relax == 17) {
@@ -358,7 +443,14 @@ struct AMG : deco_prec {
 
   void ensure_t() {
     if (have_t) return;
-    for (auto& lv : L) lv.At = csr_transpose(lv.A);
+    for (auto& lv : L) {
+      lv.At = csr_transpose(lv.A);
+      if (relax != 17) {
+        deco_csr* Q = spgemm(lv.At, lv.P);
+        lv.St = up_operator(lv.P, Q, lv.d.p);
+        delete Q;
+      }
+    }
     have_t = true;
   }
 
