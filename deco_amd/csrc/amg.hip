@@ -237,6 +237,7 @@ struct AMG : deco_prec {
   DBuf<double> CT, Cinv;      // coarse inverse: row-major (forward) and its transpose (transposed cycle)
   DBuf<double> xc, bc;
   DBuf<float> xcf, bcf;
+  DBuf<float> r32, x32;       // FP32 copies of a standalone apply's input and output
   int relax = 18;
   double weight = 1.0;
   bool have_t = false;
@@ -465,11 +466,23 @@ struct AMG : deco_prec {
     }
     cycle<T>(0, r, x, tr);
   }
+  // FP64 in and out; with FP32 precision the cycle runs on FP32 copies
+  void run64(const double* r, double* x, bool tr) {
+    if (precision != DECO_PREC_FP32) {
+      run<double>(r, x, tr);
+      return;
+    }
+    r32.ensure(n);
+    x32.ensure(n);
+    convert(r, r32.p, n);
+    run<float>(r32.p, x32.p, tr);
+    convert(x32.p, x, n);
+  }
   void apply(const double* r, double* x) override {
     ProfPhase ph("AMG apply");                                           // hypre_AMG.hpp:27-35
-    run<double>(r, x, false);
+    run64(r, x, false);
   }
-  void apply_t(const double* r, double* x) override { run<double>(r, x, true); }
+  void apply_t(const double* r, double* x) override { run64(r, x, true); }
   void apply32(const float* r, float* x) override {
     ProfPhase ph("AMG apply");
     run<float>(r, x, false);

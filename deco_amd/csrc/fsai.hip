@@ -371,6 +371,11 @@ struct FSAI : deco_prec {
 
   void apply(const double* r, double* x) override {
     ProfPhase ph("FSAI apply");                                          // FSAI.hpp:42-45
+    if (precision == DECO_PREC_FP32) {   // FP32 factors, FP64 vectors in and out
+      f32_product(FL, r, EpiStd<EPI_SET, float, float>{temp32.p, nullptr});
+      f32_product(FUt, (const float*)temp32.p, EpiStd<EPI_SET, double, float>{x, nullptr});
+      return;
+    }
     spmv(FL, r, temp.p);
     spmv(FUt, temp.p, x);
   }
@@ -385,6 +390,11 @@ struct FSAI : deco_prec {
   }
   void apply_t(const double* r, double* x) override {  // (FUt FL)^T = FLt FU
     ensure_t();
+    if (precision == DECO_PREC_FP32) {
+      f32_product(FU, r, EpiStd<EPI_SET, float, float>{temp32.p, nullptr});
+      f32_product(FLt, (const float*)temp32.p, EpiStd<EPI_SET, double, float>{x, nullptr});
+      return;
+    }
     spmv(FU, r, temp.p);
     spmv(FLt, temp.p, x);
   }

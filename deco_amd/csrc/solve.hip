@@ -886,16 +886,17 @@ __global__ void k_jacobi_point_setup(int n, const int* __restrict__ r, const int
   J[i] = 1 / v[mid];
 }
 
-template <class T, bool TR>
-__global__ void k_jacobi_apply(int n, int bs, const T* __restrict__ J, const T* __restrict__ x, T* __restrict__ b) {
+// b = J x with J in T; vectors in IO (IO = T, or FP64 vectors around FP32 inverses)
+template <class T, class IO, bool TR>
+__global__ void k_jacobi_apply(int n, int bs, const T* __restrict__ J, const IO* __restrict__ x, IO* __restrict__ b) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  if (bs == 1) { b[i] = J[i] * x[i]; return; }
+  if (bs == 1) { b[i] = (IO)(J[i] * (T)x[i]); return; }
   const int s = i / bs, p = i - s * bs;
   const T* Ap = J + (size_t)s * bs * bs;
-  T sum = (TR ? Ap[p] : Ap[p * bs]) * x[s * bs];
-  for (int k = 1; k < bs; ++k) sum += (TR ? Ap[k * bs + p] : Ap[p * bs + k]) * x[s * bs + k];
-  b[i] = sum;
+  T sum = (TR ? Ap[p] : Ap[p * bs]) * (T)x[s * bs];
+  for (int k = 1; k < bs; ++k) sum += (TR ? Ap[k * bs + p] : Ap[p * bs + k]) * (T)x[s * bs + k];
+  b[i] = (IO)sum;
 }
 
 Jacobi::Jacobi(const deco_csr* A, int bs_) {
@@ -918,22 +919,30 @@ Jacobi::Jacobi(const deco_csr* A, int bs_) {
   convert(J.p, Jf.p, J.n);
 }
 void Jacobi::apply(const double* r, double* x) {
-  hipLaunchKernelGGL((k_jacobi_apply<double, false>), dim3(grid_for(n, 256)), dim3(256), 0, stream(), n, bs,
-                     (const double*)J.p, r, x);
+  if (precision == DECO_PREC_FP32)
+    hipLaunchKernelGGL((k_jacobi_apply<float, double, false>), dim3(grid_for(n, 256)), dim3(256), 0, stream(), n, bs,
+                       (const float*)Jf.p, r, x);
+  else
+    hipLaunchKernelGGL((k_jacobi_apply<double, double, false>), dim3(grid_for(n, 256)), dim3(256), 0, stream(), n,
+                       bs, (const double*)J.p, r, x);
   DECO_CHECK_LAUNCH();
 }
 void Jacobi::apply_t(const double* r, double* x) {
-  hipLaunchKernelGGL((k_jacobi_apply<double, true>), dim3(grid_for(n, 256)), dim3(256), 0, stream(), n, bs,
-                     (const double*)J.p, r, x);
+  if (precision == DECO_PREC_FP32)
+    hipLaunchKernelGGL((k_jacobi_apply<float, double, true>), dim3(grid_for(n, 256)), dim3(256), 0, stream(), n, bs,
+                       (const float*)Jf.p, r, x);
+  else
+    hipLaunchKernelGGL((k_jacobi_apply<double, double, true>), dim3(grid_for(n, 256)), dim3(256), 0, stream(), n,
+                       bs, (const double*)J.p, r, x);
   DECO_CHECK_LAUNCH();
 }
 void Jacobi::apply32(const float* r, float* x) {
-  hipLaunchKernelGGL((k_jacobi_apply<float, false>), dim3(grid_for(n, 256)), dim3(256), 0, stream(), n, bs,
+  hipLaunchKernelGGL((k_jacobi_apply<float, float, false>), dim3(grid_for(n, 256)), dim3(256), 0, stream(), n, bs,
                      (const float*)Jf.p, r, x);
   DECO_CHECK_LAUNCH();
 }
 void Jacobi::apply32_t(const float* r, float* x) {
-  hipLaunchKernelGGL((k_jacobi_apply<float, true>), dim3(grid_for(n, 256)), dim3(256), 0, stream(), n, bs,
+  hipLaunchKernelGGL((k_jacobi_apply<float, float, true>), dim3(grid_for(n, 256)), dim3(256), 0, stream(), n, bs,
                      (const float*)Jf.p, r, x);
   DECO_CHECK_LAUNCH();
 }

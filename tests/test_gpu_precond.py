@@ -73,8 +73,15 @@ def test_fsai_factors_parity(gpu, cfg, power):
     # same LAPACK-order LU: equal up to the last bits
     assert rel(vL, FLo.v) <= 1e-13 and rel(vU, FUto.v) <= 1e-13
     x = np.random.default_rng(1).uniform(-1, 1, g["n"])
-    assert rel(F.apply(x).numpy(), Fo.apply(x)) <= 1e-13
-    assert rel(F.apply_t(x).numpy(), Fo.apply_t(x)) <= 1e-13
+    y, yt = F.apply(x).numpy(), F.apply_t(x).numpy()
+    assert rel(y, Fo.apply(x)) <= 1e-13
+    assert rel(yt, Fo.apply_t(x)) <= 1e-13
+    # FP32 factors (cell blocks), FP64 vectors in and out
+    F.set_precision(deco_amd.PREC_FP32)
+    y32, yt32 = F.apply(x).numpy(), F.apply_t(x).numpy()
+    F.set_precision(deco_amd.PREC_FP64)
+    assert rel(y32, y) <= 1e-5 and rel(yt32, yt) <= 1e-5
+    assert not np.array_equal(y32, y)                     # the FP32 path did run
 
 
 @pytest.mark.parametrize("cfg", [4, 2])
@@ -159,7 +166,15 @@ def test_amg_variants(gpu, relax, coarse):
     Go, levels, _ = oracle_amg(G, relax)
     assert len(levels) >= 2
     x = np.random.default_rng(4).uniform(-1, 1, Ap.n)
-    assert rel(G.apply(x).numpy(), Go.apply(x)) <= 1e-12
+    y = G.apply(x).numpy()
+    # for the Jacobi-type smoothers the device's up-sweep is one product with P - D^-1 A P (and
+    # P - D^-1 A^T P transposed), the oracle's the two steps of the reference form
+    assert rel(y, Go.apply(x)) <= 1e-12
+    assert rel(G.apply_t(x).numpy(), Go.apply_t(x)) <= 1e-12
+    G.set_precision(deco_amd.PREC_FP32)
+    y32 = G.apply(x).numpy()
+    G.set_precision(deco_amd.PREC_FP64)
+    assert rel(y32, y) <= 1e-5 and not np.array_equal(y32, y)
 
 
 def test_jacobi_block_parity(gpu):
@@ -168,8 +183,14 @@ def test_jacobi_block_parity(gpu):
     J = deco_amd.Jacobi({"block_size": "3"}, A)
     Jo = oracle.jacobi(3, to_mat(g))
     x = np.random.default_rng(5).uniform(-1, 1, g["n"])
-    assert rel(J.apply(x).numpy(), Jo.apply(x)) <= 1e-15
-    assert rel(J.apply_t(x).numpy(), Jo.apply_t(x)) <= 1e-15
+    y, yt = J.apply(x).numpy(), J.apply_t(x).numpy()
+    assert rel(y, Jo.apply(x)) <= 1e-15
+    assert rel(yt, Jo.apply_t(x)) <= 1e-15
+    J.set_precision(deco_amd.PREC_FP32)                   # FP32 inverses, FP64 vectors
+    y32, yt32 = J.apply(x).numpy(), J.apply_t(x).numpy()
+    J.set_precision(deco_amd.PREC_FP64)
+    assert rel(y32, y) <= 1e-6 and rel(yt32, yt) <= 1e-6
+    assert not np.array_equal(y32, y)                     # the FP32 path did run
 
 
 def test_spgemm_long_rows(gpu):
