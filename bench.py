@@ -127,10 +127,12 @@ def cpu_model():
     return f"{os.cpu_count()} logical CPUs"
 
 
-def bsr_bytes(nb, mb, nnzb, R, C, sv, sx, sy):
-    # algorithmic bytes of the block product: block values + block columns + block row pointers
-    # + x + y
-    return nnzb * (R * C * sv + 4) + 4 * (nb + 1) + sx * mb * C + sy * nb * R
+def bsr_bytes(nb, mb, nnzb, R, C, sv, sx, sy, sliced=False):
+    # algorithmic bytes of the block product: block values + block columns + x + y, and the row
+    # structure the layout reads: block row pointers (chunked) or per-slice offsets and widths
+    # of 64 block rows (sliced; its padding blocks are not counted)
+    structure = 8 * ((nb + 63) // 64) if sliced else 4 * (nb + 1)
+    return nnzb * (R * C * sv + 4) + structure + sx * mb * C + sy * nb * R
 
 
 def spmv_section(deco_amd, A, n, nnz, iters, bs):
@@ -147,9 +149,12 @@ def spmv_section(deco_amd, A, n, nnz, iters, bs):
     if bs > 1:
         msb64, nnzb = A.spmv_time(P64, iters, block=(bs, bs))
         msb32, _ = A.spmv_time(P32, iters, block=(bs, bs))
-        bb64 = bsr_bytes(n // bs, n // bs, nnzb, bs, bs, 8, 8, 8)
-        bb32 = bsr_bytes(n // bs, n // bs, nnzb, bs, bs, 4, 4, 4)
-        out.update({"blocks": nnzb, "block_fp64_us": msb64 * 1e3, "block_fp64_GBps": bb64 / (msb64 * 1e-3) / 1e9,
+        sliced, _, stored = A.block_info(bs, bs)
+        bb64 = bsr_bytes(n // bs, n // bs, nnzb, bs, bs, 8, 8, 8, sliced)
+        bb32 = bsr_bytes(n // bs, n // bs, nnzb, bs, bs, 4, 4, 4, sliced)
+        out.update({"block_layout": "sliced (SELL, 64 block rows per slice)" if sliced else "chunked block CSR",
+                    "block_padding": stored / max(nnzb, 1) - 1.0,
+                    "blocks": nnzb, "block_fp64_us": msb64 * 1e3, "block_fp64_GBps": bb64 / (msb64 * 1e-3) / 1e9,
                     "block_fp32_us": msb32 * 1e3, "block_fp32_GBps": bb32 / (msb32 * 1e-3) / 1e9,
                     "bytes_block_fp64": bb64, "bytes_block_fp32": bb32})
     return out
@@ -264,7 +269,9 @@ def main():
         result["roofline"] = {
             "bound": "hbm", "achieved": spmv["block_fp64_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": spmv["block_fp64_GBps"] / HBM_PEAK_GBS, "traffic": traffic_from_profiles("spmv_block_fp64", key),
-            "kernel": f"k_bsr_stream<double,{bs},{bs},double,EpiOuter> (FP64 {bs}x{bs} cell-block product with A)",
+            "kernel": (f"k_sell_stream<double,{bs},{bs},double,EpiOuter>" if spmv["block_layout"].startswith("sliced")
+                       else f"k_bsr_stream<double,{bs},{bs},double,EpiOuter>")
+                      + f" (FP64 {bs}x{bs} cell-block product with A, {spmv['block_layout']})",
             "bytes_per_launch": spmv["bytes_block_fp64"]}
         result["roofline"].update(copy_GBps=copy_GBps, read_GBps=read_GBps,
                                   frac_of_read=spmv["block_fp64_GBps"] / read_GBps)

@@ -100,6 +100,8 @@ SIGNATURES = {
     "deco_spmv_time": (i, [vp, i, i, dp]),
     "deco_spmv_block": (i, [vp, i, i, vp, vp]),
     "deco_read_bandwidth": (i, [C.c_size_t, i, dp]),
+    "deco_set_sell_padding": (i, [C.c_double]),
+    "deco_block_info": (i, [vp, i, i, ip, ip, C.POINTER(C.c_longlong)]),
     "deco_spmv_block_time": (i, [vp, i, i, i, i, dp, ip]),
     "deco_spgemm": (i, [vp, vp, pp]),
     "deco_spgemm_tril_pattern": (i, [vp, pp]),

@@ -180,6 +180,12 @@ class CSR:
                                          C.byref(nb)))
         return ms.value, nb.value
 
+    def block_info(self, R, C_):
+        """(sliced, blocks, stored blocks) of the R x C_ block shadow (created if needed)."""
+        sl, nb, ent = C.c_int(), C.c_int(), C.c_longlong()
+        check(lib().deco_block_info(self.h, int(R), int(C_), C.byref(sl), C.byref(nb), C.byref(ent)))
+        return bool(sl.value), nb.value, ent.value
+
     def copy(self):
         h = C.c_void_p()
         check(lib().deco_csr_copy(self.h, C.byref(h)))
@@ -547,6 +553,12 @@ class SOL:
 
     def get_result(self):
         return self.x
+
+
+def set_sell_padding(max_fraction):
+    """Block shadows made from now on take the sliced layout when it pads the blocks by at most
+    max_fraction (default 0.10); a negative value selects the chunked block CSR always."""
+    check(lib().deco_set_sell_padding(float(max_fraction)))
 
 
 # ---------------------------------------------------------------------------- profiler ---

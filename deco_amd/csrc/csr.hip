@@ -649,6 +649,45 @@ __global__ void k_bsr_fill(int nb, int R, int C, const int* __restrict__ r, cons
   }
 }
 
+// Largest padding (fraction of the blocks) for which block shadows take the sliced layout;
+// negative: always the chunked one.
+static double g_sell_padding = 0.10;
+
+// per slice of 64 block rows: its width (longest block row) and its entries (width * 64)
+__global__ void k_sell_width(int nb, int nslices, const int* __restrict__ cnt, int* __restrict__ ent,
+                             int* __restrict__ wid) {
+  const int sl = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sl >= nslices) return;
+  int w = 0;
+  const int I1 = min(nb, (sl + 1) * kSellRows);
+  for (int I = sl * kSellRows; I < I1; ++I) w = max(w, cnt[I]);
+  wid[sl] = w;
+  ent[sl] = w * kSellRows;
+}
+
+// sliced layout: block k of block row I at soff[I/64] + 64*k + I%64 (bc and bv zeroed beforehand)
+template <class VT>
+__global__ void k_sell_fill(int nb, int R, int C, const int* __restrict__ r, const int* __restrict__ c,
+                            const double* __restrict__ v, const int* __restrict__ soff, size_t T,
+                            int* __restrict__ bc, VT* __restrict__ bv) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= nb) return;
+  int pos[3], end[3];
+  for (int q = 0; q < R; ++q) { pos[q] = r[I * R + q]; end[q] = r[I * R + q + 1]; }
+  size_t e = (size_t)soff[I / kSellRows] + (I % kSellRows);
+  while (true) {
+    int mn = INT_MAX;
+    for (int q = 0; q < R; ++q)
+      if (pos[q] < end[q]) mn = min(mn, c[pos[q]] / C);
+    if (mn == INT_MAX) break;
+    bc[e] = mn;
+    for (int q = 0; q < R; ++q)
+      for (; pos[q] < end[q] && c[pos[q]] / C == mn; ++pos[q])
+        bv[sell_vidx(C, T, q, c[pos[q]] - mn * C, e)] = (VT)v[pos[q]];
+    e += kSellRows;
+  }
+}
+
 // rb[k] = (i, ptr[i]) for the first row i in [0, n] with ptr[i] >= k*cap; rb[nchunks] = (n, ptr[n])
 __global__ void k_chunks_cap(int n, const int* __restrict__ ptr, int nchunks, int cap, int2* __restrict__ rb) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -683,6 +722,31 @@ const deco_bsr* deco_csr::block32(int R, int C) const {
   }
   const int64_t nnzb = scan_to_ptr(cnt.p, B->br.p, nb);
   B->nnzb = (int)nnzb;
+  if (g_sell_padding >= 0.0 && nb > 0 && m >= C) {   // sliced layout when its padding is small
+    const int ns = (nb + kSellRows - 1) / kSellRows;
+    DBuf<int> ent(ns + 1);
+    B->swid.alloc(ns);
+    hipLaunchKernelGGL(k_sell_width, dim3(grid_for(ns, 256)), dim3(256), 0, stream(), nb, ns, cnt.p, ent.p, B->swid.p);
+    DECO_CHECK_LAUNCH();
+    B->soff.alloc(ns + 1);
+    const int64_t T = scan_to_ptr(ent.p, B->soff.p, ns);
+    if (T <= INT_MAX && (double)T <= (double)nnzb * (1.0 + g_sell_padding)) {
+      B->sell = true;
+      B->nslices = ns;
+      B->stot = (size_t)T;
+      B->bc.alloc(std::max<int64_t>(T, 1));
+      B->bv.alloc(std::max<int64_t>(T, 1) * R * C);
+      DECO_HIP(hipMemsetAsync(B->bc.p, 0, sizeof(int) * std::max<int64_t>(T, 1), stream()));
+      DECO_HIP(hipMemsetAsync(B->bv.p, 0, sizeof(float) * std::max<int64_t>(T, 1) * R * C, stream()));
+      hipLaunchKernelGGL(k_sell_fill<float>, dim3(grid_for(nb, 256)), dim3(256), 0, stream(), nb, R, C, r.p, c.p,
+                         v.p, (const int*)B->soff.p, B->stot, B->bc.p, B->bv.p);
+      DECO_CHECK_LAUNCH();
+      b32 = B;
+      return B;
+    }
+    B->swid.release();
+    B->soff.release();
+  }
   B->bc.alloc(std::max<int64_t>(nnzb, 1));
   B->bv.alloc(std::max<int64_t>(nnzb, 1) * R * C);
   DECO_HIP(hipMemsetAsync(B->bv.p, 0, sizeof(float) * std::max<int64_t>(nnzb, 1) * R * C, stream()));
@@ -731,6 +795,17 @@ const deco_bsr* deco_csr::block64(int R, int C) const {
   const deco_bsr* B0 = block32(R, C);
   if (!B0 || B0->has64) return B0;
   deco_bsr* B = b32;
+  if (B->sell) {
+    const size_t nv = std::max<size_t>(B->stot, 1) * R * C;
+    B->bv64.alloc(nv);
+    DECO_HIP(hipMemsetAsync(B->bv64.p, 0, sizeof(double) * nv, stream()));
+    // rewrites the same block columns, adds the FP64 values
+    hipLaunchKernelGGL(k_sell_fill<double>, dim3(grid_for(B->nb, 256)), dim3(256), 0, stream(), B->nb, R, C, r.p,
+                       c.p, v.p, (const int*)B->soff.p, B->stot, B->bc.p, B->bv64.p);
+    DECO_CHECK_LAUNCH();
+    B->has64 = true;
+    return B;
+  }
   const size_t nv = (size_t)std::max(B->nnzb, 1) * R * C;
   B->bv64.alloc(nv);
   DECO_HIP(hipMemsetAsync(B->bv64.p, 0, sizeof(double) * nv, stream()));
@@ -990,6 +1065,24 @@ int deco_spmv_block_f32(deco_csr* A, int R, int C, const float* x, float* y) {
     if (!B) fail(DECO_ERR_ARG, "%d x %d blocks do not tile a %d x %d matrix", R, C, A->n, A->m);
     bsr_stream_f(B, x, EpiStd<EPI_SET, float, float>{y, nullptr});
     sync();
+  });
+}
+
+int deco_block_info(deco_csr* A, int R, int C, int* sliced, int* nnzb, long long* entries) {
+  return guard([&] {
+    if (!A) fail(DECO_ERR_ARG, "null CSR");
+    const deco_bsr* B = A->block32(R, C);
+    if (!B) fail(DECO_ERR_ARG, "%d x %d blocks do not tile a %d x %d matrix", R, C, A->n, A->m);
+    if (sliced) *sliced = B->sell ? 1 : 0;
+    if (nnzb) *nnzb = B->nnzb;
+    if (entries) *entries = B->sell ? (long long)B->stot : (long long)B->nnzb;
+  });
+}
+
+int deco_set_sell_padding(double max_fraction) {
+  return guard([&] {
+    if (!(max_fraction == max_fraction)) fail(DECO_ERR_ARG, "padding fraction is NaN");
+    g_sell_padding = max_fraction;
   });
 }
 

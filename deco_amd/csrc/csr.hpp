@@ -9,8 +9,11 @@
 // row, C columns per block column): block row I covers rows I*R .. I*R+R-1, block (I, J) holds
 // the R x C values of columns J*C .. J*C+C-1, row-major, zeros where the CSR has no entry. The
 // FP32 values are made with the structure; the FP64 values on first FP64 use.
-// Chunk table: block rows whose first block lies in [k*kBsrCap, (k+1)*kBsrCap), as
-// (first block row, its first block) pairs.
+// Two layouts. Chunked block CSR: chunk table of the block rows whose first block lies in
+// [k*kBsrCap, (k+1)*kBsrCap), as (first block row, its first block) pairs. Sliced (SELL): block
+// rows in slices of 64, one lane per block row; entry k of block row 64*s + l sits at
+// soff[s] + 64*k + l (slices padded to their longest block row with zero blocks of column 0),
+// chosen when that padding stays within the library's threshold (deco_set_sell_padding).
 struct deco_bsr {
   int R = 1, C = 1, nb = 0, nnzb = 0, nchunks = 0;
   deco::DBuf<int> br, bc;
@@ -18,6 +21,10 @@ struct deco_bsr {
   deco::DBuf<float> bv;
   deco::DBuf<double> bv64;
   bool has64 = false;
+  bool sell = false;
+  int nslices = 0;
+  size_t stot = 0;             // entries of the sliced layout, padding included
+  deco::DBuf<int> soff, swid;  // per slice: first entry, width (blocks per block row)
 };
 
 struct deco_csr {
@@ -484,9 +491,115 @@ __global__ __launch_bounds__(BLK) void k_bsr_stream(const int2* __restrict__ des
   }
 }
 
+// ------------------------------------------------------------------ sliced (SELL) products --
+// One lane per block row, 64 block rows per slice (a wavefront), 4 slices per workgroup. Step k
+// of a wavefront loads entry k of 64 consecutive block rows: contiguous columns and values, and
+// for stencil-like patterns x gathers that fall into a few cache lines, with no LDS staging and
+// no barrier. Each row sums its blocks in column order, the C products of a block paired first
+// (the chunked kernel's order). Values: C == 2, one plane per block row holding the pair
+// (one 8- or 16-byte load); otherwise one plane per block element.
+constexpr int kSellRows = 64;
+
+__host__ __device__ inline size_t sell_vidx(int C, size_t T, int rr, int q, size_t e) {
+  return C == 2 ? (size_t)rr * 2 * T + 2 * e + q : (size_t)(rr * C + q) * T + e;
+}
+
+template <class VT, int R, int C>
+__device__ __forceinline__ BsrVals<VT, R * C> sell_load(const VT* __restrict__ bv, size_t e, size_t T) {
+  BsrVals<VT, R * C> o;
+  if constexpr (C == 2) {
+    using V2 = typename std::conditional<sizeof(VT) == 8, f64x2, f32x2>::type;
+#pragma unroll
+    for (int rr = 0; rr < R; ++rr) {
+      const V2 t = __builtin_nontemporal_load(reinterpret_cast<const V2*>(bv + (size_t)rr * 2 * T) + e);
+      o.a[rr * 2] = t.x; o.a[rr * 2 + 1] = t.y;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < R * C; ++q) o.a[q] = __builtin_nontemporal_load(bv + (size_t)q * T + e);
+  }
+  return o;
+}
+
+template <class VT, int R, int C, class XT, class E, int KU = 4>
+__global__ __launch_bounds__(256) void k_sell_stream(int nb, int nslices, const int* __restrict__ soff,
+                                                    const int* __restrict__ swid, const int* __restrict__ bc,
+                                                    const VT* __restrict__ bv, size_t T, const XT* __restrict__ x,
+                                                    E epi) {
+  using AT = acc_t<VT, XT>;
+  const int sl = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+  if (sl >= nslices) return;
+  const int lane = threadIdx.x & 63;
+  const int I = sl * kSellRows + lane;
+  const bool live = I < nb;
+  typename E::Pre pre[R];
+  if (live) {
+#pragma unroll
+    for (int rr = 0; rr < R; ++rr) pre[rr] = epi.pre(I * R + rr);
+  }
+  const int o = soff[sl], w = swid[sl];
+  AT acc[R];
+#pragma unroll
+  for (int rr = 0; rr < R; ++rr) acc[rr] = AT(0);
+  int k = 0;
+  for (; k + KU <= w; k += KU) {
+    int cc[KU];
+    BsrVals<VT, R * C> vv[KU];
+    XT xv[KU][C];
+#pragma unroll
+    for (int u = 0; u < KU; ++u) {
+      const size_t e = (size_t)o + (size_t)(k + u) * kSellRows + lane;
+      cc[u] = __builtin_nontemporal_load(bc + e);
+      vv[u] = sell_load<VT, R, C>(bv, e, T);
+    }
+#pragma unroll
+    for (int u = 0; u < KU; ++u) bsr_gather<C>(x, cc[u], xv[u]);
+#pragma unroll
+    for (int u = 0; u < KU; ++u)
+#pragma unroll
+      for (int rr = 0; rr < R; ++rr) {
+        AT p = (AT)vv[u].a[rr * C] * (AT)xv[u][0];
+#pragma unroll
+        for (int q = 1; q < C; ++q) p += (AT)vv[u].a[rr * C + q] * (AT)xv[u][q];
+        acc[rr] += p;
+      }
+  }
+  for (; k < w; ++k) {
+    const size_t e = (size_t)o + (size_t)k * kSellRows + lane;
+    const BsrVals<VT, R * C> vv = sell_load<VT, R, C>(bv, e, T);
+    XT xv[C];
+    bsr_gather<C>(x, bc[e], xv);
+#pragma unroll
+    for (int rr = 0; rr < R; ++rr) {
+      AT p = (AT)vv.a[rr * C] * (AT)xv[0];
+#pragma unroll
+      for (int q = 1; q < C; ++q) p += (AT)vv.a[rr * C + q] * (AT)xv[q];
+      acc[rr] += p;
+    }
+  }
+  if (live) {
+#pragma unroll
+    for (int rr = 0; rr < R; ++rr) epi(I * R + rr, acc[rr], pre[rr]);
+  }
+}
+
 template <class VT, class XT, class E>
 void bsr_stream_v(const deco_bsr* B, const VT* bv, const XT* x, E epi) {
   if (B->nb <= 0) return;
+  if (B->sell) {
+    const unsigned grid = (unsigned)((B->nslices + 3) / 4);
+#define DECO_SELL_LAUNCH(RR, CC)                                                                                 \
+  hipLaunchKernelGGL((k_sell_stream<VT, RR, CC, XT, E>), dim3(grid), dim3(256), 0, stream(), B->nb, B->nslices, \
+                     B->soff.p, B->swid.p, B->bc.p, bv, B->stot, x, epi)
+    if (B->R == 2 && B->C == 2) DECO_SELL_LAUNCH(2, 2);
+    else if (B->R == 1 && B->C == 2) DECO_SELL_LAUNCH(1, 2);
+    else if (B->R == 3 && B->C == 3) DECO_SELL_LAUNCH(3, 3);
+    else if (B->R == 1 && B->C == 3) DECO_SELL_LAUNCH(1, 3);
+    else fail(DECO_ERR_UNSUPPORTED, "block shape %d x %d", B->R, B->C);
+#undef DECO_SELL_LAUNCH
+    DECO_CHECK_LAUNCH();
+    return;
+  }
 #define DECO_BSR_LAUNCH(RR, CC)                                                                                  \
   hipLaunchKernelGGL((k_bsr_stream<VT, RR, CC, XT, E>), dim3(B->nchunks), dim3(kBsrBlock), 0, stream(), B->rb.p, \
                      B->br.p, B->bc.p, bv, (size_t)B->nnzb * CC, x, epi)

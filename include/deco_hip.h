@@ -197,6 +197,14 @@ int deco_spmv_block(deco_csr* A, int R, int C, const double* x_dev, double* y_de
  * back-to-back launches timed with hipEvents (reads well past the 256 MB cache measure HBM). The reference rate for the products' read-dominated traffic; no reference
  * counterpart. */
 int deco_read_bandwidth(size_t bytes, int iters, double* gbps);
+/* Layout of the block shadows made from now on: the sliced one (64 block rows per slice, one
+ * lane per block row) when it pads the blocks by at most max_fraction (default 0.10), the
+ * chunked block CSR otherwise; a negative value selects the chunked layout always. No reference
+ * counterpart (a layout choice of this library). */
+int deco_set_sell_padding(double max_fraction);
+/* The R x C block shadow of A (created if needed): sliced (1) or chunked (0) layout, its number of
+ * blocks, and the blocks it stores (padding included). No reference counterpart. */
+int deco_block_info(deco_csr* A, int R, int C, int* sliced, int* nnzb, long long* entries);
 /* deco_spmv_time for the block-shadow product (precision as there); nnzb (may be NULL)
  * receives the number of R x C blocks. */
 int deco_spmv_block_time(deco_csr* A, int R, int C, int precision, int iters, double* avg_ms, int* nnzb);

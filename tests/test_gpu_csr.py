@@ -123,11 +123,21 @@ def test_block_reorder_bitexact(gpu, cfg, bs, ro, co):
         A.block_reorder(bs, "1" * bs, co)
 
 
+@pytest.mark.parametrize("pad", [10.0, -1.0])
 @pytest.mark.parametrize("cfg,shape", [(1, (2, 2)), (2, (3, 3)), (1, "prow"), (1, "tril"), (2, "tril")])
-def test_block_f32_product(gpu, cfg, shape):
-    """FP32 block-shadow products against the FP64 CSR product: full cell blocks, the pressure rows
+def test_block_f32_product(gpu, cfg, shape, pad):
+    """Block-shadow products against the CSR product: full cell blocks, the pressure rows
     (1 x bs blocks), and lower-triangular patterns whose diagonal blocks are partly empty (the
-    FSAI factor shape)."""
+    FSAI factor shape); in the sliced layout (pad 10: chosen whatever its padding) and the
+    chunked one (pad -1)."""
+    deco_amd.set_sell_padding(pad)
+    try:
+        _block_products(cfg, shape)
+    finally:
+        deco_amd.set_sell_padding(0.10)
+
+
+def _block_products(cfg, shape):
     g = gen_system(cfg)
     bs = g["bs"]
     S = to_mat(g).to_scipy()
