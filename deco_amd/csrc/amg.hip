@@ -385,8 +385,7 @@ struct AMG : deco_prec {
   template <class T>
   void relax_sweep(Level& lv, const deco_csr* A, const T* x, const T* b, T* y, int subset, T* add_to = nullptr,
                    int stride = 0) {
-    spmv_stream_f(A, vals<T>(const_cast<deco_csr*>(A)), x,
-                  EpiRelax<T>{x, b, D<T>(lv), lv.cf.p, subset, y, add_to, stride});
+    spmv_auto<T>(A, x, EpiRelax<T>{x, b, D<T>(lv), lv.cf.p, subset, y, add_to, stride});
   }
 
   // one V-cycle on level l with right-hand side b, result in x (x != b); x_ready: x already
@@ -417,22 +416,22 @@ struct AMG : deco_prec {
     }
     // residual and restriction (the restriction also does the next level's zero-guess sweep
     // for the Jacobi-type smoothers)
-    spmv_stream<EPI_RES, T, T, T, T>(A, vals<T>(const_cast<deco_csr*>(A)), x, r, b);
+    spmv_auto<T>(A, x, EpiStd<EPI_RES, T, T>{r, b});
     const bool last = (l + 1 == L.size());
     T* bn = last ? BC<T>() : B<T>(L[l + 1]);
     T* xn = last ? XC<T>() : X<T>(L[l + 1]);
     const bool fuse = !last && relax != 17;
     if (fuse)
-      spmv_stream_f(lv.R, vals<T>(lv.R), (const T*)r, EpiRestrictJacobi0<T>{bn, D<T>(L[l + 1]), xn});
+      spmv_auto<T>(lv.R, (const T*)r, EpiRestrictJacobi0<T>{bn, D<T>(L[l + 1]), xn});
     else
-      spmv_stream<EPI_SET, T, T, T, T>(lv.R, vals<T>(lv.R), r, bn, (const T*)nullptr);
+      spmv_auto<T>(lv.R, (const T*)r, EpiStd<EPI_SET, T, T>{bn, nullptr});
     cycle<T>(l + 1, bn, xn, tr, fuse);
     if (deco_csr* S = tr ? lv.St : lv.S) {   // x <- x + r/d + S xn: prolongation and sweep in one product
-      spmv_stream_f(S, vals<T>(S), (const T*)xn, EpiUp<T>{x, (const T*)r, D<T>(lv), add_to, stride});
+      spmv_auto<T>(S, (const T*)xn, EpiUp<T>{x, (const T*)r, D<T>(lv), add_to, stride});
       return;
     }
     // prolongation t = x + P xn, then the up smoother back into x
-    spmv_stream<EPI_ADDZ, T, T, T, T>(lv.P, vals<T>(lv.P), xn, t, x);
+    spmv_auto<T>(lv.P, (const T*)xn, EpiStd<EPI_ADDZ, T, T>{t, x});
     if (relax == 17) {
       relax_sweep<T>(lv, A, t, b, x, 0);
       relax_sweep<T>(lv, A, x, b, t, 1);

@@ -599,6 +599,7 @@ deco_csr::~deco_csr() {
 void deco_csr::invalidate() {
   vf_valid = false;
   rb_valid = false;
+  sell_ok = -1;
   delete b32;
   b32 = nullptr;
 }
@@ -789,6 +790,25 @@ __global__ __launch_bounds__(256) void k_read_stream(const f64x2* __restrict__ p
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 }  // namespace deco
+
+bool deco_csr::sell_scalar() const {
+  using namespace deco;
+  if (sell_ok < 0) {
+    sell_ok = 0;
+    if (g_sell_padding >= 0.0 && n > 0 && m > 0 && nnz > 0) {
+      const int ns = (n + kSellRows - 1) / kSellRows;
+      DBuf<int> len(n), ent(ns + 1), wid(ns), off(ns + 1);
+      hipLaunchKernelGGL(k_row_len, dim3(grid_for(n, 256)), dim3(256), 0, stream(), n, r.p, len.p);
+      DECO_CHECK_LAUNCH();
+      hipLaunchKernelGGL(k_sell_width, dim3(grid_for(ns, 256)), dim3(256), 0, stream(), n, ns, (const int*)len.p, ent.p,
+                         wid.p);
+      DECO_CHECK_LAUNCH();
+      const int64_t T = scan_to_ptr(ent.p, off.p, ns);
+      sell_ok = (T <= INT_MAX && (double)T <= (double)nnz * (1.0 + g_sell_padding)) ? 1 : 0;
+    }
+  }
+  return sell_ok == 1;
+}
 
 const deco_bsr* deco_csr::block64(int R, int C) const {
   using namespace deco;

@@ -51,6 +51,9 @@ struct deco_csr {
   // unknowns per cell of the system this matrix belongs to (set by the CPR setup): above 1 the
   // solvers' FP64 product with the matrix runs on the cell x cell block shadow
   int cell = 1;
+  // whether the rows fit the sliced layout within the padding threshold (decided on first call)
+  bool sell_scalar() const;
+  mutable signed char sell_ok = -1;
   void invalidate();
 };
 
@@ -592,6 +595,7 @@ void bsr_stream_v(const deco_bsr* B, const VT* bv, const XT* x, E epi) {
   hipLaunchKernelGGL((k_sell_stream<VT, RR, CC, XT, E>), dim3(grid), dim3(256), 0, stream(), B->nb, B->nslices, \
                      B->soff.p, B->swid.p, B->bc.p, bv, B->stot, x, epi)
     if (B->R == 2 && B->C == 2) DECO_SELL_LAUNCH(2, 2);
+    else if (B->R == 1 && B->C == 1) DECO_SELL_LAUNCH(1, 1);
     else if (B->R == 1 && B->C == 2) DECO_SELL_LAUNCH(1, 2);
     else if (B->R == 3 && B->C == 3) DECO_SELL_LAUNCH(3, 3);
     else if (B->R == 1 && B->C == 3) DECO_SELL_LAUNCH(1, 3);
@@ -604,12 +608,29 @@ void bsr_stream_v(const deco_bsr* B, const VT* bv, const XT* x, E epi) {
   hipLaunchKernelGGL((k_bsr_stream<VT, RR, CC, XT, E>), dim3(B->nchunks), dim3(kBsrBlock), 0, stream(), B->rb.p, \
                      B->br.p, B->bc.p, bv, (size_t)B->nnzb * CC, x, epi)
   if (B->R == 2 && B->C == 2) DECO_BSR_LAUNCH(2, 2);
+  else if (B->R == 1 && B->C == 1) DECO_BSR_LAUNCH(1, 1);
   else if (B->R == 1 && B->C == 2) DECO_BSR_LAUNCH(1, 2);
   else if (B->R == 3 && B->C == 3) DECO_BSR_LAUNCH(3, 3);
   else if (B->R == 1 && B->C == 3) DECO_BSR_LAUNCH(1, 3);
   else fail(DECO_ERR_UNSUPPORTED, "block shape %d x %d", B->R, B->C);
 #undef DECO_BSR_LAUNCH
   DECO_CHECK_LAUNCH();
+}
+
+// Scalar product y = A x (epilogue e) with VT values: through A's sliced 1 x 1 shadow when its
+// padding is within the threshold (built on first use), else the CSR-stream kernel. Each row
+// sums in column order in the sliced layout.
+template <class VT, class XT, class E>
+void spmv_auto(const deco_csr* A, const XT* x, E epi) {
+  const deco_bsr* B = !A->sell_scalar() ? nullptr
+                      : std::is_same<VT, double>::value ? A->block64(1, 1) : A->block32(1, 1);
+  if (B && B->sell) {
+    if constexpr (std::is_same<VT, double>::value) bsr_stream_v<double>(B, B->bv64.p, x, epi);
+    else bsr_stream_v<float>(B, B->bv.p, x, epi);
+    return;
+  }
+  if constexpr (std::is_same<VT, double>::value) spmv_stream_f(A, A->v.p, x, epi);
+  else spmv_stream_f(A, const_cast<deco_csr*>(A)->values_f32(), x, epi);
 }
 
 // product with the FP32 block values
