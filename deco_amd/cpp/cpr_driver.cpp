@@ -12,11 +12,7 @@
 using Clock = std::chrono::steady_clock;
 static double secs(Clock::time_point a, Clock::time_point b) { return std::chrono::duration<double>(b - a).count(); }
 
-int main(int argc, char* argv[]) {
-  if (argc < 2) {
-    std::fprintf(stderr, "Usage: %s param.json [synthetic:<cfg>]\n", argv[0]);
-    return 1;
-  }
+static int run(int argc, char* argv[]) {
   deco_facade::check(deco_init(0), "deco_init");
   deco_ptree param = deco::read_ptree(argv[1]);
   std::shared_ptr<CSR<int, int, double>> A;
@@ -77,4 +73,19 @@ int main(int argc, char* argv[]) {
   std::printf("iteration count =  %d\nresidual = %.2e\nflag = %d\n", iter, res, flag);
   profiler.report();
   return flag;
+}
+
+// Errors the reference's driver would end on (unreadable files, bad parameters, numerical
+// failures reported by the library) print their message and return 2 here.
+int main(int argc, char* argv[]) {
+  if (argc < 2) {
+    std::fprintf(stderr, "Usage: %s param.json [synthetic:<cfg>]\n", argv[0]);
+    return 1;
+  }
+  try {
+    return run(argc, argv);
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "cpr_driver: %s\n", e.what());
+    return 2;
+  }
 }

@@ -43,3 +43,91 @@ def test_cpp_driver_solves_synthetic_system(gpu):
     assert "iteration count" in res.stdout
     res_line = [l for l in res.stdout.splitlines() if l.startswith("residual")][-1]
     assert float(res_line.split("=")[1]) < 1e-8
+
+
+def _write_system(tmp_path, g):
+    """The system in the reference's binary layout (IO/cpp_read.hpp:331-442): int32 n, m, nz;
+    int32 r[n+1]; int32 c[nz]; f64 v[nz]; rhs int32 n; f64 b[n]."""
+    import numpy as np
+    n, nnz = g["n"], g["nnz"]
+    with open(tmp_path / "A.bin", "wb") as f:
+        np.array([n, n, nnz], np.int32).tofile(f)
+        np.asarray(g["r"], np.int32).tofile(f)
+        np.asarray(g["c"], np.int32).tofile(f)
+        np.asarray(g["v"], np.float64).tofile(f)
+    with open(tmp_path / "b.bin", "wb") as f:
+        np.array([n], np.int32).tofile(f)
+        np.asarray(g["b"], np.float64).tofile(f)
+
+
+def _param_with_files(tmp_path, file_a, file_b):
+    import json
+    from deco_amd import remove_json_comments
+    p = json.loads(remove_json_comments(open(PARAM).read()))
+    p["IO"] = {"format": "bin", "file_A": str(file_a), "file_b": str(file_b)}
+    out = tmp_path / "param.json"
+    out.write_text(json.dumps(p))
+    return out
+
+
+@pytest.mark.gpu
+def test_cpp_driver_reads_binary_files(gpu, tmp_path):
+    """The driver on files in the reference's binary format: the generator's system written with
+    numpy converges as it does from memory; rhs "ones" builds b = A 1; a truncated matrix file and
+    an rhs of the wrong length are errors."""
+    import deco_amd
+    exe = os.path.join(CPP, "cpr_driver")
+    assert os.path.exists(exe), "build the driver with __graft_entry__.build()"
+    g = deco_amd.generate(1)
+    _write_system(tmp_path, g)
+    for rhs in (tmp_path / "b.bin", "ones"):
+        res = subprocess.run([exe, str(_param_with_files(tmp_path, tmp_path / "A.bin", rhs))],
+                             capture_output=True, text=True, timeout=120)
+        assert res.returncode == 0, res.stdout + res.stderr
+        res_line = [l for l in res.stdout.splitlines() if l.startswith("residual")][-1]
+        assert float(res_line.split("=")[1]) < 1e-8
+    data = (tmp_path / "A.bin").read_bytes()
+    (tmp_path / "T.bin").write_bytes(data[: len(data) // 2])
+    res = subprocess.run([exe, str(_param_with_files(tmp_path, tmp_path / "T.bin", "ones"))],
+                         capture_output=True, text=True, timeout=120)
+    assert res.returncode != 0 and "truncated" in (res.stdout + res.stderr)
+    import numpy as np
+    with open(tmp_path / "s.bin", "wb") as f:
+        np.array([7], np.int32).tofile(f)
+        np.zeros(7).tofile(f)
+    res = subprocess.run([exe, str(_param_with_files(tmp_path, tmp_path / "A.bin", tmp_path / "s.bin"))],
+                         capture_output=True, text=True, timeout=120)
+    assert res.returncode != 0 and "does not match" in (res.stdout + res.stderr)
+
+
+@pytest.mark.gpu
+def test_binary_writer_reader_round_trip(gpu, tmp_path):
+    """IO::cpp_write_binary_matrix / cpp_write_binary_array then the readers: bit-identical."""
+    src = tmp_path / "rt.cpp"
+    src.write_text(r'''
+#include <cstdio>
+#include <cstring>
+#include "io.hpp"
+int main(int argc, char** argv) {
+  auto A = std::make_shared<CSR<int, int, double>>(3, 4, 5);
+  const int r[] = {0, 2, 3, 5}, c[] = {0, 3, 1, 0, 2};
+  const double v[] = {1.5, -2.25, 3e-300, 1e300, -0.0};
+  std::memcpy(A->r, r, sizeof r); std::memcpy(A->c, c, sizeof c); std::memcpy(A->v, v, sizeof v);
+  const std::string fa = std::string(argv[1]) + "/m.bin", fb = std::string(argv[1]) + "/v.bin";
+  IO::cpp_write_binary_matrix(fa, A);
+  const double b[] = {0.1, -7.0, 2.5};
+  IO::cpp_write_binary_array(fb, 3, b);
+  auto B = IO::cpp_read_binary_matrix<int, int, double>(fa);
+  double* bb = IO::cpp_read_binary_rhs<int, int, double>(fb, B);
+  int ok = B->n == 3 && B->m == 4 && B->nz == 5 && !std::memcmp(B->r, r, sizeof r) &&
+           !std::memcmp(B->c, c, sizeof c) && !std::memcmp(B->v, v, sizeof v) && !std::memcmp(bb, b, sizeof b);
+  std::free(bb);
+  std::printf("%s\n", ok ? "round-trip-ok" : "MISMATCH");
+  return ok ? 0 : 1;
+}''')
+    exe = tmp_path / "rt"
+    lib_dir = os.path.join(ROOT, "deco_amd")
+    subprocess.run(["g++", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-I", CPP, str(src), "-o", str(exe),
+                    "-L", lib_dir, "-ldeco_hip", "-Wl,-rpath," + lib_dir], check=True)
+    res = subprocess.run([str(exe), str(tmp_path)], capture_output=True, text=True, timeout=60)
+    assert res.returncode == 0 and "round-trip-ok" in res.stdout, res.stdout + res.stderr
