@@ -202,16 +202,25 @@ HostCSR interp_extpi(const HostCSR& A, const std::vector<char>& strong, const st
   nc = 0;
   for (int i = 0; i < n; ++i)
     if (cf[i] == 1) cidx[i] = nc++;
-  std::vector<std::vector<std::pair<int, double>>> rows(n);
+  // Rows are appended to per-thread buffers (no allocation per row); row i lives in the buffer
+  // of thread owner[i] at rstart[i], rcnt[i] entries, and is copied into P afterwards.
+  const int nthreads = omp_get_max_threads();
+  std::vector<std::vector<std::pair<int, double>>> tbuf(nthreads);
+  std::vector<int> owner(n, 0), rcnt(n, 0);
+  std::vector<size_t> rstart(n, 0);
 #pragma omp parallel
   {
+    auto& buf = tbuf[omp_get_thread_num()];
+    const int me = omp_get_thread_num();
     std::vector<int> pos(n, -1);
     std::vector<int> list;
     std::vector<double> coef;
+    std::vector<std::pair<int, double>> ent, kept;
 #pragma omp for schedule(dynamic, 512)
     for (int i = 0; i < n; ++i) {
-      auto& out = rows[i];
-      if (cf[i] == 1) { out.push_back({cidx[i], 1.0}); continue; }
+      owner[i] = me;
+      rstart[i] = buf.size();
+      if (cf[i] == 1) { buf.push_back({cidx[i], 1.0}); rcnt[i] = 1; continue; }
       list.clear();
       coef.clear();
       auto add = [&](int j) {
@@ -263,14 +272,14 @@ HostCSR interp_extpi(const HostCSR& A, const std::vector<char>& strong, const st
       for (int j : list) pos[j] = -1;
       if (list.empty()) continue;
       for (size_t q = 0; q < coef.size(); ++q) coef[q] = (diagonal != 0.0) ? -coef[q] / diagonal : 0.0;
-      std::vector<std::pair<int, double>> ent(list.size());
-      for (size_t q = 0; q < list.size(); ++q) ent[q] = {list[q], coef[q]};
+      ent.clear();
+      for (size_t q = 0; q < list.size(); ++q) ent.push_back({list[q], coef[q]});
       // truncation relative to the row maximum, keeping the row sum
       if (trunc > 0.0) {
         double mx = 0.0, rs = 0.0;
         for (auto& p : ent) { mx = std::max(mx, std::fabs(p.second)); rs += p.second; }
         const double thr = trunc * mx;
-        std::vector<std::pair<int, double>> kept;
+        kept.clear();
         double ks = 0.0;
         for (auto& p : ent)
           if (std::fabs(p.second) >= thr) { kept.push_back(p); ks += p.second; }
@@ -296,22 +305,27 @@ HostCSR interp_extpi(const HostCSR& A, const std::vector<char>& strong, const st
           for (auto& p : ent) p.second *= sc;
         }
       }
-      for (auto& p : ent) out.push_back({cidx[p.first], p.second});
-      std::sort(out.begin(), out.end());
+      const size_t s0 = buf.size();
+      for (auto& p : ent) buf.push_back({cidx[p.first], p.second});
+      std::sort(buf.begin() + s0, buf.end());
+      rcnt[i] = (int)(buf.size() - s0);
     }
   }
   HostCSR P;
   P.n = n;
   P.m = nc;
   P.r.assign(n + 1, 0);
-  for (int i = 0; i < n; ++i) P.r[i + 1] = P.r[i] + (int)rows[i].size();
+  for (int i = 0; i < n; ++i) P.r[i + 1] = P.r[i] + rcnt[i];
   P.c.resize(P.r[n]);
   P.v.resize(P.r[n]);
-  for (int i = 0; i < n; ++i)
-    for (size_t q = 0; q < rows[i].size(); ++q) {
-      P.c[P.r[i] + q] = rows[i][q].first;
-      P.v[P.r[i] + q] = rows[i][q].second;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; ++i) {
+    const auto* src = tbuf[owner[i]].data() + rstart[i];
+    for (int q = 0; q < rcnt[i]; ++q) {
+      P.c[P.r[i] + q] = src[q].first;
+      P.v[P.r[i] + q] = src[q].second;
     }
+  }
   return P;
 }
 

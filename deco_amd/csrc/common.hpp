@@ -57,6 +57,9 @@ struct Context {
   size_t lds_per_block = 0;
   // pinned host staging for scalar readbacks
   double* pinned = nullptr;
+  // library buffers come from the device's stream-ordered pool (allocation and release queued
+  // on the library stream: no device-wide synchronisation, freed blocks reused)
+  bool pooled = false;
 };
 Context& ctx();
 void ensure_init();
@@ -98,12 +101,18 @@ struct DBuf {
   void alloc(size_t count) {
     release();
     ensure_init();
-    if (count) DECO_HIP(hipMalloc(&p, count * sizeof(T)));
+    if (count) {
+      if (ctx().pooled) DECO_HIP(hipMallocAsync(reinterpret_cast<void**>(&p), count * sizeof(T), ctx().stream));
+      else DECO_HIP(hipMalloc(&p, count * sizeof(T)));
+    }
     n = count;
   }
   void ensure(size_t count) { if (count > n) alloc(count); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      if (ctx().pooled && ctx().stream) (void)hipFreeAsync(p, ctx().stream);
+      else (void)hipFree(p);
+    }
     p = nullptr;
     n = 0;
   }

@@ -39,6 +39,13 @@ static void init_device(int device) {
   if (g_ctx.stream) { (void)hipStreamDestroy(g_ctx.stream); g_ctx.stream = nullptr; }
   // default-priority, non-blocking stream
   DECO_HIP(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
+  // stream-ordered pool for the library's buffers, keeping freed memory for reuse
+  hipMemPool_t pool = nullptr;
+  if (!g_ctx.pooled && hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) {
+    uint64_t keep = UINT64_MAX;
+    g_ctx.pooled = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) == hipSuccess;
+  }
+  (void)hipGetLastError();
   if (!g_ctx.pinned) DECO_HIP(hipHostMalloc(&g_ctx.pinned, 4096 * sizeof(double), hipHostMallocDefault));
   g_ctx.device = device;
   g_ctx.num_cu = prop.multiProcessorCount;
