@@ -258,12 +258,15 @@ struct AMG : deco_prec {
     const double theta = p.amg_SoC_alpha;
     const double mrs = p.amg_max_row_sum > 0 ? p.amg_max_row_sum : 0.9;
     deco_csr* Al = csr_clone(A0);
+    amg::HostCSR H;   // host copy of the current level, its arrays reused level to level
     try {
       for (int lev = 0;; ++lev) {
         if (Al->n <= max_coarse || lev >= max_levels - 1) break;
-        amg::HostCSR H;
         H.n = Al->n;
         H.m = Al->m;
+        H.r.clear();
+        H.c.clear();
+        H.v.clear();
         H.r.resize(Al->n + 1);
         H.c.resize(Al->nnz);
         H.v.resize(Al->nnz);
@@ -338,9 +341,11 @@ struct AMG : deco_prec {
     // coarsest: dense FP64 inverse
     ProfPhase ph_ci("coarse inverse");
     nc = Ac->n;
-    amg::HostCSR H;
     H.n = nc;
     H.m = nc;
+    H.r.clear();
+    H.c.clear();
+    H.v.clear();
     H.r.resize(nc + 1);
     H.c.resize(Ac->nnz);
     H.v.resize(Ac->nnz);

@@ -47,8 +47,8 @@ void strength(const HostCSR& A, double theta, double max_row_sum, std::vector<ch
 
 // strong dependencies S (rows) and strong influences S^T as CSR adjacency; the S^T lists are
 // filled in parallel and then sorted, which gives them the ascending order of a serial fill
-static void adjacency(const HostCSR& A, const std::vector<char>& strong, std::vector<int>& sr, std::vector<int>& sc,
-                      std::vector<int>& tr, std::vector<int>& tc) {
+static void adjacency(const HostCSR& A, const std::vector<char>& strong, hvec<int>& sr, hvec<int>& sc, hvec<int>& tr,
+                      hvec<int>& tc) {
   const int n = A.n;
   sr.assign(n + 1, 0);
   tr.assign(n + 1, 0);
@@ -86,8 +86,8 @@ static void adjacency(const HostCSR& A, const std::vector<char>& strong, std::ve
 // Ruge-Stueben first pass with bucket lists (max measure first, LIFO inside a bucket). The
 // measure, the list links and the C/F state of a point share one 16-byte record: the pass is
 // bound by random accesses to the neighbours of each new C point, one cache line each.
-static void coarsen_rs(int n, const std::vector<int>& sr, const std::vector<int>& sc, const std::vector<int>& tr,
-                       const std::vector<int>& tc, std::vector<int>& cf_out) {
+static void coarsen_rs(int n, const hvec<int>& sr, const hvec<int>& sc, const hvec<int>& tr, const hvec<int>& tc,
+                       std::vector<int>& cf_out) {
   struct Pt {
     int lam, nxt, prv, cf;
   };
@@ -150,8 +150,8 @@ static void coarsen_rs(int n, const std::vector<int>& sr, const std::vector<int>
 }
 
 // PMIS: measure |S^T_i| + U[0,1) from a seeded hash; independent sets of local maxima.
-static void coarsen_pmis(int n, const std::vector<int>& sr, const std::vector<int>& sc, const std::vector<int>& tr,
-                         const std::vector<int>& tc, uint64_t seed, std::vector<int>& cf) {
+static void coarsen_pmis(int n, const hvec<int>& sr, const hvec<int>& sc, const hvec<int>& tr, const hvec<int>& tc,
+                         uint64_t seed, std::vector<int>& cf) {
   cf.assign(n, 0);
   std::vector<double> meas(n);
   for (int i = 0; i < n; ++i) {
@@ -189,7 +189,7 @@ static void coarsen_pmis(int n, const std::vector<int>& sr, const std::vector<in
 }
 
 void coarsen(const HostCSR& A, const std::vector<char>& strong, int coarse_type, uint64_t seed, std::vector<int>& cf) {
-  std::vector<int> sr, sc, tr, tc;
+  hvec<int> sr, sc, tr, tc;
   adjacency(A, strong, sr, sc, tr, tc);
   if (coarse_type == 8) coarsen_pmis(A.n, sr, sc, tr, tc, seed, cf);
   else coarsen_rs(A.n, sr, sc, tr, tc, cf);

@@ -3,15 +3,33 @@
 // the device (amg.hip); this part sees one level at a time as host CSR arrays.
 #pragma once
 #include <cstdint>
+#include <memory>
+#include <utility>
 #include <vector>
 
 namespace deco {
 namespace amg {
 
+// std::allocator whose value-less construct leaves the element uninitialised: the host level
+// arrays are always overwritten after a resize (by a download or a fill), so resize() need not
+// zero hundreds of MB first.
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+  using std::allocator<T>::allocator;
+  template <class U>
+  struct rebind { using other = NoInitAlloc<U>; };
+  template <class U>
+  void construct(U* p) noexcept { ::new (static_cast<void*>(p)) U; }
+  template <class U, class... Args>
+  void construct(U* p, Args&&... args) { ::new (static_cast<void*>(p)) U(std::forward<Args>(args)...); }
+};
+template <class T>
+using hvec = std::vector<T, NoInitAlloc<T>>;
+
 struct HostCSR {
   int n = 0, m = 0;
-  std::vector<int> r, c;
-  std::vector<double> v;
+  hvec<int> r, c;
+  hvec<double> v;
 };
 
 // Classical strength of connection with hypre's conventions (the sign of the diagonal decides
