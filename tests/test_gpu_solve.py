@@ -167,3 +167,26 @@ def test_fp32_transposed_cpr_and_adjoint_solve(gpu):
     assert rel(y32, y64) <= 2e-4
     assert S32.flag == 0 and S64.flag == 0 and abs(S32.iter - S64.iter) <= 2
     assert rel(S32.x.numpy(), S64.x.numpy()) <= 1e-6
+
+
+@pytest.mark.parametrize("kind", ["fsai", "jacobi"])
+def test_standalone_fp32_preconditioners_in_gmres(gpu, kind):
+    """FSAI and block Jacobi alone with FP32 precision: the right-preconditioned GMRES runs in
+    flexible form and reaches the same tolerance in the same number of iterations (+-2) with the
+    FP64 solution to 1e-6."""
+    g = gen_system(1)
+    A = dev_csr(g)
+    b = deco_amd.DeviceArray.from_numpy(g["b"])
+    if kind == "fsai":
+        M = deco_amd.FSAI(small_params(2), A)
+    else:
+        M = deco_amd.Jacobi({"block_size": "2"}, A)
+    sol = {"type": "rgmres", "reltol": "1e-9", "restart": "60", "itmax": "2000", "verbosity": "0"}
+    S64 = deco_amd.SOL(sol, A, M, b)
+    M.set_precision(deco_amd.PREC_FP32)
+    S32 = deco_amd.SOL(sol, A, M, b)
+    M.set_precision(deco_amd.PREC_FP64)
+    assert S64.flag == 0 and S32.flag == 0
+    assert S32.res_rel <= 1e-9
+    assert abs(S32.iter - S64.iter) <= 2
+    assert rel(S32.x.numpy(), S64.x.numpy()) <= 1e-6
