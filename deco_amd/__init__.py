@@ -11,4 +11,4 @@ from .core import (  # noqa: F401
     profile_enable, profile_reset, profile_start, profile_stop, profile_time, profile_report,
     set_sell_padding,
 )
-from ._lib import PREC_FP64, PREC_FP32, check  # noqa: F401
+from ._lib import PREC_FP64, PREC_FP32, LAYOUT_BLOCK, LAYOUT_CSR, check  # noqa: F401

@@ -21,6 +21,7 @@ ERRORS = {
 W_NONE, W_QUASIIMPES, W_QUASIIMPES_GE = 0, 1, 2
 GLO_FSAI, GLO_JACOBI = 0, 1
 PREC_FP64, PREC_FP32 = 0, 1
+LAYOUT_BLOCK, LAYOUT_CSR = 0, 1
 SOL_TYPES = {"rgmres": 0, "lgmres": 1, "bicgstab": 2, "lbicgstab": 3, "pcg": 4}
 ORTHO = {"dcgs2": 0, "mgs": 1}
 
@@ -43,6 +44,7 @@ class CprParams(C.Structure):
         ("amg_prol_trunc_fac", C.c_double), ("amg_SoC_alpha", C.c_double),
         ("amg_max_coarse", C.c_int), ("amg_max_levels", C.c_int), ("amg_P_max_elmts", C.c_int),
         ("amg_max_row_sum", C.c_double), ("precision", C.c_int), ("amg_relax_weight", C.c_double),
+        ("layout", C.c_int),
     ]
 
 
@@ -118,6 +120,7 @@ SIGNATURES = {
     "deco_prec_apply_transpose": (i, [vp, vp, vp]),
     "deco_prec_size": (i, [vp, ip]),
     "deco_prec_set_precision": (i, [vp, i]),
+    "deco_prec_set_layout": (i, [vp, i]),
     "deco_cpr_pressure_matrix": (i, [vp, pp]),
     "deco_cpr_fsai": (i, [vp, pp]),
     "deco_cpr_amg": (i, [vp, pp]),

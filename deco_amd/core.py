@@ -291,6 +291,7 @@ def generate(cfg, seed=0, custom=None, grid=None):
 
 # ---------------------------------------------------------------------------- params -----
 _WEIGHTS = {"none": _lib.W_NONE, "quasiimpes": _lib.W_QUASIIMPES, "quasiimpes_GE": _lib.W_QUASIIMPES_GE}
+_LAYOUTS = {"block": _lib.LAYOUT_BLOCK, "csr": _lib.LAYOUT_CSR}
 
 
 def cpr_params(prec, precision=_lib.PREC_FP64):
@@ -335,6 +336,10 @@ def cpr_params(prec, precision=_lib.PREC_FP64):
     p.amg_max_row_sum = loc.get("amg_max_row_sum", float, 0.9)
     p.amg_relax_weight = loc.get("amg_relax_weight", float, 0.0)
     p.precision = precision
+    lay = prec.get("layout", str, "block")
+    if lay not in _LAYOUTS:
+        raise ValueError("Unknown layout " + lay)
+    p.layout = _LAYOUTS[lay]
     return p
 
 
@@ -393,6 +398,11 @@ class Prec:
 
     def set_precision(self, precision):
         check(lib().deco_prec_set_precision(self.h, precision))
+
+    def set_layout(self, layout):
+        """Storage of the products in either precision: LAYOUT_BLOCK (cell-block shadows, the
+        default) or LAYOUT_CSR (the CSR arrays, the reference's format and summation order)."""
+        check(lib().deco_prec_set_layout(self.h, layout))
 
     def __del__(self):
         try:

@@ -95,6 +95,10 @@ inline deco_cpr_params params_from(const deco_ptree& prec) {
   p.amg_max_row_sum = prec.get<double>("prec_loc.amg_max_row_sum", 0.9);
   p.amg_relax_weight = prec.get<double>("prec_loc.amg_relax_weight", 0.0);
   p.precision = prec.get<std::string>("precision", std::string("fp64")) == "fp32" ? DECO_PREC_FP32 : DECO_PREC_FP64;
+  const std::string lay = prec.get<std::string>("layout", std::string("block"));
+  if (lay == "block") p.layout = DECO_LAYOUT_BLOCK;
+  else if (lay == "csr") p.layout = DECO_LAYOUT_CSR;
+  else throw std::invalid_argument("Unknown layout: " + lay);
   return p;
 }
 

@@ -402,9 +402,11 @@ struct CPR : deco_prec {
     if (bs != 2 && bs != 3) fail(DECO_ERR_UNSUPPORTED, "unknown block size CPD decoupling (bs = %d)", bs);
     if (p.pressure_id != 0) fail(DECO_ERR_UNSUPPORTED, "pressure_id must be 0 (strided products assume it)");
     if (p.seq < 0 || p.seq > 3) fail(DECO_ERR_ARG, "unknown sequence, seq = %d", p.seq);
+    if (p.layout != DECO_LAYOUT_BLOCK && p.layout != DECO_LAYOUT_CSR) fail(DECO_ERR_ARG, "bad layout %d", p.layout);
     if (p.weights != DECO_W_NONE && p.weights != DECO_W_QUASIIMPES && p.weights != DECO_W_QUASIIMPES_GE)
       fail(DECO_ERR_ARG, "Unknown weights type %d", p.weights);
     precision = p.precision;
+    layout = p.layout;
     sn = n / bs;
     try {
       setup(b);
@@ -548,27 +550,46 @@ struct CPR : deco_prec {
     }
   }
 
-  // strided pressure residual rt[i] = r[bs i] - (A x)[bs i]   (CPR.hpp:241-246)
-  void pressure_residual(const double* x, const double* r, double* out) {
-    spmv_stream_f(Apr, Apr->v.p, x, EpiStrideRes<double, double>{out, r, bs});
+  // Products with the pressure rows and the decoupled A: on their cell-block shadows (1 x bs and
+  // bs x bs blocks) or the CSR arrays, as the layout selects.
+  template <class E>
+  void apr_product(const double* x, E e) {
+    if (const deco_bsr* B = blocks64() ? Apr->block64(1, bs) : nullptr) bsr_stream_d(B, x, e);
+    else spmv_stream_f(Apr, Apr->v.p, x, e);
   }
-  void pressure_residual(const float* x, const double* r, float* out) {
-    if (const deco_bsr* B = Apr->block32(1, bs)) bsr_stream_f(B, x, EpiStrideRes<float, double>{out, r, bs});
-    else spmv_stream_f(Apr, Apr->values_f32(), x, EpiStrideRes<float, double>{out, r, bs});
+  template <class E>
+  void apr_product(const float* x, E e) {
+    if (const deco_bsr* B = blocks32() ? Apr->block32(1, bs) : nullptr) bsr_stream_f(B, x, e);
+    else spmv_stream_f(Apr, Apr->values_f32(), x, e);
+  }
+  template <class E>
+  void a_product(const deco_csr* M, const double* x, E e) {
+    if (const deco_bsr* B = blocks64() ? M->block64(bs, bs) : nullptr) bsr_stream_d(B, x, e);
+    else spmv_stream_f(M, M->v.p, x, e);
+  }
+  template <class E>
+  void a_product(const deco_csr* M, const float* x, E e) {
+    if (const deco_bsr* B = blocks32() ? M->block32(bs, bs) : nullptr) bsr_stream_f(B, x, e);
+    else spmv_stream_f(M, const_cast<deco_csr*>(M)->values_f32(), x, e);
+  }
+
+  // strided pressure residual rt[i] = r[bs i] - (A x)[bs i]   (CPR.hpp:241-246)
+  template <class T>
+  void pressure_residual(const T* x, const double* r, T* out) {
+    apr_product(x, EpiStrideRes<T, double>{out, r, bs});
   }
   // the same, also writing the AMG's first zero-guess sweep x0 = rt/d0
-  void pressure_residual_j0(const double* x, const double* r, double* out, const double* d0, double* x0) {
-    spmv_stream_f(Apr, Apr->v.p, x, EpiStrideResJ0<double, double>{out, r, bs, d0, x0});
+  template <class T>
+  void pressure_residual_j0(const T* x, const double* r, T* out, const T* d0, T* x0) {
+    apr_product(x, EpiStrideResJ0<T, double>{out, r, bs, d0, x0});
   }
-  void pressure_residual_j0(const float* x, const double* r, float* out, const float* d0, float* x0) {
-    if (const deco_bsr* B = Apr->block32(1, bs))
-      bsr_stream_f(B, x, EpiStrideResJ0<float, double>{out, r, bs, d0, x0});
-    else spmv_stream_f(Apr, Apr->values_f32(), x, EpiStrideResJ0<float, double>{out, r, bs, d0, x0});
+  // out = r - A x with FP64 values of the decoupled A
+  void residual64(const double* x, const double* r, double* out) {
+    a_product(A, x, EpiStd<EPI_RES, double, double>{out, r});
   }
-  // out = r - A x with FP32 values of the decoupled A (cell blocks)
+  // out = r - A x with FP32 values of the decoupled A
   void residual32(const float* x, const double* r, float* out) {
-    if (const deco_bsr* B = A->block32(bs, bs)) bsr_stream_f(B, x, EpiStd<EPI_RES, float, double>{out, r});
-    else spmv_stream<EPI_RES, float, float, float, double>(A, A->values_f32(), x, out, r);
+    a_product(A, x, EpiStd<EPI_RES, float, double>{out, r});
   }
 
   void apply(const double* r, double* x) override {
@@ -580,7 +601,7 @@ struct CPR : deco_prec {
       DECO_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, stream()));
       hipLaunchKernelGGL((k_set_pressure<double, double>), dim3(gs()), dim3(256), 0, stream(), sn, bs, x,
                          (const double*)xp.p);
-      spmv_res(A, x, r, rt.p);
+      residual64(x, r, rt.p);
       glo->apply(rt.p, x);
       hipLaunchKernelGGL((k_add_pressure<double, double>), dim3(gs()), dim3(256), 0, stream(), sn, bs, x,
                          (const double*)xp.p);
@@ -607,7 +628,7 @@ struct CPR : deco_prec {
       DECO_CHECK_LAUNCH();
     }
     if (seq == 0) return;
-    spmv_res(A, x, r, rt.p);
+    residual64(x, r, rt.p);
     ph_rest.end();
     glo_apply_add(rt.p, x);
   }
@@ -666,12 +687,10 @@ struct CPR : deco_prec {
     }
   }
 
-  // out = r - A^T x with FP32 values of the decoupled A^T (cell blocks)
+  // out = r - A^T x with FP32 values of the decoupled A^T
   template <class TR>
   void residual32_t(const float* x, const TR* r, float* out) {
-    const deco_csr* At = A->At;
-    if (const deco_bsr* B = At->block32(bs, bs)) bsr_stream_f(B, x, EpiStd<EPI_RES, float, TR>{out, r});
-    else spmv_stream<EPI_RES, float, float, float, TR>(At, const_cast<deco_csr*>(At)->values_f32(), x, out, r);
+    a_product(A->At, x, EpiStd<EPI_RES, float, TR>{out, r});
   }
 
   // M^T u for seq 3 with FP32 internals, FP64 in and out (the sequence of apply_t)
@@ -711,20 +730,24 @@ struct CPR : deco_prec {
     double* a = tmp.p;
     double* s = tmp2.p;
     glo->apply_t(u, a);
-    spmv_res(At, a, u, s);
+    a_product(At, a, EpiStd<EPI_RES, double, double>{s, u});
     hipLaunchKernelGGL((k_gather_pressure<double, double>), dim3(gs()), dim3(256), 0, stream(), sn, bs,
                        (const double*)s, rt.p);
     loc->apply_t(rt.p, xp.p);
     DECO_HIP(hipMemsetAsync(x2.p, 0, sizeof(double) * n, stream()));
     hipLaunchKernelGGL((k_set_pressure<double, double>), dim3(gs()), dim3(256), 0, stream(), sn, bs, x2.p,
                        (const double*)xp.p);
-    spmv_res(At, x2.p, s, rt.p);   // rt (length n) = s - A^T e
+    a_product(At, x2.p, EpiStd<EPI_RES, double, double>{rt.p, s});   // rt (length n) = s - A^T e
     glo->apply_t(rt.p, x);
     hipLaunchKernelGGL(k_add3, dim3(gn()), dim3(256), 0, stream(), n, (const double*)a, (const double*)x2.p, x);
     DECO_CHECK_LAUNCH();
   }
 
   void set_precision(int pr) override { precision = pr; }
+  void set_layout(int l) override {
+    layout = l;
+    if (glo) glo->set_layout(l);
+  }
 };
 
 deco_prec* make_cpr(const deco_cpr_params& p, deco_csr* A, double* b) { return new CPR(p, A, b); }

@@ -257,11 +257,13 @@ struct FSAI : deco_prec {
   deco_csr* FLt = nullptr;   // transposed FL (transposed apply), built on demand
   DBuf<double> temp;
   DBuf<float> temp32;
-  int cb = 1;                // cell size of the unknowns: the FP32 factors are stored in cb x cb blocks
+  int cb = 1;                // cell size of the unknowns: the factors' block shadows use cb x cb blocks
 
   FSAI(const deco_cpr_params& p, const deco_csr* A_in) {
     kind = FSAI_KIND;
     n = A_in->n;
+    if (p.layout != DECO_LAYOUT_BLOCK && p.layout != DECO_LAYOUT_CSR) fail(DECO_ERR_ARG, "bad layout %d", p.layout);
+    layout = p.layout;
     if ((p.block_size == 2 || p.block_size == 3) && n % p.block_size == 0) cb = p.block_size;
     deco_csr* A = nullptr;
     const deco_csr* Au = A_in;
@@ -376,14 +378,14 @@ struct FSAI : deco_prec {
       f32_product(FUt, (const float*)temp32.p, EpiStd<EPI_SET, double, float>{x, nullptr});
       return;
     }
-    spmv(FL, r, temp.p);
-    spmv(FUt, temp.p, x);
+    f64_product(FL, r, EpiStd<EPI_SET, double, double>{temp.p, nullptr});
+    f64_product(FUt, (const double*)temp.p, EpiStd<EPI_SET, double, double>{x, nullptr});
   }
   // x += M r
   void apply_add(const double* r, double* x) {
     ProfPhase ph("FSAI apply");
-    spmv(FL, r, temp.p);
-    spmv_stream<EPI_ACC, double, double, double, double>(FUt, FUt->v.p, temp.p, x, (const double*)nullptr);
+    f64_product(FL, r, EpiStd<EPI_SET, double, double>{temp.p, nullptr});
+    f64_product(FUt, (const double*)temp.p, EpiStd<EPI_ACC, double, double>{x, nullptr});
   }
   void ensure_t() {
     if (!FLt) FLt = csr_transpose(FL);
@@ -395,18 +397,26 @@ struct FSAI : deco_prec {
       f32_product(FLt, (const float*)temp32.p, EpiStd<EPI_SET, double, float>{x, nullptr});
       return;
     }
-    spmv(FU, r, temp.p);
-    spmv(FLt, temp.p, x);
+    f64_product(FU, r, EpiStd<EPI_SET, double, double>{temp.p, nullptr});
+    f64_product(FLt, (const double*)temp.p, EpiStd<EPI_SET, double, double>{x, nullptr});
   }
   void apply_t_add(const double* r, double* x) {
     ensure_t();
-    spmv(FU, r, temp.p);
-    spmv_stream<EPI_ACC, double, double, double, double>(FLt, FLt->v.p, temp.p, x, (const double*)nullptr);
+    f64_product(FU, r, EpiStd<EPI_SET, double, double>{temp.p, nullptr});
+    f64_product(FLt, (const double*)temp.p, EpiStd<EPI_ACC, double, double>{x, nullptr});
+  }
+  // y = F x with FP64 factor values: cell blocks when selected and the unknowns come in cells,
+  // else the CSR arrays
+  template <class E>
+  void f64_product(const deco_csr* F, const double* x, E e) {
+    const deco_bsr* B = cb > 1 && blocks64() ? F->block64(cb, cb) : nullptr;
+    if (B) bsr_stream_d(B, x, e);
+    else spmv_stream_f(F, F->v.p, x, e);
   }
   // y = F x with FP32 factor values (cell blocks when the unknowns come in cells), epilogue e
   template <class XT, class E>
   void f32_product(const deco_csr* F, const XT* x, E e) {
-    const deco_bsr* B = cb > 1 ? F->block32(cb, cb) : nullptr;
+    const deco_bsr* B = cb > 1 && blocks32() ? F->block32(cb, cb) : nullptr;
     if (B) bsr_stream_f(B, x, e);
     else spmv_stream_f(F, const_cast<deco_csr*>(F)->values_f32(), x, e);
   }

@@ -9,6 +9,7 @@ struct deco_prec {
   int kind = IDENTITY;
   int n = 0;
   int precision = DECO_PREC_FP64;
+  int layout = DECO_LAYOUT_BLOCK;
   virtual ~deco_prec() = default;
   virtual void apply(const double* r, double* x) = 0;
   virtual void apply_t(const double* r, double* x) {
@@ -25,6 +26,10 @@ struct deco_prec {
     deco::fail(DECO_ERR_UNSUPPORTED, "FP32 transposed apply not available for this preconditioner");
   }
   virtual void set_precision(int p) { precision = p; }
+  virtual void set_layout(int l) { layout = l; }
+  // products on cell-block shadows (else on the CSR arrays), FP32 and FP64 alike
+  bool blocks32() const { return layout == DECO_LAYOUT_BLOCK; }
+  bool blocks64() const { return layout == DECO_LAYOUT_BLOCK; }
   // Pressure stage of CPR with two steps folded in (AMG only; false: not available, nothing
   // done). d0 receives the diagonal of the first zero-guess sweep x = b/d, which the caller's
   // residual epilogue then applies itself; stage() runs the rest of the cycle on (b, x) and adds

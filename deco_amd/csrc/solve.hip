@@ -1013,6 +1013,15 @@ int deco_prec_set_precision(deco_prec* M, int precision) {
   });
 }
 
+int deco_prec_set_layout(deco_prec* M, int layout) {
+  return guard([&] {
+    if (!M) fail(DECO_ERR_ARG, "null preconditioner");
+    if (layout != DECO_LAYOUT_BLOCK && layout != DECO_LAYOUT_CSR) fail(DECO_ERR_ARG, "bad layout %d", layout);
+    M->set_layout(layout);
+    sync();
+  });
+}
+
 int deco_workspace_release(void) {
   return guard([&] {
     sync();

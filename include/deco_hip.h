@@ -57,6 +57,13 @@ typedef struct deco_gen  deco_gen;   /* host-side synthetic reservoir system    
 #define DECO_PREC_FP64  0
 #define DECO_PREC_FP32  1
 
+/* Storage the preconditioner's products read (FSAI factors, the decoupled A and its pressure
+ * rows; the AMG levels are scalar and not affected), in either precision: cell-block shadows
+ * (default) or the CSR arrays (the reference's format and summation order). No reference
+ * counterpart (a layout choice). */
+#define DECO_LAYOUT_BLOCK 0
+#define DECO_LAYOUT_CSR   1
+
 /* Solver types ("sol.type", solver/SOL.hpp:25-38). */
 #define DECO_SOL_RGMRES    0
 #define DECO_SOL_LGMRES    1
@@ -102,6 +109,7 @@ typedef struct deco_cpr_params {
   double amg_max_row_sum;   /* default 0.9                                      */
   int    precision;         /* DECO_PREC_FP64 / DECO_PREC_FP32                  */
   double amg_relax_weight;  /* Jacobi weight; 0 = default (1 for relax 0/7, 2/3 for -1) */
+  int    layout;            /* DECO_LAYOUT_*; 0 (BLOCK) in a zero-initialised struct */
 } deco_cpr_params;
 
 /* "sol" keys (solver/rgmres.tpp:13-16, bicgstab.tpp:16-22). */
@@ -250,6 +258,9 @@ int deco_prec_apply(deco_prec* M, const double* r_dev, double* x_dev);
 int deco_prec_apply_transpose(deco_prec* M, const double* r_dev, double* x_dev);
 int deco_prec_size(const deco_prec* M, int* n);
 int deco_prec_set_precision(deco_prec* M, int precision);
+/* Storage of the preconditioner's products from now on (DECO_LAYOUT_*); block shadows are
+ * built on first use. No reference counterpart. */
+int deco_prec_set_layout(deco_prec* M, int layout);
 
 /* Introspection used by the parity tests. */
 int deco_cpr_pressure_matrix(const deco_prec* M, const deco_csr** Ap);   /* CPR::Ap         */

@@ -29,6 +29,37 @@ int main(int argc, char** argv) {
     assert out == ["2", "quasiimpes_GE", "0.2", "30", "FSAI", "missing-ok"]
 
 
+def test_prec_layout_key(tmp_path):
+    """The "layout" key of the prec tree in the C++ facade and the Python binding: block (the
+    default) or csr; anything else is an invalid argument, like an unknown weights type."""
+    import deco_amd
+    from conftest import small_params
+    assert deco_amd.cpr_params(small_params(2)).layout == deco_amd.LAYOUT_BLOCK
+    for k, v in (("csr", deco_amd.LAYOUT_CSR), ("block", deco_amd.LAYOUT_BLOCK)):
+        assert deco_amd.cpr_params(small_params(2, layout=k)).layout == v
+    with pytest.raises(ValueError):
+        deco_amd.cpr_params(small_params(2, layout="ell"))
+    src = tmp_path / "t.cpp"
+    src.write_text(r'''
+#include <cstdio>
+#include "deco.hpp"
+int main() {
+  for (const char* k : {"", "csr", "block", "ell"}) {
+    deco::ptree t;
+    t.put("block_size", "2");
+    if (*k) t.put("layout", k);
+    try { std::printf("%d\n", deco_facade::params_from(t).layout); }
+    catch (const std::invalid_argument&) { std::printf("invalid\n"); }
+  }
+  return 0;
+}''')
+    exe = tmp_path / "t"
+    subprocess.run(["g++", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-I", CPP, str(src), "-o", str(exe)],
+                   check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    assert out == [str(deco_amd.LAYOUT_BLOCK), str(deco_amd.LAYOUT_CSR), str(deco_amd.LAYOUT_BLOCK), "invalid"]
+
+
 def test_facade_compiles_against_header():
     subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
                     os.path.join(CPP, "cpr_driver.cpp")], check=True)

@@ -203,3 +203,47 @@ def test_spgemm_long_rows(gpu):
     Co = oracle.spgemm(oracle.Mat.from_scipy(S1), oracle.Mat.from_scipy(S2))
     ok, v = same_pattern(C, Co)
     assert ok and np.array_equal(v, Co.v)
+
+
+@pytest.mark.parametrize("cfg", [4, 2])
+def test_cpr_product_layouts(gpu, cfg):
+    """CPR (FSAI factors, pressure rows and decoupled A) with its products on the CSR arrays and
+    on the cell-block shadows (2 x 2 and 3 x 3 cells), in both precisions and transposed, against
+    the oracle's CPR on the same hierarchy; the layout given in the "prec" tree and switched later
+    give the same operator."""
+    g = gen_system(cfg)
+    bs = g["bs"]
+    A = dev_csr(g)
+    M = deco_amd.CPR(small_params(bs, layout="csr"), A, None)
+    Mo = oracle.CPR(deco_amd.cpr_params(small_params(bs)), to_mat(g), None)
+    Go, _, _ = oracle_amg(M.amg(), 18)
+    Mo.set_local(Go)
+    x = np.random.default_rng(11).uniform(-1, 1, g["n"])
+    yo, yto = Mo.apply(x), Mo.apply_t(x)
+    first = M.apply(x).numpy()
+    for lay in (deco_amd.LAYOUT_CSR, deco_amd.LAYOUT_BLOCK):
+        M.set_layout(lay)
+        y, yt = M.apply(x).numpy(), M.apply_t(x).numpy()
+        assert rel(y, yo) <= 1e-12 and rel(yt, yto) <= 1e-11
+        if lay == deco_amd.LAYOUT_CSR:
+            assert np.array_equal(y, first)
+        M.set_precision(deco_amd.PREC_FP32)
+        y32, yt32 = M.apply(x).numpy(), M.apply_t(x).numpy()
+        M.set_precision(deco_amd.PREC_FP64)
+        assert rel(y32, yo) <= 2e-4 and rel(yt32, yto) <= 2e-4
+
+
+def test_fsai_product_layouts(gpu):
+    """Standalone FSAI with FP64 products on its factors' 2 x 2 block shadows and on CSR."""
+    g = gen_system(4)
+    A = dev_csr(g)
+    tree = small_params(2)
+    F = deco_amd.FSAI(tree, A)
+    Fo = oracle.fsai(deco_amd.cpr_params(tree), to_mat(g))
+    x = np.random.default_rng(12).uniform(-1, 1, g["n"])
+    for lay in (deco_amd.LAYOUT_BLOCK, deco_amd.LAYOUT_CSR):
+        F.set_layout(lay)
+        assert rel(F.apply(x).numpy(), Fo.apply(x)) <= 1e-13
+        assert rel(F.apply_t(x).numpy(), Fo.apply_t(x)) <= 1e-13
+    with pytest.raises(deco_amd._lib.DecoError):
+        F.set_layout(7)

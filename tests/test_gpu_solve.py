@@ -190,3 +190,26 @@ def test_standalone_fp32_preconditioners_in_gmres(gpu, kind):
     assert S32.res_rel <= 1e-9
     assert abs(S32.iter - S64.iter) <= 2
     assert rel(S32.x.numpy(), S64.x.numpy()) <= 1e-6
+
+
+def test_product_layouts_same_solve(gpu):
+    """GMRES with the FP64 preconditioner on cell blocks and on CSR: the same operator to
+    rounding, so the same iteration count and solutions within 1e-10; the FP32 preconditioner on
+    either storage reaches rtol 1e-8 within 1e-6 of the FP64 solution."""
+    g, A, b, M, Ao, bo, Mo = setup_pair(1, small_params(2))
+    sol = {"type": "rgmres", "reltol": "1e-8", "restart": "30", "itmax": "300", "verbosity": "0"}
+    res = {}
+    for prec in (deco_amd.PREC_FP64, deco_amd.PREC_FP32):
+        M.set_precision(prec)
+        for lay in (deco_amd.LAYOUT_CSR, deco_amd.LAYOUT_BLOCK):
+            M.set_layout(lay)
+            S = deco_amd.SOL(sol, A, M, b)
+            assert S.flag == 0 and S.res_rel < 1e-8
+            res[prec, lay] = (S.iter, S.x.numpy())
+    M.set_precision(deco_amd.PREC_FP64)
+    M.set_layout(deco_amd.LAYOUT_BLOCK)
+    (i0, x0), (i1, x1) = res[deco_amd.PREC_FP64, deco_amd.LAYOUT_CSR], res[deco_amd.PREC_FP64, deco_amd.LAYOUT_BLOCK]
+    assert i0 == i1 and rel(x1, x0) <= 1e-10
+    for lay in (deco_amd.LAYOUT_CSR, deco_amd.LAYOUT_BLOCK):
+        it, x32 = res[deco_amd.PREC_FP32, lay]
+        assert abs(it - i0) <= 2 and rel(x32, x0) <= 1e-6
