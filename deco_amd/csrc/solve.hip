@@ -131,15 +131,16 @@ __global__ __launch_bounds__(kArnBlock) void k_arn_dots(int n, int j, const doub
     part[(size_t)v * nb + blockIdx.x] = ((red[v * 4] + red[v * 4 + 1]) + red[v * 4 + 2]) + red[v * 4 + 3];
 }
 
-// out[v] = sum over the nb block partials of value v (one wavefront per value, fixed order)
-__global__ __launch_bounds__(256) void k_arn_reduce(int nv, int nb, const double* __restrict__ part,
-                                                    double* __restrict__ out) {
-  const int v = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (v >= nv) return;
+// out[v] = sum over the nb block partials of value v: one workgroup per value, every lane a
+// strided subset (loads issued together), then a fixed-order workgroup sum
+__global__ __launch_bounds__(256) void k_arn_reduce(int nb, const double* __restrict__ part, double* __restrict__ out) {
+  __shared__ double sm[4];
+  const double* p = part + (size_t)blockIdx.x * nb;
   double s = 0.0;
-  for (int b = lane; b < nb; b += 64) s += part[(size_t)v * nb + b];
-  s = wave_sum(s);
-  if (lane == 0) out[v] = s;
+#pragma unroll 8
+  for (int b = threadIdx.x; b < nb; b += 256) s += p[b];
+  s = block_sum<256>(s, sm);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
 // The step's scalars from the reduced values (same operation order on host and device):
@@ -317,20 +318,24 @@ static void comp_givens(double x, double y, double& c, double& s) {      // rgmr
   else { const double nu = y / x; c = 1.0 / std::sqrt(1.0 + nu * nu); s = nu * c; }
 }
 
+// The solvers' operator: products with A (or A^T) on its FP64 cell-block shadow when the CPR
+// setup recorded a cell size, else on the CSR arrays.
 struct AOp {
   const deco_csr* A;
   bool transpose;
-  void apply(const double* x, double* y) const {
+  template <class E>
+  void product(const double* x, E e) const {
     const deco_csr* M = transpose ? A->At : A;
     if (A->cell > 1)
       if (const deco_bsr* B = M->block64(A->cell, A->cell)) {
-        bsr_stream_d(B, x, EpiOuter{y});
+        bsr_stream_d(B, x, e);
         return;
       }
-    spmv_stream_f(M, M->v.p, x, EpiOuter{y});
+    spmv_stream_f(M, M->v.p, x, e);
   }
+  void apply(const double* x, double* y) const { product(x, EpiOuter{y}); }
   void residual(const double* x, const double* b, double* r) const {  // r = b - A x
-    spmv_res(transpose ? A->At : A, x, b, r);
+    product(x, EpiStd<EPI_RES, double, double>{r, b});
   }
 };
 
@@ -528,8 +533,7 @@ static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, cons
       hipLaunchKernelGGL(k_arn_dots, dim3(nbk), dim3(kArnBlock), sizeof(double) * 4 * (2 * j + 2), stream(), n, j,
                          (const double*)V.p, uu, wp, part.p, nbk);
       DECO_CHECK_LAUNCH();
-      hipLaunchKernelGGL(k_arn_reduce, dim3((nv + 3) / 4), dim3(256), 0, stream(), nv, nbk, (const double*)part.p,
-                         redd.p);
+      hipLaunchKernelGGL(k_arn_reduce, dim3(nv), dim3(256), 0, stream(), nbk, (const double*)part.p, redd.p);
       DECO_CHECK_LAUNCH();
       if (wp) {   // queued before the host decision; unused if the cycle ends here
         hipLaunchKernelGGL(k_arn_update, dim3(nbk), dim3(kArnBlock), 0, stream(), n, j, (const double*)V.p, uu, wp,
