@@ -28,8 +28,10 @@ void fsai_apply_t_add(FSAI* F, const double* r, double* x);
 void fsai_apply32(FSAI* F, const double* r, float* x);
 void fsai_apply32_f(FSAI* F, const float* r, float* x);
 void fsai_apply32_addz(FSAI* F, const float* r, const float* z, double* x);
+void fsai_apply32_addz(FSAI* F, const float* r, const float* z, float* x);
 void fsai_apply32_t(FSAI* F, const double* r, float* x);
 void fsai_apply32_t_addz(FSAI* F, const float* r, const float* z, double* x);
+void fsai_apply32_t_addz(FSAI* F, const float* r, const float* z, float* x);
 
 enum CprErr { ERR_DET = 1, ERR_ROWLEN = 2 };
 
@@ -368,10 +370,10 @@ __global__ void k_gather_pressure(int sn, int bs, const TI* __restrict__ r, TO* 
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < sn) rt[i] = (TO)r[(size_t)i * bs];
 }
-template <class TA, class TB>
-__global__ void k_addz(int n, const TA* __restrict__ z, const TB* __restrict__ a, double* __restrict__ x) {
+template <class TA, class TB, class TO = double>
+__global__ void k_addz(int n, const TA* __restrict__ z, const TB* __restrict__ a, TO* __restrict__ x) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;  // x = z + a (z, a of any precision)
-  if (i < n) x[i] = (double)z[i] + (double)a[i];
+  if (i < n) x[i] = (TO)((double)z[i] + (double)a[i]);
 }
 __global__ void k_add_f(int n, const float* __restrict__ a, float* __restrict__ e) {   // e += a
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -633,8 +635,9 @@ struct CPR : deco_prec {
     glo_apply_add(rt.p, x);
   }
 
-  // FP32 internals, FP64 in and out
-  void apply32_seq(const double* r, double* xo) {
+  // FP32 internals, FP64 in, FP64 or FP32 out
+  template <class TO>
+  void apply32_seq(const double* r, TO* xo) {
     deco::FSAI* F = as_fsai(glo);
     auto glo32_from64 = [&](const double* in, float* out) {
       if (F) fsai_apply32(F, in, out);
@@ -655,7 +658,7 @@ struct CPR : deco_prec {
       else glo->apply32(rt32.p, x32.p);
       hipLaunchKernelGGL((k_add_pressure<float, float>), dim3(gs()), dim3(256), 0, stream(), sn, bs, x32.p,
                          (const float*)xp32.p);
-      convert(x32.p, xo, n);
+      convert((const float*)x32.p, xo, n);
       return;
     }
     if (seq == 2) { convert(r, t32.p, n); jac->apply32(t32.p, x32.p); }
@@ -675,13 +678,13 @@ struct CPR : deco_prec {
                          (const float*)xp32.p);
       DECO_CHECK_LAUNCH();
     }
-    if (seq == 0) { convert(x32.p, xo, n); return; }
+    if (seq == 0) { convert((const float*)x32.p, xo, n); return; }
     residual32(x32.p, r, rt32.p);
     ph_rest.end();
     if (F) fsai_apply32_addz(F, rt32.p, x32.p, xo);
     else {
       glo->apply32(rt32.p, t32.p);
-      hipLaunchKernelGGL((k_addz<float, float>), dim3(gn()), dim3(256), 0, stream(), n, (const float*)x32.p,
+      hipLaunchKernelGGL((k_addz<float, float, TO>), dim3(gn()), dim3(256), 0, stream(), n, (const float*)x32.p,
                          (const float*)t32.p, xo);
       DECO_CHECK_LAUNCH();
     }
@@ -693,8 +696,9 @@ struct CPR : deco_prec {
     a_product(A->At, x, EpiStd<EPI_RES, float, TR>{out, r});
   }
 
-  // M^T u for seq 3 with FP32 internals, FP64 in and out (the sequence of apply_t)
-  void apply32_t_seq(const double* u, double* xo) {
+  // M^T u for seq 3 with FP32 internals, FP64 in, FP64 or FP32 out (the sequence of apply_t)
+  template <class TO>
+  void apply32_t_seq(const double* u, TO* xo) {
     deco::FSAI* F = as_fsai(glo);
     if (!F) fail(DECO_ERR_UNSUPPORTED, "the FP32 transposed CPR needs the FSAI global stage");
     s32.ensure(n);
@@ -743,6 +747,16 @@ struct CPR : deco_prec {
     DECO_CHECK_LAUNCH();
   }
 
+  void apply_f32out(const double* r, float* x) override {
+    if (precision != DECO_PREC_FP32) { deco_prec::apply_f32out(r, x); return; }
+    apply32_seq(r, x);
+  }
+  void apply_t_f32out(const double* u, float* x) override {
+    if (precision != DECO_PREC_FP32) { deco_prec::apply_t_f32out(u, x); return; }
+    if (p.seq != 3) fail(DECO_ERR_UNSUPPORTED, "transposed CPR is implemented for seq 3");
+    if (!A->At) A->At = csr_transpose(A);
+    apply32_t_seq(u, x);
+  }
   void set_precision(int pr) override { precision = pr; }
   void set_layout(int l) override {
     layout = l;

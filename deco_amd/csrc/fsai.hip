@@ -427,12 +427,12 @@ struct FSAI : deco_prec {
     f32_product(FL, r, EpiStd<EPI_SET, float, float>{temp32.p, nullptr});
     f32_product(FUt, (const float*)temp32.p, EpiStd<EPI_SET, float, float>{x, nullptr});
   }
-  // xo (f64) = z (f32) + M r
-  template <class TI>
-  void apply32_addz(const TI* r, const float* z, double* xo) {
+  // xo = z (f32) + M r, output f64 or f32
+  template <class TI, class TO>
+  void apply32_addz(const TI* r, const float* z, TO* xo) {
     ProfPhase ph("FSAI apply");
     f32_product(FL, r, EpiStd<EPI_SET, float, float>{temp32.p, nullptr});
-    f32_product(FUt, (const float*)temp32.p, EpiStd<EPI_ADDZ, double, float>{xo, z});
+    f32_product(FUt, (const float*)temp32.p, EpiStd<EPI_ADDZ, TO, float>{xo, z});
   }
   template <class TI>
   void apply32_t_in(const TI* r, float* x) {
@@ -441,15 +441,23 @@ struct FSAI : deco_prec {
     f32_product(FU, r, EpiStd<EPI_SET, float, float>{temp32.p, nullptr});
     f32_product(FLt, (const float*)temp32.p, EpiStd<EPI_SET, float, float>{x, nullptr});
   }
-  template <class TI>
-  void apply32_t_addz(const TI* r, const float* z, double* xo) {
+  template <class TI, class TO>
+  void apply32_t_addz(const TI* r, const float* z, TO* xo) {
     ProfPhase ph("FSAI apply");
     ensure_t();
     f32_product(FU, r, EpiStd<EPI_SET, float, float>{temp32.p, nullptr});
-    f32_product(FLt, (const float*)temp32.p, EpiStd<EPI_ADDZ, double, float>{xo, z});
+    f32_product(FLt, (const float*)temp32.p, EpiStd<EPI_ADDZ, TO, float>{xo, z});
   }
   void apply32(const float* r, float* x) override { apply32_in(r, x); }
   void apply32_t(const float* r, float* x) override { apply32_t_in(r, x); }
+  void apply_f32out(const double* r, float* x) override {
+    if (precision != DECO_PREC_FP32) { deco_prec::apply_f32out(r, x); return; }
+    apply32_in(r, x);
+  }
+  void apply_t_f32out(const double* r, float* x) override {
+    if (precision != DECO_PREC_FP32) { deco_prec::apply_t_f32out(r, x); return; }
+    apply32_t_in(r, x);
+  }
 };
 
 deco_prec* make_fsai(const deco_cpr_params& p, const deco_csr* A) { return new FSAI(p, A); }
@@ -459,9 +467,15 @@ void fsai_apply_add(FSAI* F, const double* r, double* x) { F->apply_add(r, x); }
 void fsai_apply_t_add(FSAI* F, const double* r, double* x) { F->apply_t_add(r, x); }
 void fsai_apply32(FSAI* F, const double* r, float* x) { F->apply32_in<double>(r, x); }
 void fsai_apply32_f(FSAI* F, const float* r, float* x) { F->apply32_in<float>(r, x); }
-void fsai_apply32_addz(FSAI* F, const float* r, const float* z, double* x) { F->apply32_addz<float>(r, z, x); }
+void fsai_apply32_addz(FSAI* F, const float* r, const float* z, double* x) { F->apply32_addz<float, double>(r, z, x); }
+void fsai_apply32_addz(FSAI* F, const float* r, const float* z, float* x) { F->apply32_addz<float, float>(r, z, x); }
 void fsai_apply32_t(FSAI* F, const double* r, float* x) { F->apply32_t_in<double>(r, x); }
-void fsai_apply32_t_addz(FSAI* F, const float* r, const float* z, double* x) { F->apply32_t_addz<float>(r, z, x); }
+void fsai_apply32_t_addz(FSAI* F, const float* r, const float* z, double* x) {
+  F->apply32_t_addz<float, double>(r, z, x);
+}
+void fsai_apply32_t_addz(FSAI* F, const float* r, const float* z, float* x) {
+  F->apply32_t_addz<float, float>(r, z, x);
+}
 
 }  // namespace deco
 

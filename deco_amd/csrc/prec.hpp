@@ -25,6 +25,11 @@ struct deco_prec {
     (void)r; (void)x;
     deco::fail(DECO_ERR_UNSUPPORTED, "FP32 transposed apply not available for this preconditioner");
   }
+  // x = M^{-1} r (M^{-T} r) written in FP32: the flexible GMRES directions of an FP32
+  // preconditioner, whose results are FP32 values already. Default: the FP64-output apply and a
+  // conversion; CPR and FSAI write FP32 directly.
+  virtual void apply_f32out(const double* r, float* x);
+  virtual void apply_t_f32out(const double* r, float* x);
   virtual void set_precision(int p) { precision = p; }
   virtual void set_layout(int l) { layout = l; }
   // products on cell-block shadows (else on the CSR arrays), FP32 and FP64 alike
@@ -81,8 +86,18 @@ struct Jacobi : deco_prec {
   void apply32_t(const float* r, float* x) override;
 };
 
+inline void convert_apply(deco_prec* M, bool tr, const double* r, float* x) {
+  DBuf<double> t(M->n);
+  if (tr) M->apply_t(r, t.p);
+  else M->apply(r, t.p);
+  convert((const double*)t.p, x, M->n);
+}
+
 deco_prec* make_fsai(const deco_cpr_params& p, const deco_csr* A);
 deco_prec* make_amg(const deco_cpr_params& p, const deco_csr* Ap);
 deco_prec* make_cpr(const deco_cpr_params& p, deco_csr* A, double* b);
 
 }  // namespace deco
+
+inline void deco_prec::apply_f32out(const double* r, float* x) { deco::convert_apply(this, false, r, x); }
+inline void deco_prec::apply_t_f32out(const double* r, float* x) { deco::convert_apply(this, true, r, x); }

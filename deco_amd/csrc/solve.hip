@@ -65,12 +65,13 @@ __global__ __launch_bounds__(kRedBlock) void k_mgs_norm(int n, const double* __r
 }
 
 // w = g0*V0 + sum_{i=1}^{j-1} g_i V_i in the order of rgmres.tpp:106-109
-__global__ void k_combine(int n, int j, const double* __restrict__ V, int64_t ldv, const double* __restrict__ g,
+template <class QT>
+__global__ void k_combine(int n, int j, const QT* __restrict__ V, int64_t ldv, const double* __restrict__ g,
                           double* __restrict__ w) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  double s = g[0] * V[k];
-  for (int i = 1; i < j; ++i) s = 1.0 * s + g[i] * V[(int64_t)i * ldv + k];
+  double s = g[0] * (double)V[k];
+  for (int i = 1; i < j; ++i) s = 1.0 * s + g[i] * (double)V[(int64_t)i * ldv + k];
   w[k] = s;
 }
 
@@ -246,8 +247,8 @@ __global__ void k_axpby2(int n, double a, const double* __restrict__ x, double b
 
 // ------------------------------------------------------------------ workspace ----------
 // Work vectors of the Krylov solvers, kept between solves and grown on demand, so that a solve
-// does not allocate and free its basis (1.6 GB at 4M unknowns and restart 50, twice that with
-// flexible directions) every time. One solve at a time uses them (the library runs on one
+// does not allocate and free its basis (1.6 GB at 4M unknowns and restart 50, half as much again
+// for the FP32 flexible directions) every time. One solve at a time uses them (the library runs on one
 // stream); deco_workspace_release() frees them.
 constexpr int kWsSlots = 12;
 static DBuf<double> g_ws[kWsSlots];
@@ -323,17 +324,19 @@ static void comp_givens(double x, double y, double& c, double& s) {      // rgmr
 struct AOp {
   const deco_csr* A;
   bool transpose;
-  template <class E>
-  void product(const double* x, E e) const {
+  template <class XT, class E>
+  void product(const XT* x, E e) const {
     const deco_csr* M = transpose ? A->At : A;
     if (A->cell > 1)
       if (const deco_bsr* B = M->block64(A->cell, A->cell)) {
-        bsr_stream_d(B, x, e);
+        bsr_stream_v<double>(B, B->bv64.p, x, e);
         return;
       }
     spmv_stream_f(M, M->v.p, x, e);
   }
   void apply(const double* x, double* y) const { product(x, EpiOuter{y}); }
+  // x held in FP32 (flexible GMRES directions), FP64 values and sums
+  void apply(const float* x, double* y) const { product(x, EpiOuter{y}); }
   void residual(const double* x, const double* b, double* r) const {  // r = b - A x
     product(x, EpiStd<EPI_RES, double, double>{r, b});
   }
@@ -348,7 +351,19 @@ struct MOp {
     else if (transpose) M->apply_t(r, x);
     else M->apply(r, x);
   }
+  // the same written in FP32 (an FP32 preconditioner's flexible GMRES directions)
+  void apply_f32(const double* r, float* x) const {
+    if (transpose) M->apply_t_f32out(r, x);
+    else M->apply_f32out(r, x);
+  }
 };
+
+// Flexible GMRES directions Z = [M^{-1} v_j], n x m, stored in FP32: an FP32 preconditioner's
+// results are FP32 values, and the method only needs the same stored z_j in w = A z_j and in
+// the update x += Z y.
+static float* flex_basis(int n, int m) {
+  return reinterpret_cast<float*>(ws(6, ((size_t)n * m + 1) / 2).p);
+}
 
 // rgmres (solver/rgmres.tpp:9-139) and lgmres (solver/lgmres.tpp:2-134)
 static void gmres(bool left, const deco_sol_params& sp, const AOp& A, const MOp& M, const double* b, double* x,
@@ -365,7 +380,7 @@ static void gmres(bool left, const deco_sol_params& sp, const AOp& A, const MOp&
   const bool flex = !left && M.M && M.M->precision == DECO_PREC_FP32;
   WsRef r = ws(0, n), z = ws(1, n), w = ws(2, n), V = ws(3, (size_t)n * (m + 1)), gdev = ws(4, m + 1),
         hcol = ws(5, m + 2);
-  WsRef Z = flex ? ws(6, (size_t)n * m) : WsRef{nullptr};
+  float* Z = flex ? flex_basis(n, m) : nullptr;
   std::vector<double> H((size_t)(m + 1) * (m + 3), 0.0);
   double* g = H.data() + (size_t)(m + 1) * m;
   double* cs = g + (m + 1);
@@ -395,10 +410,13 @@ static void gmres(bool left, const deco_sol_params& sp, const AOp& A, const MOp&
       rnorm = std::fabs(g[j]);
       if (rnorm < absTol) { flag = 0; break; }
       double* vj = V.p + (size_t)j * n;
-      if (!left) {
-        double* zj = flex ? Z.p + (size_t)j * n : z.p;
-        M.apply(vj, zj);
+      if (!left && flex) {
+        float* zj = Z + (size_t)j * n;
+        M.apply_f32(vj, zj);
         A.apply(zj, w.p);
+      } else if (!left) {
+        M.apply(vj, z.p);
+        A.apply(z.p, w.p);
       } else { A.apply(vj, z.p); M.apply(z.p, w.p); }
       ++napp; ++nspmv;
       // fused MGS: j+2 kernels, partial slots ping-pong
@@ -432,11 +450,13 @@ static void gmres(bool left, const deco_sol_params& sp, const AOp& A, const MOp&
       for (int k = 0; k < i; ++k) g[k] -= Hm(k, i) * g[i];
     }
     copy_h2d(gdev.p, g, std::max(j, 1));
-    if (!flex || j > 0) {
-      hipLaunchKernelGGL(k_combine, dim3(S.g256()), dim3(256), 0, stream(), n, j, (const double*)(flex ? Z.p : V.p),
+    if (flex && j > 0)
+      hipLaunchKernelGGL(k_combine<float>, dim3(S.g256()), dim3(256), 0, stream(), n, j, (const float*)Z, (int64_t)n,
+                         (const double*)gdev.p, w.p);
+    else if (!flex)
+      hipLaunchKernelGGL(k_combine<double>, dim3(S.g256()), dim3(256), 0, stream(), n, j, (const double*)V.p,
                          (int64_t)n, (const double*)gdev.p, w.p);
-      DECO_CHECK_LAUNCH();
-    }
+    DECO_CHECK_LAUNCH();
     if (flex) {
       if (j > 0) S.add(w.p, x);
       A.residual(x, b, r.p); ++nspmv;
@@ -486,7 +506,7 @@ static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, cons
   WsRef r = ws(0, n), z = ws(1, n), w = ws(2, n), ua = ws(7, n), ub = ws(8, n), V = ws(3, (size_t)n * (m + 1)),
         gdev = ws(4, m + 1);
   WsRef part = ws(9, (size_t)(2 * m + 2) * nbk), redd = ws(10, 2 * m + 2);
-  WsRef Z = flex ? ws(6, (size_t)n * m) : WsRef{nullptr};
+  float* Z = flex ? flex_basis(n, m) : nullptr;
   const size_t ld = (size_t)m + 1;
   std::vector<double> Hr(ld * m), Rm(ld * m), T(flex ? ld * m : 0), g(m + 1), cs(m), sn(m), hs(m + 1);
   double* red = ctx().pinned;
@@ -505,10 +525,13 @@ static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, cons
   }
   // w = B u_j for step j (B = A M^{-1} right, M^{-1} A left); z / Z column j hold M^{-1} u_j
   auto apply_op = [&](int jj, const double* uu) {
-    if (!left) {
-      double* zj = flex ? Z.p + (size_t)jj * n : z.p;
-      M.apply(uu, zj);
+    if (!left && flex) {
+      float* zj = Z + (size_t)jj * n;
+      M.apply_f32(uu, zj);
       A.apply(zj, w.p);
+    } else if (!left) {
+      M.apply(uu, z.p);
+      A.apply(z.p, w.p);
     } else { A.apply(uu, z.p); M.apply(z.p, w.p); }
     ++napp; ++nspmv;
   };
@@ -646,8 +669,12 @@ static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, cons
         }
       }
       copy_h2d(gdev.p, g.data(), k);
-      hipLaunchKernelGGL(k_combine, dim3(S.g256()), dim3(256), 0, stream(), n, k, (const double*)(flex ? Z.p : V.p),
-                         (int64_t)n, (const double*)gdev.p, w.p);
+      if (flex)
+        hipLaunchKernelGGL(k_combine<float>, dim3(S.g256()), dim3(256), 0, stream(), n, k, (const float*)Z,
+                           (int64_t)n, (const double*)gdev.p, w.p);
+      else
+        hipLaunchKernelGGL(k_combine<double>, dim3(S.g256()), dim3(256), 0, stream(), n, k, (const double*)V.p,
+                           (int64_t)n, (const double*)gdev.p, w.p);
       DECO_CHECK_LAUNCH();
       if (flex || left) S.add(w.p, x);
       else { M.apply(w.p, z.p); ++napp; S.add(z.p, x); }

@@ -55,17 +55,19 @@ def test_gmres_gram_schmidt_variants(gpu, typ, m):
         assert S.flag == 0, ortho
         assert abs(S.iter - info.iter) <= (0 if ortho == "mgs" else 2), (ortho, S.iter, info.iter)
         assert rel(S.x.numpy(), xo) <= (1e-10 if ortho == "mgs" else 1e-6), ortho
-    # FP32 preconditioner. Right variant (flexible directions): same iterations, same x. The
-    # left variant's residual is itself a product with the FP32 preconditioner, so reaching a
-    # tight tolerance on it takes extra iterations; its x still agrees.
+    # FP32 preconditioner, both orthogonalisations. Right variant (flexible directions, kept in
+    # FP32): same iterations, same x. The left variant's residual is itself a product with the
+    # FP32 preconditioner, so reaching a tight tolerance on it takes extra iterations; its x
+    # still agrees.
     S64 = deco_amd.SOL(sol, A, M, b)
     M.set_precision(deco_amd.PREC_FP32)
-    S32 = deco_amd.SOL(sol, A, M, b)
+    S32 = {o: deco_amd.SOL(dict(sol, ortho=o), A, M, b) for o in ("dcgs2", "mgs")}
     M.set_precision(deco_amd.PREC_FP64)
-    assert S32.flag == 0
-    if typ == "rgmres":
-        assert abs(S32.iter - S64.iter) <= 2
-    assert rel(S32.x.numpy(), S64.x.numpy()) <= 1e-6
+    for o, S in S32.items():
+        assert S.flag == 0, o
+        if typ == "rgmres":
+            assert abs(S.iter - S64.iter) <= 2, o
+        assert rel(S.x.numpy(), S64.x.numpy()) <= 1e-6, o
 
 
 def g_bs(cfg):
