@@ -49,6 +49,8 @@ extern "C" {
 int deco_dot(int n, const double* x, const double* y, double* result) {
   return guard([&] {
     if (n < 0 || !result) fail(DECO_ERR_ARG, "bad argument");
+    need(x, n, "x");
+    need(y, n, "y");
     RedScratch rs;
     *result = dot_host(n, x, y, rs);
   });
@@ -57,6 +59,7 @@ int deco_dot(int n, const double* x, const double* y, double* result) {
 int deco_nrm2(int n, const double* x, double* result) {
   return guard([&] {
     if (n < 0 || !result) fail(DECO_ERR_ARG, "bad argument");
+    need(x, n, "x");
     RedScratch rs;
     *result = nrm2_host(n, x, rs);
   });
@@ -65,6 +68,8 @@ int deco_nrm2(int n, const double* x, double* result) {
 int deco_axpby(int n, double a, const double* x, double b, double* y) {
   return guard([&] {
     if (n < 0) fail(DECO_ERR_ARG, "bad argument");
+    need(x, n, "x");
+    need(y, n, "y");
     if (n == 0) return;
     hipLaunchKernelGGL(k_axpby, dim3(grid_for(n, 256)), dim3(256), 0, stream(), n, a, x, b, y);
     DECO_CHECK_LAUNCH();

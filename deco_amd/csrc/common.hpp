@@ -188,6 +188,23 @@ __device__ __forceinline__ void wave_bin_append(int bin, int item, int* cnt, int
   }
 }
 
+// A HIP event owned by its scope (destroyed on every exit path, errors included).
+struct Event {
+  hipEvent_t e = nullptr;
+  explicit Event(unsigned flags = hipEventDefault) { DECO_HIP(hipEventCreateWithFlags(&e, flags)); }
+  ~Event() {
+    if (e) (void)hipEventDestroy(e);
+  }
+  Event(const Event&) = delete;
+  Event& operator=(const Event&) = delete;
+};
+
+// A device-array argument of the C ABI: NULL is an error unless the array is empty (a kernel
+// would fault on it).
+inline void need(const void* p, int64_t count, const char* what) {
+  if (count > 0 && !p) fail(DECO_ERR_ARG, "null %s", what);
+}
+
 // Block-wide sum of one value per thread, fixed order (deterministic). BLOCK multiple of 64.
 template <int BLOCK, class T>
 __device__ __forceinline__ T block_sum(T v, T* smem /* BLOCK/64 entries */) {

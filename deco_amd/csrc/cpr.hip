@@ -817,6 +817,8 @@ int deco_cpr_left_transform(const deco_prec* M, double* W) {
 int deco_cpr_transform_transpose(const deco_prec* M, const double* mu, double* lam) {
   return guard([&] {
     CPR* c = as_cpr(M);
+    need(mu, c->n, "mu");
+    need(lam, c->n, "lambda");
     hipLaunchKernelGGL(k_transform_t, dim3(grid_for(c->sn, 128)), dim3(128), 0, stream(), c->sn, c->bs,
                        (const double*)c->W.p, mu, lam);
     DECO_CHECK_LAUNCH();

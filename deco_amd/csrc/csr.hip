@@ -897,6 +897,8 @@ int deco_free(void* p) {
 
 int deco_memcpy_h2d(void* dst, const void* src, size_t bytes) {
   return guard([&] {
+    need(dst, (int64_t)bytes, "destination");
+    need(src, (int64_t)bytes, "source");
     if (bytes) DECO_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream()));
     sync();
   });
@@ -904,6 +906,8 @@ int deco_memcpy_h2d(void* dst, const void* src, size_t bytes) {
 
 int deco_memcpy_d2h(void* dst, const void* src, size_t bytes) {
   return guard([&] {
+    need(dst, (int64_t)bytes, "destination");
+    need(src, (int64_t)bytes, "source");
     if (bytes) DECO_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream()));
     sync();
   });
@@ -911,6 +915,8 @@ int deco_memcpy_d2h(void* dst, const void* src, size_t bytes) {
 
 int deco_memcpy_d2d(void* dst, const void* src, size_t bytes) {
   return guard([&] {
+    need(dst, (int64_t)bytes, "destination");
+    need(src, (int64_t)bytes, "source");
     if (bytes) DECO_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream()));
     sync();
   });
@@ -918,6 +924,7 @@ int deco_memcpy_d2d(void* dst, const void* src, size_t bytes) {
 
 int deco_memset(void* dst, int value, size_t bytes) {
   return guard([&] {
+    need(dst, (int64_t)bytes, "destination");
     if (bytes) DECO_HIP(hipMemsetAsync(dst, value, bytes, stream()));
     sync();
   });
@@ -946,6 +953,9 @@ int deco_csr_create_dev(int n, int m, int nnz, const int* r, const int* c, const
   return guard([&] {
     check_csr_args(n, m, nnz);
     if (!out) fail(DECO_ERR_ARG, "null output");
+    need(r, (int64_t)n + 1, "r");
+    need(c, nnz, "c");
+    need(v, nnz, "v");
     auto* A = csr_new(n, m);
     A->nnz = nnz;
     A->r.alloc(n + 1);
@@ -1051,6 +1061,8 @@ int deco_spmv(const deco_csr* A, const double* x, double* y, int bs) {
   return guard([&] {
     if (!A) fail(DECO_ERR_ARG, "null CSR");
     if (bs < 1 || A->n % bs != 0) fail(DECO_ERR_ARG, "n=%d not divisible by bs=%d", A->n, bs);
+    need(x, A->m, "x");
+    need(y, A->n / bs, "y");
     if (bs == 1) spmv(A, x, y);
     else
       launch_spmv<EPI_SET, double, double, double, double>(csr_group(A), A->n / bs, bs, A->r.p, A->c.p, A->v.p, x, y,
@@ -1062,6 +1074,9 @@ int deco_spmv(const deco_csr* A, const double* x, double* y, int bs) {
 int deco_spmv_residual(const deco_csr* A, const double* x, const double* z, double* y) {
   return guard([&] {
     if (!A) fail(DECO_ERR_ARG, "null CSR");
+    need(x, A->m, "x");
+    need(z, A->n, "z");
+    need(y, A->n, "y");
     spmv_res(A, x, z, y);
     sync();
   });
@@ -1070,6 +1085,8 @@ int deco_spmv_residual(const deco_csr* A, const double* x, const double* z, doub
 int deco_spmv_transpose(deco_csr* A, const double* x, double* y) {
   return guard([&] {
     if (!A) fail(DECO_ERR_ARG, "null CSR");
+    need(x, A->n, "x");
+    need(y, A->m, "y");
     if (!A->At) A->At = csr_transpose(A);
     spmv(A->At, x, y);
     sync();
@@ -1079,6 +1096,8 @@ int deco_spmv_transpose(deco_csr* A, const double* x, double* y) {
 int deco_spmv_f32(deco_csr* A, const float* x, float* y) {
   return guard([&] {
     if (!A) fail(DECO_ERR_ARG, "null CSR");
+    need(x, A->m, "x");
+    need(y, A->n, "y");
     const float* vf = A->values_f32();
     spmv_stream<EPI_SET, float, float, float, float>(A, vf, x, y, (const float*)nullptr);
     sync();
@@ -1088,6 +1107,8 @@ int deco_spmv_f32(deco_csr* A, const float* x, float* y) {
 int deco_spmv_block_f32(deco_csr* A, int R, int C, const float* x, float* y) {
   return guard([&] {
     if (!A) fail(DECO_ERR_ARG, "null CSR");
+    need(x, A->m, "x");
+    need(y, A->n, "y");
     const deco_bsr* B = A->block32(R, C);
     if (!B) fail(DECO_ERR_ARG, "%d x %d blocks do not tile a %d x %d matrix", R, C, A->n, A->m);
     bsr_stream_f(B, x, EpiStd<EPI_SET, float, float>{y, nullptr});
@@ -1125,9 +1146,8 @@ int deco_read_bandwidth(size_t bytes, int iters, double* gbps) {
     buf.zero();
     hipLaunchKernelGGL(k_read_stream, dim3(G), dim3(256), 0, stream(), (const f64x2*)buf.p, n2, part.p);
     DECO_CHECK_LAUNCH();
-    hipEvent_t e0, e1;
-    DECO_HIP(hipEventCreate(&e0));
-    DECO_HIP(hipEventCreate(&e1));
+    Event ev0, ev1;
+    hipEvent_t e0 = ev0.e, e1 = ev1.e;
     DECO_HIP(hipEventRecord(e0, stream()));
     for (int k = 0; k < iters; ++k)
       hipLaunchKernelGGL(k_read_stream, dim3(G), dim3(256), 0, stream(), (const f64x2*)buf.p, n2, part.p);
@@ -1137,14 +1157,14 @@ int deco_read_bandwidth(size_t bytes, int iters, double* gbps) {
     float ms = 0.f;
     DECO_HIP(hipEventElapsedTime(&ms, e0, e1));
     *gbps = (double)(n2 * 16) * iters / ((double)ms * 1e-3) / 1e9;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
   });
 }
 
 int deco_spmv_block(deco_csr* A, int R, int C, const double* x, double* y) {
   return guard([&] {
     if (!A) fail(DECO_ERR_ARG, "null CSR");
+    need(x, A->m, "x");
+    need(y, A->n, "y");
     const deco_bsr* B = A->block64(R, C);
     if (!B) fail(DECO_ERR_ARG, "%d x %d blocks do not tile a %d x %d matrix", R, C, A->n, A->m);
     bsr_stream_d(B, x, EpiOuter{y});
@@ -1155,12 +1175,12 @@ int deco_spmv_block(deco_csr* A, int R, int C, const double* x, double* y) {
 int deco_spmv_block_time(deco_csr* A, int R, int C, int precision, int iters, double* avg_ms, int* nnzb) {
   return guard([&] {
     if (!A || !avg_ms || iters <= 0) fail(DECO_ERR_ARG, "bad argument");
+    if (precision != DECO_PREC_FP64 && precision != DECO_PREC_FP32) fail(DECO_ERR_ARG, "bad precision %d", precision);
     const deco_bsr* B = precision == DECO_PREC_FP32 ? A->block32(R, C) : A->block64(R, C);
     if (!B) fail(DECO_ERR_ARG, "%d x %d blocks do not tile a %d x %d matrix", R, C, A->n, A->m);
     if (nnzb) *nnzb = B->nnzb;
-    hipEvent_t e0, e1;
-    DECO_HIP(hipEventCreate(&e0));
-    DECO_HIP(hipEventCreate(&e1));
+    Event ev0, ev1;
+    hipEvent_t e0 = ev0.e, e1 = ev1.e;
     if (precision == DECO_PREC_FP32) {
       DBuf<float> x(A->m), y(A->n);
       x.zero();
@@ -1180,17 +1200,15 @@ int deco_spmv_block_time(deco_csr* A, int R, int C, int precision, int iters, do
     float ms = 0.f;
     DECO_HIP(hipEventElapsedTime(&ms, e0, e1));
     *avg_ms = (double)ms / iters;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
   });
 }
 
 int deco_spmv_time(deco_csr* A, int precision, int iters, double* avg_ms) {
   return guard([&] {
     if (!A || !avg_ms || iters <= 0) fail(DECO_ERR_ARG, "bad argument");
-    hipEvent_t e0, e1;
-    DECO_HIP(hipEventCreate(&e0));
-    DECO_HIP(hipEventCreate(&e1));
+    if (precision != DECO_PREC_FP64 && precision != DECO_PREC_FP32) fail(DECO_ERR_ARG, "bad precision %d", precision);
+    Event ev0, ev1;
+    hipEvent_t e0 = ev0.e, e1 = ev1.e;
     if (precision == DECO_PREC_FP32) {
       const float* vf = A->values_f32();
       DBuf<float> x(A->m), y(A->n);
@@ -1211,8 +1229,6 @@ int deco_spmv_time(deco_csr* A, int precision, int iters, double* avg_ms) {
     float ms = 0.f;
     DECO_HIP(hipEventElapsedTime(&ms, e0, e1));
     *avg_ms = (double)ms / iters;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
   });
 }
 

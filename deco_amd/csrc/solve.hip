@@ -535,11 +535,7 @@ static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, cons
     } else { A.apply(uu, z.p); M.apply(z.p, w.p); }
     ++napp; ++nspmv;
   };
-  struct Ev {
-    hipEvent_t e = nullptr;
-    Ev() { DECO_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming)); }
-    ~Ev() { (void)hipEventDestroy(e); }
-  } ev;
+  Event ev(hipEventDisableTiming);
   while (iter < itmax) {
     std::fill(Hr.begin(), Hr.end(), 0.0);
     std::fill(Rm.begin(), Rm.end(), 0.0);
@@ -825,9 +821,8 @@ static void release_workspace() {
 
 static void run_solver(const deco_sol_params& sp, const AOp& A, const MOp& M, const double* b, double* x,
                        deco_sol_info& info) {
-  hipEvent_t e0, e1;
-  DECO_HIP(hipEventCreate(&e0));
-  DECO_HIP(hipEventCreate(&e1));
+  Event ev0, ev1;
+  hipEvent_t e0 = ev0.e, e1 = ev1.e;
   DECO_HIP(hipEventRecord(e0, stream()));
   switch (sp.type) {
     case DECO_SOL_RGMRES:
@@ -848,8 +843,6 @@ static void run_solver(const deco_sol_params& sp, const AOp& A, const MOp& M, co
   float ms = 0.f;
   DECO_HIP(hipEventElapsedTime(&ms, e0, e1));
   info.time_ms = ms;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   if (sp.verbosity > 0) {
     printf("\n--------------------------------\n");
     printf("Profile of %s:\n", sp.type == DECO_SOL_RGMRES ? "rGMRES" : sp.type == DECO_SOL_LGMRES ? "lGMRES"
@@ -1015,6 +1008,8 @@ int deco_prec_destroy(deco_prec* M) {
 int deco_prec_apply(deco_prec* M, const double* r, double* x) {
   return guard([&] {
     if (!M) fail(DECO_ERR_ARG, "null preconditioner");
+    need(r, M->n, "r");
+    need(x, M->n, "x");
     M->apply(r, x);
     sync();
   });
@@ -1023,6 +1018,8 @@ int deco_prec_apply(deco_prec* M, const double* r, double* x) {
 int deco_prec_apply_transpose(deco_prec* M, const double* r, double* x) {
   return guard([&] {
     if (!M) fail(DECO_ERR_ARG, "null preconditioner");
+    need(r, M->n, "r");
+    need(x, M->n, "x");
     M->apply_t(r, x);
     sync();
   });
@@ -1087,6 +1084,9 @@ int deco_solve_transpose(const deco_sol_params* sp, deco_csr* A, deco_prec* M, c
 int deco_grad_A_pattern(const deco_csr* A, const double* lam, const double* x, double* dA) {
   return guard([&] {
     if (!A) fail(DECO_ERR_ARG, "null CSR");
+    need(lam, A->n, "lambda");
+    need(x, A->m, "x");
+    need(dA, A->nnz, "dA");
     if (A->n == 0) return;
     hipLaunchKernelGGL(k_grad_A, dim3(grid_for(A->n, 256)), dim3(256), 0, stream(), A->n, A->r.p, A->c.p, lam, x, dA);
     DECO_CHECK_LAUNCH();

@@ -107,6 +107,27 @@ def test_empty_and_single_row_edges(gpu):
     assert E.transpose().nz == 0
 
 
+def test_null_device_arrays_are_rejected(gpu):
+    """A NULL device array for a non-empty operand returns DECO_ERR_ARG (no kernel runs on it);
+    empty operands may be NULL."""
+    import ctypes
+    A = deco_amd.CSR(3, 3, [0, 1, 2, 3], [0, 1, 2], [1.0, 2.0, 3.0])
+    x, y = deco_amd.DeviceArray.from_numpy(np.ones(3)), deco_amd.DeviceArray(3)
+    lib = deco_amd.lib()
+    null = None
+    assert lib.deco_spmv(A.h, null, y.ptr, 1) == -1
+    assert lib.deco_spmv(A.h, x.ptr, null, 1) == -1
+    assert lib.deco_spmv_residual(A.h, x.ptr, null, y.ptr) == -1
+    assert lib.deco_spmv_block(A.h, 1, 1, null, y.ptr) == -1
+    assert lib.deco_dot(3, x.ptr, null, ctypes.byref(ctypes.c_double())) == -1
+    assert lib.deco_axpby(3, 1.0, null, 1.0, y.ptr) == -1
+    assert lib.deco_memcpy_d2d(y.ptr, null, 24) == -1
+    assert lib.deco_grad_A_pattern(A.h, x.ptr, x.ptr, null) == -1
+    assert lib.deco_spmv_time(A.h, 5, 1, ctypes.byref(ctypes.c_double())) == -1
+    assert lib.deco_dot(0, null, null, ctypes.byref(ctypes.c_double())) == 0
+    assert lib.deco_spmv(A.h, x.ptr, y.ptr, 1) == 0 and np.array_equal(y.numpy(), [1.0, 2.0, 3.0])
+
+
 @pytest.mark.parametrize("cfg,bs,ro,co", [(2, 3, "213", "213"), (2, 3, "312", "132"), (1, 2, "21", "21"),
                                           (1, 2, "12", "12")])
 def test_block_reorder_bitexact(gpu, cfg, bs, ro, co):
