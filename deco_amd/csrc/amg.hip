@@ -270,7 +270,14 @@ struct AMG : deco_prec {
         H.r.resize(Al->n + 1);
         H.c.resize(Al->nnz);
         H.v.resize(Al->nnz);
+        {
+          ProfPhase ph_w("queued device work");   // decoupling and FSAI kernels still running
+          sync();
+        }
         ProfPhase ph_dl("download");
+        amg::prefault(H.r.data(), sizeof(int) * H.r.size());
+        amg::prefault(H.c.data(), sizeof(int) * H.c.size());
+        amg::prefault(H.v.data(), sizeof(double) * H.v.size());
         Al->r.download(H.r.data(), Al->n + 1);
         Al->c.download(H.c.data(), Al->nnz);
         Al->v.download(H.v.data(), Al->nnz);

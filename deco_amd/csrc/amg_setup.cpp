@@ -18,6 +18,13 @@ static inline uint64_t mix64(uint64_t x) {
   return x ^ (x >> 31);
 }
 
+void prefault(void* p, size_t bytes) {
+  char* b = static_cast<char*>(p);
+  const int64_t pages = (int64_t)((bytes + 4095) / 4096);
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < pages; ++k) b[k * 4096] = 0;
+}
+
 void strength(const HostCSR& A, double theta, double max_row_sum, std::vector<char>& strong) {
   strong.assign(A.c.size(), 0);
 #pragma omp parallel for schedule(static)

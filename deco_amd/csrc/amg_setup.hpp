@@ -32,6 +32,10 @@ struct HostCSR {
   hvec<double> v;
 };
 
+// Touches every page of [p, p + bytes) from all threads, so a copy into fresh (never written)
+// host memory does not take its page faults one at a time.
+void prefault(void* p, size_t bytes);
+
 // Classical strength of connection with hypre's conventions (the sign of the diagonal decides
 // which off-diagonals can be strong; rows whose |row sum / diagonal| exceeds max_row_sum have
 // only weak connections). strong[k] is 1 for a strong off-diagonal entry k of A.
