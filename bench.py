@@ -218,7 +218,7 @@ def main():
     ap.add_argument("--cpu-reps", type=int, default=3, help="repetitions of the whole CPU solve (median)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (every GPU box here has one GPU); "
+        print("bench.py: the solve path runs on one GPU (the multi-GPU row partition is not implemented); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
         return 2
     if args.cfg not in WORKLOADS:

@@ -331,7 +331,7 @@ class CPR : public CPRBase<V> {
     bd.ensure(n);
     if (b) deco_facade::check(deco_memcpy_h2d(bd.p, b, sizeof(V) * n), "h2d");
     deco_facade::check(deco_cpr_create(&prm, A->device(), b ? bd.p : nullptr, &this->handle), "CPR");
-    A->pull();   // the host arrays see the decoupled system, as in the reference (Q1)
+    A->pull();   // the host arrays see the decoupled system: the reference decouples A in place (CPR.hpp:91-114)
     if (b) deco_facade::check(deco_memcpy_d2h(b, bd.p, sizeof(V) * n), "d2h");
     const deco_csr* ap = nullptr;
     deco_facade::check(deco_cpr_pressure_matrix(this->handle, &ap), "pressure matrix");

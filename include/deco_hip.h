@@ -34,7 +34,7 @@ extern "C" {
 #define DECO_ERR_DEVICE      -2   /* HIP runtime failure                                           */
 #define DECO_ERR_SINGULAR    -3   /* zero determinant / zero pivot (CPR.hpp:416-419,643-646)       */
 #define DECO_ERR_BREAKDOWN   -4   /* Krylov breakdown / zero pivot in back-substitution            */
-#define DECO_ERR_UNSUPPORTED -5   /* parameter value this build does not implement                 */
+#define DECO_ERR_UNSUPPORTED -5   /* parameter value the library does not implement                */
 #define DECO_ERR_NOMEM       -6   /* device allocation failed                                      */
 #define DECO_ERR_STATE       -7   /* handle used in a state that does not allow the call           */
 
@@ -53,7 +53,8 @@ typedef struct deco_gen  deco_gen;   /* host-side synthetic reservoir system    
 #define DECO_GLO_FSAI   0
 #define DECO_GLO_JACOBI 1
 
-/* Precision of the preconditioner (north-star mixed mode; the Krylov part is always FP64). */
+/* Precision of the preconditioner (FP32 preconditioner inside FP64 Krylov iterations; the
+ * Krylov part is always FP64). */
 #define DECO_PREC_FP64  0
 #define DECO_PREC_FP32  1
 
@@ -77,7 +78,8 @@ typedef struct deco_gen  deco_gen;   /* host-side synthetic reservoir system    
  * line of each group are extensions with documented defaults. */
 typedef struct deco_cpr_params {
   int    block_size;        /* prec.block_size                                  */
-  int    pressure_id;       /* prec.pressure_id (must be 0, see DESIGN quirk Q2) */
+  int    pressure_id;       /* prec.pressure_id (must be 0: the strided products take
+                               rows bs*i, as alg.hpp:55,62 and CPR.hpp:241 do)     */
   int    seq;               /* prec.seq: 0..3                                   */
   int    weights;           /* DECO_W_*                                         */
   int    use_drs;           /* prec.use_drs                                     */

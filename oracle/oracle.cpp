@@ -649,7 +649,7 @@ struct AMG : Prec {
 // CPR (precond/CPR.hpp)
 struct CPR : Prec {
   deco_cpr_params p{};
-  CSR* A = nullptr;          // the caller's matrix, decoupled in place (Q1)
+  CSR* A = nullptr;          // the caller's matrix, decoupled in place (CPR.hpp:91-114)
   CSR Ap;
   int bs = 1, sn = 0, pid = 0;
   std::unique_ptr<Prec> glo, loc, jac;
@@ -756,7 +756,7 @@ static std::vector<double> quasi_impes(const CSR& A, int bs) {
   return rhs;
 }
 
-// drs_weights (precond/CPR.hpp:518-584); signed diagonal compare (Q8)
+// drs_weights (precond/CPR.hpp:518-584); the diagonal is compared signed (CPR.hpp:545)
 static std::vector<char> drs_weights(const CSR& A, int bs, double e_dd, double e_ps) {
   const int sn = A.n / bs;
   std::vector<char> w(A.n, 1);
@@ -977,14 +977,14 @@ void CPR::apply(const double* r, double* x) {
     return;
   }
   if (seq == 2) jac->apply(r, x); else glo->apply(r, x);      // :205 / :238 / :279
-  spmv(*A, x, rt.data(), bs);                                  // strided rows (Q2)
+  spmv(*A, x, rt.data(), bs);                                  // rows bs*i: pressure rows as pid = 0
   for (int i = 0; i < sn; ++i) rt[i] = r[i * bs + pid] - rt[i];
   loc->apply(rt.data(), xp.data());
   for (int i = 0; i < sn; ++i) x[i * bs + pid] += xp[i];
   if (seq == 0) return;
   spmv(*A, x, rt.data());
   for (int i = 0; i < n; ++i) rt[i] = r[i] - rt[i];
-  glo->apply(rt.data(), tmp1.data());                          // precond_glo(rt,rt) (Q9)
+  glo->apply(rt.data(), tmp1.data());   // the reference aliases in and out here (CPR.hpp:265)
   for (int i = 0; i < n; ++i) x[i] += tmp1[i];
 }
 
