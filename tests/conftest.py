@@ -55,3 +55,38 @@ def small_params(bs, **kw):
             node = node[kk]
         node[keys[-1]] = str(v)
     return deco_amd.PTree(prec)
+
+
+def diag_pos(g, i):
+    """Index of the diagonal entry of row i in the generated system's CSR arrays."""
+    import numpy as np
+    b, e = g["r"][i], g["r"][i + 1]
+    k = b + int(np.searchsorted(g["c"][b:e], i))
+    assert g["c"][k] == i
+    return k
+
+
+def zero_diag_system(bs):
+    """A small generated system with chosen zero entries that drive every branch of the CPR's
+    tiny-diagonal fix (precond/CPR.hpp:753-840): returns (system with the modified values,
+    {row: expected outcome}). bs 2: a zero pressure and a zero saturation diagonal. bs 3: a zero
+    pressure diagonal; a zero diagonal in row i+1 whose right neighbour is nonzero (rows i+1 and
+    i+2 swap); a zero diagonal in row i+1 whose neighbours (i+1, i+2) and (i+2, i+1) are zero too
+    (the diagonal becomes -1e-20 and is flagged)."""
+    g = dict(gen_system(0, custom=(4, 4, 3, bs)))
+    v = g["v"].copy()
+    if bs == 2:
+        cases = {3 * 2: "flag", 5 * 2 + 1: "flag"}
+        for i in cases:
+            v[diag_pos(g, i)] = 0.0
+    else:
+        cases = {2 * 3: "flag", 4 * 3 + 1: "swap", 7 * 3 + 1: "flag"}
+        v[diag_pos(g, 2 * 3)] = 0.0
+        k = diag_pos(g, 4 * 3 + 1)
+        assert v[k + 1] != 0.0
+        v[k] = 0.0
+        k = diag_pos(g, 7 * 3 + 1)
+        k2 = diag_pos(g, 7 * 3 + 2)
+        v[k] = v[k + 1] = v[k2 - 1] = 0.0
+    g["v"] = v
+    return g, cases

@@ -247,3 +247,29 @@ def test_fsai_product_layouts(gpu):
         assert rel(F.apply_t(x).numpy(), Fo.apply_t(x)) <= 1e-13
     with pytest.raises(deco_amd._lib.DecoError):
         F.set_layout(7)
+
+
+@pytest.mark.parametrize("bs", [2, 3])
+def test_cpr_zero_diagonals_bitexact(gpu, bs):
+    """Systems whose zero diagonals drive every branch of the tiny-diagonal fix (flagged values,
+    the 3-unknown row swap, the undo after setup): decoupled A and b, pressure matrix and left
+    transform bit-identical to the oracle's."""
+    from conftest import zero_diag_system
+    g, cases = zero_diag_system(bs)
+    tree = small_params(bs)
+    A = dev_csr(g)
+    b = deco_amd.DeviceArray.from_numpy(g["b"])
+    M = deco_amd.CPR(tree, A, b)
+    Mo_A = to_mat(g)
+    bo = g["b"].copy()
+    Mo = oracle.CPR(deco_amd.cpr_params(tree), Mo_A, bo)
+    r, c, v = A.download()
+    assert np.array_equal(r, Mo_A.r) and np.array_equal(c, Mo_A.c) and np.array_equal(v, Mo_A.v)
+    assert np.array_equal(b.numpy(), bo)
+    for i, what in cases.items():
+        if what == "flag":
+            k = r[i] + int(np.searchsorted(c[r[i]:r[i + 1]], i))
+            assert v[k] == 0.0
+    ok, vp = same_pattern(M.pressure_matrix(), Mo.pressure_matrix())
+    assert ok and np.array_equal(vp, Mo.pressure_matrix().v)
+    assert np.array_equal(M.left_transform(), Mo.left_transform(bs))

@@ -223,3 +223,41 @@ def test_cpr_gmres_with_exact_local_stage_recovers_xstar():
     rng = np.random.default_rng(5)
     u, v = rng.standard_normal(A.n), rng.standard_normal(A.n)
     assert np.isclose(M.apply(u) @ v, u @ M.apply_t(v), rtol=1e-10)
+
+
+@pytest.mark.parametrize("bs", [2, 3])
+def test_fix_diag_zero_diagonals(bs):
+    """The tiny-diagonal fix (precond/CPR.hpp:753-855): the swapped rows of a 3-unknown cell are
+    part of the left transform (W A0 = A_dec and W b0 = b_dec hold for every other entry), a
+    flagged diagonal is 0 again in the decoupled matrix after the setup (CPR.hpp:169,842-855),
+    and only the cells that carry a zero get either treatment."""
+    from conftest import zero_diag_system, diag_pos
+    g, cases = zero_diag_system(bs)
+    A = to_mat(g)
+    S0 = A.to_scipy().toarray()
+    b0 = g["b"].copy()
+    b = g["b"].copy()
+    M = oracle.CPR(deco_amd.cpr_params(small_params(bs, use_drs="1")), A, b)
+    S1 = A.to_scipy().toarray()
+    W = M.left_transform(bs)
+    flagged = [i for i, what in cases.items() if what == "flag"]
+    for i in flagged:
+        assert S1[i, i] == 0.0
+    for s in range(g["ncells"]):
+        Ws = W[s * bs * bs:(s + 1) * bs * bs].reshape(bs, bs)
+        rows = slice(s * bs, (s + 1) * bs)
+        D = Ws @ S0[rows]
+        mask = np.ones_like(D, bool)
+        for i in flagged:
+            if s * bs <= i < (s + 1) * bs:
+                mask[i - s * bs, i] = False
+        scale = np.abs(S1[rows]).max()
+        assert np.allclose(D[mask], S1[rows][mask], rtol=1e-12, atol=1e-12 * scale), s
+        assert np.allclose(Ws @ b0[rows], b[rows], rtol=1e-12, atol=1e-12 * np.abs(b[rows]).max()), s
+    # after the swap the saturation rows of the cell trade places, so their decoupled diagonals
+    # come from the nonzero pair (i+2, i+1), (i+1, i+2) (scaled by the weights): a missing swap
+    # would leave the zero there, the -1e-20 branch a value of that size
+    for i, what in cases.items():
+        if what == "swap":
+            for q in (i, i + 1):
+                assert abs(S1[q, q]) > 1e-8 * np.abs(S1[q]).max(), q
