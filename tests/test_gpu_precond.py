@@ -273,3 +273,37 @@ def test_cpr_zero_diagonals_bitexact(gpu, bs):
     ok, vp = same_pattern(M.pressure_matrix(), Mo.pressure_matrix())
     assert ok and np.array_equal(vp, Mo.pressure_matrix().v)
     assert np.array_equal(M.left_transform(), Mo.left_transform(bs))
+
+
+@pytest.mark.parametrize("cfg,opts", [
+    (4, {"weights": "none", "use_drs": "0"}),
+    (4, {"weights": "quasiimpes", "use_drs": "1", "drs.e_ps": "0.05"}),
+    (4, {"CPD": "0"}),
+    (4, {"tol_zero": "1e-3"}),
+    (2, {"weights": "quasiimpes", "use_drs": "0"}),
+    (2, {"weights": "none", "CPD": "0", "tol_zero": "1e-4"}),
+])
+def test_cpr_setup_options_bitexact(gpu, cfg, opts):
+    """The decoupling options beyond the default set (weights none / quasiimpes, DRS with its
+    e_ps test, no CPD, the tol_zero filter) in 2- and 3-unknown cells: decoupled A and b,
+    pressure matrix and left transform bit-identical to the oracle's, and the CPR apply on the
+    same hierarchy."""
+    g = gen_system(cfg)
+    bs = g["bs"]
+    tree = small_params(bs, **opts)
+    A = dev_csr(g)
+    b = deco_amd.DeviceArray.from_numpy(g["b"])
+    M = deco_amd.CPR(tree, A, b)
+    Mo_A = to_mat(g)
+    bo = g["b"].copy()
+    Mo = oracle.CPR(deco_amd.cpr_params(tree), Mo_A, bo)
+    r, c, v = A.download()
+    assert np.array_equal(r, Mo_A.r) and np.array_equal(c, Mo_A.c) and np.array_equal(v, Mo_A.v)
+    assert np.array_equal(b.numpy(), bo)
+    ok, vp = same_pattern(M.pressure_matrix(), Mo.pressure_matrix())
+    assert ok and np.array_equal(vp, Mo.pressure_matrix().v)
+    assert np.array_equal(M.left_transform(), Mo.left_transform(bs))
+    Go, _, _ = oracle_amg(M.amg(), 18)
+    Mo.set_local(Go)
+    x = np.random.default_rng(13).uniform(-1, 1, g["n"])
+    assert rel(M.apply(x).numpy(), Mo.apply(x)) <= 1e-12

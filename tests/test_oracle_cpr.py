@@ -47,6 +47,11 @@ def test_drs_weights_small_closed_case():
     assert list(w) == [True, False, True, False]
     w2 = oracle.drs_weights(A, 2, -10.0, 0.0)
     assert list(w2) == [True, True, True, True]
+    # the e_ps test (CPR.hpp:551-577): a cell's pressure row, summed over the columns of each
+    # unknown class without its diagonal, against e_ps * |a_ii|; rows 0 and 2 hold 0.2 in the
+    # class of unknown 1 against e_ps * 4, which clears the flags of rows 1 and 3 for e_ps 0.1
+    assert list(oracle.drs_weights(A, 2, -10.0, 0.1)) == [True, False, True, False]
+    assert list(oracle.drs_weights(A, 2, -10.0, 0.01)) == [True, True, True, True]
 
 
 @pytest.mark.parametrize("custom", [(4, 4, 3, 2), (4, 3, 3, 3)])
