@@ -215,3 +215,19 @@ def test_product_layouts_same_solve(gpu):
     for lay in (deco_amd.LAYOUT_CSR, deco_amd.LAYOUT_BLOCK):
         it, x32 = res[deco_amd.PREC_FP32, lay]
         assert abs(it - i0) <= 2 and rel(x32, x0) <= 1e-6
+
+
+@pytest.mark.parametrize("cfg", [2, 1])
+def test_bicgstab_fp32_preconditioner(gpu, cfg):
+    """BiCGSTAB with the FP32 preconditioner and FP64 inner products and residual (the black-oil
+    case with 3 unknowns per cell, and the 2-unknown case): reaches rtol 1e-8 within two
+    iterations of the FP64-only solve, solution within 1e-6 of it."""
+    g, A, b, M, Ao, bo, Mo = setup_pair(cfg, small_params(g_bs(cfg)))
+    sol = {"type": "bicgstab", "reltol": "1e-8", "restart": "30", "itmax": "300", "verbosity": "0"}
+    S64 = deco_amd.SOL(sol, A, M, b)
+    M.set_precision(deco_amd.PREC_FP32)
+    S32 = deco_amd.SOL(sol, A, M, b)
+    M.set_precision(deco_amd.PREC_FP64)
+    assert S64.flag == 0 and S32.flag == 0
+    assert S32.res_rel < 1e-8 and abs(S32.iter - S64.iter) <= 2
+    assert rel(S32.x.numpy(), S64.x.numpy()) <= 1e-6
