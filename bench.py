@@ -128,12 +128,14 @@ def cpu_model():
     return f"{os.cpu_count()} logical CPUs"
 
 
-def bsr_bytes(nb, mb, nnzb, R, C, sv, sx, sy, sliced=False):
-    # algorithmic bytes of the block product: block values + block columns + x + y, and the row
-    # structure the layout reads: block row pointers (chunked) or per-slice offsets and widths
-    # of 64 block rows (sliced; its padding blocks are not counted)
-    structure = 8 * ((nb + 63) // 64) if sliced else 4 * (nb + 1)
-    return nnzb * (R * C * sv + 4) + structure + sx * mb * C + sy * nb * R
+def bsr_bytes(nb, mb, nnzb, R, C, sv, sx, sy, layout=0):
+    # algorithmic bytes of the block product: block values + block columns (4 bytes, 2 for the
+    # 16-bit offsets of layout 2) + x + y, and the row structure the layout reads: block row
+    # pointers (chunked, layout 0) or per-slice offsets and widths of 64 block rows (sliced,
+    # layouts 1 and 2; their padding blocks are not counted)
+    structure = 8 * ((nb + 63) // 64) if layout else 4 * (nb + 1)
+    col = 2 if layout == 2 else 4
+    return nnzb * (R * C * sv + col) + structure + sx * mb * C + sy * nb * R
 
 
 def spmv_section(deco_amd, A, n, nnz, iters, bs):
@@ -153,7 +155,8 @@ def spmv_section(deco_amd, A, n, nnz, iters, bs):
         sliced, _, stored = A.block_info(bs, bs)
         bb64 = bsr_bytes(n // bs, n // bs, nnzb, bs, bs, 8, 8, 8, sliced)
         bb32 = bsr_bytes(n // bs, n // bs, nnzb, bs, bs, 4, 4, 4, sliced)
-        out.update({"block_layout": "sliced (SELL, 64 block rows per slice)" if sliced else "chunked block CSR",
+        out.update({"block_layout": ("chunked block CSR", "sliced (SELL, 64 block rows per slice)",
+                                     "sliced (SELL, 64 block rows per slice, 16-bit block-column offsets)")[sliced],
                     "block_padding": stored / max(nnzb, 1) - 1.0,
                     "blocks": nnzb, "block_fp64_us": msb64 * 1e3, "block_fp64_GBps": bb64 / (msb64 * 1e-3) / 1e9,
                     "block_fp32_us": msb32 * 1e3, "block_fp32_GBps": bb32 / (msb32 * 1e-3) / 1e9,
@@ -270,7 +273,9 @@ def main():
         result["roofline"] = {
             "bound": "hbm", "achieved": spmv["block_fp64_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": spmv["block_fp64_GBps"] / HBM_PEAK_GBS, "traffic": traffic_from_profiles("spmv_block_fp64", key),
-            "kernel": (f"k_sell_stream<double,{bs},{bs},double,EpiOuter>" if spmv["block_layout"].startswith("sliced")
+            "kernel": ((f"k_sell_stream<double,{bs},{bs},double,EpiOuter,"
+                        f"{'short' if '16-bit' in spmv['block_layout'] else 'int'}>")
+                       if spmv["block_layout"].startswith("sliced")
                        else f"k_bsr_stream<double,{bs},{bs},double,EpiOuter>")
                       + f" (FP64 {bs}x{bs} cell-block product with A, {spmv['block_layout']})",
             "bytes_per_launch": spmv["bytes_block_fp64"]}

@@ -104,6 +104,7 @@ SIGNATURES = {
     "deco_read_bandwidth": (i, [C.c_size_t, i, dp]),
     "deco_set_sell_padding": (i, [C.c_double]),
     "deco_block_info": (i, [vp, i, i, ip, ip, C.POINTER(C.c_longlong)]),
+    "deco_set_sell_offsets16": (i, [i]),
     "deco_spmv_block_time": (i, [vp, i, i, i, i, dp, ip]),
     "deco_spgemm": (i, [vp, vp, pp]),
     "deco_spgemm_tril_pattern": (i, [vp, pp]),

@@ -181,10 +181,11 @@ class CSR:
         return ms.value, nb.value
 
     def block_info(self, R, C_):
-        """(sliced, blocks, stored blocks) of the R x C_ block shadow (created if needed)."""
+        """(layout, blocks, stored blocks) of the R x C_ block shadow (created if needed); layout
+        0 chunked, 1 sliced, 2 sliced with 16-bit block-column offsets."""
         sl, nb, ent = C.c_int(), C.c_int(), C.c_longlong()
         check(lib().deco_block_info(self.h, int(R), int(C_), C.byref(sl), C.byref(nb), C.byref(ent)))
-        return bool(sl.value), nb.value, ent.value
+        return sl.value, nb.value, ent.value
 
     def copy(self):
         h = C.c_void_p()
@@ -569,6 +570,12 @@ def set_sell_padding(max_fraction):
     """Block shadows made from now on take the sliced layout when it pads the blocks by at most
     max_fraction (default 0.10); a negative value selects the chunked block CSR always."""
     check(lib().deco_set_sell_padding(float(max_fraction)))
+
+
+def set_sell_offsets16(enable):
+    """Sliced block shadows made from now on store their block columns as 16-bit offsets from
+    the block row when every offset fits (default), else (or when disabled) 32-bit columns."""
+    check(lib().deco_set_sell_offsets16(1 if enable else 0))
 
 
 # ---------------------------------------------------------------------------- profiler ---

@@ -660,6 +660,26 @@ __global__ void k_bsr_fill(int nb, int R, int C, const int* __restrict__ r, cons
 // Largest padding (fraction of the blocks) for which block shadows take the sliced layout;
 // negative: always the chunked one.
 static double g_sell_padding = 0.10;
+static bool g_sell_offsets16 = true;
+
+// 16-bit offsets of the sliced layout's block columns from their block rows (bcd zeroed
+// beforehand: padding entries and the lanes past the last block row keep 0); ovf is set when an
+// offset does not fit
+__global__ void k_sell_delta(int nb, const int* __restrict__ br, const int* __restrict__ soff,
+                             const int* __restrict__ bc, short* __restrict__ bcd, int* __restrict__ ovf) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= nb) return;
+  const int cnt = br[I + 1] - br[I];
+  size_t e = (size_t)soff[I / kSellRows] + (I % kSellRows);
+  for (int k = 0; k < cnt; ++k, e += kSellRows) {
+    const int d = bc[e] - I;
+    if (d < -32768 || d > 32767) {
+      atomicOr(ovf, 1);
+      return;
+    }
+    bcd[e] = (short)d;
+  }
+}
 
 // per slice of 64 block rows: its width (longest block row) and its entries (width * 64)
 __global__ void k_sell_width(int nb, int nslices, const int* __restrict__ cnt, int* __restrict__ ent,
@@ -749,6 +769,20 @@ const deco_bsr* deco_csr::block32(int R, int C) const {
       hipLaunchKernelGGL(k_sell_fill<float>, dim3(grid_for(nb, 256)), dim3(256), 0, stream(), nb, R, C, r.p, c.p,
                          v.p, (const int*)B->soff.p, B->stot, B->bc.p, B->bv.p);
       DECO_CHECK_LAUNCH();
+      if (g_sell_offsets16 && nb <= m / C) {   // an offset 0 then addresses a block column of x
+        const size_t ne = std::max<size_t>(B->stot, 1);
+        B->bcd.alloc(ne);
+        DECO_HIP(hipMemsetAsync(B->bcd.p, 0, sizeof(short) * ne, stream()));
+        DBuf<int> ovf(1);
+        ovf.zero();
+        hipLaunchKernelGGL(k_sell_delta, dim3(grid_for(nb, 256)), dim3(256), 0, stream(), nb, (const int*)B->br.p,
+                           (const int*)B->soff.p, (const int*)B->bc.p, B->bcd.p, ovf.p);
+        DECO_CHECK_LAUNCH();
+        int o = 0;
+        copy_d2h(&o, ovf.p, 1);
+        B->d16 = (o == 0);
+        if (!B->d16) B->bcd.release();
+      }
       b32 = B;
       return B;
     }
@@ -1121,7 +1155,7 @@ int deco_block_info(deco_csr* A, int R, int C, int* sliced, int* nnzb, long long
     if (!A) fail(DECO_ERR_ARG, "null CSR");
     const deco_bsr* B = A->block32(R, C);
     if (!B) fail(DECO_ERR_ARG, "%d x %d blocks do not tile a %d x %d matrix", R, C, A->n, A->m);
-    if (sliced) *sliced = B->sell ? 1 : 0;
+    if (sliced) *sliced = B->sell ? (B->d16 ? 2 : 1) : 0;
     if (nnzb) *nnzb = B->nnzb;
     if (entries) *entries = B->sell ? (long long)B->stot : (long long)B->nnzb;
   });
@@ -1132,6 +1166,10 @@ int deco_set_sell_padding(double max_fraction) {
     if (!(max_fraction == max_fraction)) fail(DECO_ERR_ARG, "padding fraction is NaN");
     g_sell_padding = max_fraction;
   });
+}
+
+int deco_set_sell_offsets16(int enable) {
+  return guard([&] { g_sell_offsets16 = enable != 0; });
 }
 
 int deco_read_bandwidth(size_t bytes, int iters, double* gbps) {

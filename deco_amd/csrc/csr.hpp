@@ -25,6 +25,11 @@ struct deco_bsr {
   int nslices = 0;
   size_t stot = 0;             // entries of the sliced layout, padding included
   deco::DBuf<int> soff, swid;  // per slice: first entry, width (blocks per block row)
+  // sliced layout: each entry's block column as a 16-bit offset from its block row (padding and
+  // lanes past the last block row: 0), read instead of bc when every offset fits and the block
+  // rows do not outnumber the block columns (so an offset 0 stays inside x)
+  deco::DBuf<short> bcd;
+  bool d16 = false;
 };
 
 struct deco_csr {
@@ -524,9 +529,17 @@ __device__ __forceinline__ BsrVals<VT, R * C> sell_load(const VT* __restrict__ b
   return o;
 }
 
-template <class VT, int R, int C, class XT, class E, int KU = 4>
+// block column of sliced entry e: stored (int) or as an offset from the lane's block row base
+// (short; base 0 for the lanes past the last block row, whose entries hold offset 0)
+template <class CT>
+__device__ __forceinline__ int sell_col(const CT* __restrict__ bc, size_t e, int base) {
+  if constexpr (std::is_same<CT, short>::value) return base + (int)__builtin_nontemporal_load(bc + e);
+  else return __builtin_nontemporal_load(bc + e);
+}
+
+template <class VT, int R, int C, class XT, class E, class CT = int, int KU = 4>
 __global__ __launch_bounds__(256) void k_sell_stream(int nb, int nslices, const int* __restrict__ soff,
-                                                    const int* __restrict__ swid, const int* __restrict__ bc,
+                                                    const int* __restrict__ swid, const CT* __restrict__ bc,
                                                     const VT* __restrict__ bv, size_t T, const XT* __restrict__ x,
                                                     E epi) {
   using AT = acc_t<VT, XT>;
@@ -535,6 +548,7 @@ __global__ __launch_bounds__(256) void k_sell_stream(int nb, int nslices, const 
   const int lane = threadIdx.x & 63;
   const int I = sl * kSellRows + lane;
   const bool live = I < nb;
+  const int base = live ? I : 0;
   typename E::Pre pre[R];
   if (live) {
 #pragma unroll
@@ -552,7 +566,7 @@ __global__ __launch_bounds__(256) void k_sell_stream(int nb, int nslices, const 
 #pragma unroll
     for (int u = 0; u < KU; ++u) {
       const size_t e = (size_t)o + (size_t)(k + u) * kSellRows + lane;
-      cc[u] = __builtin_nontemporal_load(bc + e);
+      cc[u] = sell_col(bc, e, base);
       vv[u] = sell_load<VT, R, C>(bv, e, T);
     }
 #pragma unroll
@@ -571,7 +585,7 @@ __global__ __launch_bounds__(256) void k_sell_stream(int nb, int nslices, const 
     const size_t e = (size_t)o + (size_t)k * kSellRows + lane;
     const BsrVals<VT, R * C> vv = sell_load<VT, R, C>(bv, e, T);
     XT xv[C];
-    bsr_gather<C>(x, bc[e], xv);
+    bsr_gather<C>(x, sell_col(bc, e, base), xv);
 #pragma unroll
     for (int rr = 0; rr < R; ++rr) {
       AT p = (AT)vv.a[rr * C] * (AT)xv[0];
@@ -591,9 +605,15 @@ void bsr_stream_v(const deco_bsr* B, const VT* bv, const XT* x, E epi) {
   if (B->nb <= 0) return;
   if (B->sell) {
     const unsigned grid = (unsigned)((B->nslices + 3) / 4);
-#define DECO_SELL_LAUNCH(RR, CC)                                                                                 \
-  hipLaunchKernelGGL((k_sell_stream<VT, RR, CC, XT, E>), dim3(grid), dim3(256), 0, stream(), B->nb, B->nslices, \
-                     B->soff.p, B->swid.p, B->bc.p, bv, B->stot, x, epi)
+#define DECO_SELL_LAUNCH(RR, CC)                                                                             \
+  do {                                                                                                       \
+    if (B->d16)                                                                                              \
+      hipLaunchKernelGGL((k_sell_stream<VT, RR, CC, XT, E, short>), dim3(grid), dim3(256), 0, stream(), B->nb, \
+                         B->nslices, B->soff.p, B->swid.p, (const short*)B->bcd.p, bv, B->stot, x, epi);      \
+    else                                                                                                     \
+      hipLaunchKernelGGL((k_sell_stream<VT, RR, CC, XT, E, int>), dim3(grid), dim3(256), 0, stream(), B->nb,   \
+                         B->nslices, B->soff.p, B->swid.p, (const int*)B->bc.p, bv, B->stot, x, epi);         \
+  } while (0)
     if (B->R == 2 && B->C == 2) DECO_SELL_LAUNCH(2, 2);
     else if (B->R == 1 && B->C == 1) DECO_SELL_LAUNCH(1, 1);
     else if (B->R == 1 && B->C == 2) DECO_SELL_LAUNCH(1, 2);

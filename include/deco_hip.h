@@ -212,8 +212,13 @@ int deco_read_bandwidth(size_t bytes, int iters, double* gbps);
  * chunked block CSR otherwise; a negative value selects the chunked layout always. No reference
  * counterpart (a layout choice of this library). */
 int deco_set_sell_padding(double max_fraction);
-/* The R x C block shadow of A (created if needed): sliced (1) or chunked (0) layout, its number of
- * blocks, and the blocks it stores (padding included). No reference counterpart. */
+/* Whether sliced block shadows made from now on store their block columns as 16-bit offsets
+ * from the block row (default 1; taken when every offset fits and the block rows do not
+ * outnumber the block columns, else 32-bit columns). No reference counterpart. */
+int deco_set_sell_offsets16(int enable);
+/* The R x C block shadow of A (created if needed): its layout (0 chunked, 1 sliced, 2 sliced with
+ * 16-bit block-column offsets), its number of blocks, and the blocks it stores (padding
+ * included). No reference counterpart. */
 int deco_block_info(deco_csr* A, int R, int C, int* sliced, int* nnzb, long long* entries);
 /* deco_spmv_time for the block-shadow product (precision as there); nnzb (may be NULL)
  * receives the number of R x C blocks. */

@@ -144,21 +144,23 @@ def test_block_reorder_bitexact(gpu, cfg, bs, ro, co):
         A.block_reorder(bs, "1" * bs, co)
 
 
-@pytest.mark.parametrize("pad", [10.0, -1.0])
+@pytest.mark.parametrize("pad,off16", [(10.0, True), (10.0, False), (-1.0, True)])
 @pytest.mark.parametrize("cfg,shape", [(1, (2, 2)), (2, (3, 3)), (1, "prow"), (1, "tril"), (2, "tril")])
-def test_block_f32_product(gpu, cfg, shape, pad):
+def test_block_f32_product(gpu, cfg, shape, pad, off16):
     """Block-shadow products against the CSR product: full cell blocks, the pressure rows
     (1 x bs blocks), and lower-triangular patterns whose diagonal blocks are partly empty (the
-    FSAI factor shape); in the sliced layout (pad 10: chosen whatever its padding) and the
-    chunked one (pad -1)."""
+    FSAI factor shape); in the sliced layout (pad 10: chosen whatever its padding) with 16-bit
+    block-column offsets and with 32-bit columns, and in the chunked one (pad -1)."""
     deco_amd.set_sell_padding(pad)
+    deco_amd.set_sell_offsets16(off16)
     try:
-        _block_products(cfg, shape)
+        _block_products(cfg, shape, 0 if pad < 0 else (2 if off16 else 1))
     finally:
         deco_amd.set_sell_padding(0.10)
+        deco_amd.set_sell_offsets16(True)
 
 
-def _block_products(cfg, shape):
+def _block_products(cfg, shape, layout):
     g = gen_system(cfg)
     bs = g["bs"]
     S = to_mat(g).to_scipy()
@@ -172,6 +174,11 @@ def _block_products(cfg, shape):
         R, C = shape
     S.sort_indices()
     A = deco_amd.CSR.from_scipy(S)
+    if layout == 2:   # 16-bit offsets only where every block column lies within reach
+        rows = np.repeat(np.arange(S.shape[0]), np.diff(S.indptr)) // R
+        fits = np.abs(S.indices // C - rows).max() <= 32767 and S.shape[0] // R <= S.shape[1] // C
+        layout = 2 if fits else 1
+    assert A.block_info(R, C)[0] == layout
     x = np.random.default_rng(3).uniform(-1, 1, S.shape[1]).astype(np.float32)
     y = A.apply_f32(x, block=(R, C)).numpy().astype(np.float64)
     ref = S @ x.astype(np.float64)
@@ -209,3 +216,37 @@ def test_spmv_wide_column_offsets(gpu):
         assert np.array_equal(A.apply(x).numpy(), oracle.spmv(M, x))
         y32 = A.apply_f32(x.astype(np.float32)).numpy().astype(np.float64)
         assert rel(y32, S @ x) <= 2e-6
+
+
+def test_sliced_offsets_fall_back_to_32bit_columns(gpu):
+    """The 16-bit block-column offsets are used only where they are exact: a block column more
+    than 32767 block rows away, or more block rows than block columns (an offset 0 would leave
+    x), keep 32-bit columns (layout 1); both products stay exact."""
+    rng = np.random.default_rng(17)
+    n = 40000
+    i = np.arange(n)
+    rows = np.concatenate([i, i[:-1], [5]])
+    cols = np.concatenate([i, i[:-1] + 1, [n - 1]])   # row 5 reaches 39994 block rows away
+    S = sp.csr_matrix((rng.uniform(-1, 1, len(rows)), (rows, cols)), shape=(n, n))
+    S.sort_indices()
+    A = deco_amd.CSR.from_scipy(S)
+    assert A.block_info(1, 1)[0] == 1
+    x = rng.uniform(-1, 1, n)
+    assert rel(A.apply(x, block=(1, 1)).numpy(), oracle.spmv(oracle.Mat.from_scipy(S), x)) <= 1e-14
+    near = S.copy()
+    near.data[near.indices == n - 1] = 0.0
+    near.eliminate_zeros()
+    near = (near + sp.csr_matrix(([1.0], ([n - 1], [n - 1])), shape=(n, n))).tocsr()
+    near.sort_indices()
+    assert deco_amd.CSR.from_scipy(near).block_info(1, 1)[0] == 2
+    # 600 block rows over 300 block columns (sliced whatever its padding)
+    W = (sp.random(600, 300, density=0.02, random_state=rng, format="csr") + sp.eye(600, 300, format="csr")).tocsr()
+    W.sort_indices()
+    deco_amd.set_sell_padding(10.0)
+    try:
+        B = deco_amd.CSR.from_scipy(W)
+        assert B.block_info(1, 1)[0] == 1
+        xw = rng.uniform(-1, 1, 300)
+        assert rel(B.apply(xw, block=(1, 1)).numpy(), oracle.spmv(oracle.Mat.from_scipy(W), xw)) <= 1e-14
+    finally:
+        deco_amd.set_sell_padding(0.10)
