@@ -27,7 +27,8 @@ template <typename R, typename C, typename V>
 std::shared_ptr<CSR<R, C, V>> cpp_read_binary_matrix(const std::string& filename) {
   std::ifstream f(filename, std::ios::binary);
   if (!f) throw std::runtime_error("Could not open file " + filename);
-  R n, m, nz;
+  C n = -1, m = -1;
+  R nz = -1;
   f.read(reinterpret_cast<char*>(&n), sizeof(C));
   f.read(reinterpret_cast<char*>(&m), sizeof(C));
   f.read(reinterpret_cast<char*>(&nz), sizeof(R));
@@ -62,12 +63,21 @@ V* cpp_read_binary_rhs(const std::string& filename, std::shared_ptr<CSR<R, C, V>
     return v;
   }
   std::ifstream f(filename, std::ios::binary);
-  if (!f) throw std::runtime_error("Could not open file " + filename);
-  C nn;
+  if (!f) {
+    std::free(v);
+    throw std::runtime_error("Could not open file " + filename);
+  }
+  C nn = -1;
   f.read(reinterpret_cast<char*>(&nn), sizeof(C));
-  if (nn != n) throw std::runtime_error("rhs length does not match the matrix in " + filename);
+  if (nn != n) {
+    std::free(v);
+    throw std::runtime_error("rhs length does not match the matrix in " + filename);
+  }
   f.read(reinterpret_cast<char*>(v), sizeof(V) * n);
-  if (!f) throw std::runtime_error("truncated rhs file " + filename);
+  if (!f) {
+    std::free(v);
+    throw std::runtime_error("truncated rhs file " + filename);
+  }
   return v;
 }
 

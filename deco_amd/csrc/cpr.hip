@@ -33,7 +33,19 @@ void fsai_apply32_t(FSAI* F, const double* r, float* x);
 void fsai_apply32_t_addz(FSAI* F, const float* r, const float* z, double* x);
 void fsai_apply32_t_addz(FSAI* F, const float* r, const float* z, float* x);
 
-enum CprErr { ERR_DET = 1, ERR_ROWLEN = 2 };
+enum CprErr { ERR_DET = 1, ERR_ROWLEN = 2, ERR_NODIAG = 4 };
+
+// every row holds its diagonal entry (the decoupling reads it by binary search in the row)
+__global__ void k_diag_present(int n, const int* __restrict__ r, const int* __restrict__ c, int* __restrict__ err) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int lo = r[i], hi = r[i + 1];
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (c[mid] < i) lo = mid + 1; else hi = mid;
+  }
+  if (lo == r[i + 1] || c[lo] != i) atomicOr(err, ERR_NODIAG);
+}
 
 __device__ __forceinline__ int bsearch_dev(const int* __restrict__ c, int lower, int upper, int target) {
   int mn = lower, mx = upper, mid = (lower + upper) >> 1;  // core::binarysearch (core/alg.hpp:16-25)
@@ -440,6 +452,13 @@ struct CPR : deco_prec {
     swapped.zero();
     err.zero();
     ProfPhase ph_dec("decoupling");                                    // CPR.hpp:47-138
+    hipLaunchKernelGGL(k_diag_present, dim3(gr), dim3(256), 0, stream(), n, A->r.p, A->c.p, err.p);
+    DECO_CHECK_LAUNCH();
+    {
+      int e = 0;
+      copy_d2h(&e, err.p, 1);
+      if (e & ERR_NODIAG) fail(DECO_ERR_ARG, "a row of A has no diagonal entry (CPR needs every diagonal)");
+    }
     hipLaunchKernelGGL(k_fix_diag, dim3(gc), dim3(128), 0, stream(), sn, bs, A->r.p, A->c.p, A->v.p, b, flag.p,
                        swapped.p, err.p + 1);
     DECO_CHECK_LAUNCH();

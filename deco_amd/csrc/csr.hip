@@ -968,6 +968,43 @@ static void check_csr_args(int n, int m, int nnz) {
   if (n < 0 || m < 0 || nnz < 0) fail(DECO_ERR_ARG, "negative CSR size n=%d m=%d nnz=%d", n, m, nnz);
 }
 
+namespace deco {
+// structure check of an uploaded CSR: err |= 1 for a row pointer that decreases, leaves
+// [0, nnz] or does not span it, |= 2 for a column outside [0, m) (the loops stay inside [0, nnz)
+// either way)
+__global__ void k_csr_check(int n, int m, int nnz, const int* __restrict__ r, const int* __restrict__ c,
+                            int* __restrict__ err) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (i == 0 && (r[0] != 0 || r[n] != nnz)) atomicOr(err, 1);
+  const int b = r[i], e = r[i + 1];
+  if (b < 0 || e < b || e > nnz) {
+    atomicOr(err, 1);
+    return;
+  }
+  for (int k = b; k < e; ++k)
+    if (c[k] < 0 || c[k] >= m) {
+      atomicOr(err, 2);
+      return;
+    }
+}
+}  // namespace deco
+
+// rejects a structure the kernels cannot address safely (they index x by the columns and walk
+// the rows by the pointers)
+static void check_csr_structure(const deco_csr* A) {
+  if (A->n == 0) return;
+  DBuf<int> err(1);
+  err.zero();
+  hipLaunchKernelGGL(k_csr_check, dim3(grid_for(A->n, 256)), dim3(256), 0, stream(), A->n, A->m, A->nnz, A->r.p,
+                     A->c.p, err.p);
+  DECO_CHECK_LAUNCH();
+  int e = 0;
+  copy_d2h(&e, err.p, 1);
+  if (e & 1) fail(DECO_ERR_ARG, "row pointer decreases or leaves [0, nnz]");
+  if (e & 2) fail(DECO_ERR_ARG, "column index outside [0, %d)", A->m);
+}
+
 int deco_csr_create(int n, int m, int nnz, const int* r, const int* c, const double* v, deco_csr** out) {
   return guard([&] {
     check_csr_args(n, m, nnz);
@@ -978,6 +1015,12 @@ int deco_csr_create(int n, int m, int nnz, const int* r, const int* c, const dou
     A->r.upload(r, n + 1);
     A->c.upload(c, nnz);
     A->v.upload(v, nnz);
+    try {
+      check_csr_structure(A);
+    } catch (...) {
+      delete A;
+      throw;
+    }
     sync();
     *out = A;
   });
@@ -998,6 +1041,12 @@ int deco_csr_create_dev(int n, int m, int nnz, const int* r, const int* c, const
     copy_d2d(A->r.p, r, n + 1);
     copy_d2d(A->c.p, c, nnz);
     copy_d2d(A->v.p, v, nnz);
+    try {
+      check_csr_structure(A);
+    } catch (...) {
+      delete A;
+      throw;
+    }
     sync();
     *out = A;
   });

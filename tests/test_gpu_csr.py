@@ -250,3 +250,33 @@ def test_sliced_offsets_fall_back_to_32bit_columns(gpu):
         assert rel(B.apply(xw, block=(1, 1)).numpy(), oracle.spmv(oracle.Mat.from_scipy(W), xw)) <= 1e-14
     finally:
         deco_amd.set_sell_padding(0.10)
+
+
+def test_invalid_csr_structures_are_rejected(gpu):
+    """A column outside [0, m), a decreasing row pointer and one that does not span [0, nnz] are
+    argument errors of deco_csr_create and deco_csr_create_dev (no kernel indexes with them);
+    the CPR setup rejects a row without its diagonal."""
+    import ctypes
+    for n, m, r, c, v in [(3, 3, [0, 1, 2, 3], [0, 3, 2], [1.0] * 3),
+                          (3, 3, [0, 2, 1, 3], [0, 1, 2], [1.0] * 3)]:
+        with pytest.raises(deco_amd._lib.DecoError):
+            deco_amd.CSR(n, m, r, c, v)
+        rd = deco_amd.DeviceArray.from_numpy(np.array(r, np.int32))
+        cd = deco_amd.DeviceArray.from_numpy(np.array(c, np.int32))
+        vd = deco_amd.DeviceArray.from_numpy(np.array(v, np.float64))
+        h = ctypes.c_void_p()
+        assert deco_amd.lib().deco_csr_create_dev(n, m, len(c), rd.ptr, cd.ptr, vd.ptr, ctypes.byref(h)) == -1
+    rd = deco_amd.DeviceArray.from_numpy(np.array([0, 1, 2, 2], np.int32))   # r[n] != nnz
+    cd = deco_amd.DeviceArray.from_numpy(np.array([0, 1, 2], np.int32))
+    vd = deco_amd.DeviceArray.from_numpy(np.ones(3))
+    h = ctypes.c_void_p()
+    assert deco_amd.lib().deco_csr_create_dev(3, 3, 3, rd.ptr, cd.ptr, vd.ptr, ctypes.byref(h)) == -1
+    g = gen_system(4)
+    S = to_mat(g).to_scipy().tolil()
+    S[5, 5] = 0.0
+    S = S.tocsr()
+    S.eliminate_zeros()
+    S.sort_indices()
+    from conftest import small_params
+    with pytest.raises(deco_amd._lib.DecoError):
+        deco_amd.CPR(small_params(2), deco_amd.CSR.from_scipy(S), None)
