@@ -55,8 +55,10 @@ struct Context {
   hipStream_t stream = nullptr;
   int num_cu = 0;
   size_t lds_per_block = 0;
-  // pinned host staging for scalar readbacks
+  // pinned host staging for scalar readbacks (coherent: kernels may write it through
+  // pinned_dev, its device address, and the host reads it after an event behind them)
   double* pinned = nullptr;
+  double* pinned_dev = nullptr;
   // library buffers come from the device's stream-ordered pool (allocation and release queued
   // on the library stream: no device-wide synchronisation, freed blocks reused)
   bool pooled = false;

@@ -46,7 +46,10 @@ static void init_device(int device) {
     g_ctx.pooled = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) == hipSuccess;
   }
   (void)hipGetLastError();
-  if (!g_ctx.pinned) DECO_HIP(hipHostMalloc(&g_ctx.pinned, 4096 * sizeof(double), hipHostMallocDefault));
+  if (!g_ctx.pinned) {
+    DECO_HIP(hipHostMalloc(&g_ctx.pinned, 4096 * sizeof(double), hipHostMallocCoherent));
+    DECO_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&g_ctx.pinned_dev), g_ctx.pinned, 0));
+  }
   g_ctx.device = device;
   g_ctx.num_cu = prop.multiProcessorCount;
   g_ctx.lds_per_block = prop.sharedMemPerBlock;

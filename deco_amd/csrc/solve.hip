@@ -133,15 +133,21 @@ __global__ __launch_bounds__(kArnBlock) void k_arn_dots(int n, int j, const doub
 }
 
 // out[v] = sum over the nb block partials of value v: one workgroup per value, every lane a
-// strided subset (loads issued together), then a fixed-order workgroup sum
-__global__ __launch_bounds__(256) void k_arn_reduce(int nb, const double* __restrict__ part, double* __restrict__ out) {
+// strided subset (loads issued together), then a fixed-order workgroup sum. The value also goes
+// to host (pinned host memory, read by the host after an event behind this kernel).
+__global__ __launch_bounds__(256) void k_arn_reduce(int nb, const double* __restrict__ part, double* __restrict__ out,
+                                                   double* __restrict__ host) {
   __shared__ double sm[4];
   const double* p = part + (size_t)blockIdx.x * nb;
   double s = 0.0;
 #pragma unroll 8
   for (int b = threadIdx.x; b < nb; b += 256) s += p[b];
   s = block_sum<256>(s, sm);
-  if (threadIdx.x == 0) out[blockIdx.x] = s;
+  if (threadIdx.x == 0) {
+    out[blockIdx.x] = s;
+    host[blockIdx.x] = s;
+    __threadfence_system();
+  }
 }
 
 // The step's scalars from the reduced values (same operation order on host and device):
@@ -546,21 +552,23 @@ static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, cons
     bool have_w = false;   // step j's operator application is already queued
     double rn1 = -1.0, rn2 = -1.0;   // the last two residual estimates of this cycle
     // one reduction of step j: s = Q^T u, |u|^2 (and z = Q^T w, u.w with w); the update sweep
-    // is queued behind it when w is there. Leaves the values in red after the event.
+    // is queued behind it when w is there. The reduction writes the values to red (pinned host
+    // memory) itself, so the event behind it releases the host while the update runs; the host
+    // has read red before it queues the next reduction.
     auto reduce_step = [&](const double* uu, const double* wp, double* un) {
       const int nv = wp ? 2 * j + 2 : j + 1;
       hipLaunchKernelGGL(k_arn_dots, dim3(nbk), dim3(kArnBlock), sizeof(double) * 4 * (2 * j + 2), stream(), n, j,
                          (const double*)V.p, uu, wp, part.p, nbk);
       DECO_CHECK_LAUNCH();
-      hipLaunchKernelGGL(k_arn_reduce, dim3(nv), dim3(256), 0, stream(), nbk, (const double*)part.p, redd.p);
+      hipLaunchKernelGGL(k_arn_reduce, dim3(nv), dim3(256), 0, stream(), nbk, (const double*)part.p, redd.p,
+                         ctx().pinned_dev);
       DECO_CHECK_LAUNCH();
+      DECO_HIP(hipEventRecord(ev.e, stream()));
       if (wp) {   // queued before the host decision; unused if the cycle ends here
         hipLaunchKernelGGL(k_arn_update, dim3(nbk), dim3(kArnBlock), 0, stream(), n, j, (const double*)V.p, uu, wp,
                            (const double*)redd.p, V.p + (size_t)j * n, un);
         DECO_CHECK_LAUNCH();
       }
-      DECO_HIP(hipMemcpyAsync(red, redd.p, sizeof(double) * nv, hipMemcpyDeviceToHost, stream()));
-      DECO_HIP(hipEventRecord(ev.e, stream()));
     };
     // finalise column j-1 with u_j = Q s + rho v_j and rotate it (rgmres.tpp:80-93); true when
     // the cycle ends here
