@@ -59,7 +59,8 @@ class DeviceArray:
 
     def copy_from(self, a):
         a = np.ascontiguousarray(a, dtype=self.dtype)
-        assert a.size == self.n
+        if a.size != self.n:
+            raise ValueError(f"copy_from: {a.size} values for a device array of {self.n}")
         check(lib().deco_memcpy_h2d(self.ptr, _vp(a), self.nbytes))
 
     def __del__(self):
@@ -72,7 +73,11 @@ class DeviceArray:
 
 
 def as_dev(x, dtype=np.float64):
+    """x as a device array of dtype: a DeviceArray is passed through (it must hold dtype), host
+    data is uploaded."""
     if isinstance(x, DeviceArray):
+        if x.dtype != np.dtype(dtype):
+            raise TypeError(f"device array of {x.dtype} where {np.dtype(dtype)} is needed")
         return x
     return DeviceArray.from_numpy(np.asarray(x, dtype=dtype))
 

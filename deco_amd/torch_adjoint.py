@@ -68,17 +68,32 @@ class _Handles:
             pass
 
 
+def _check_tensor(t, name, n):
+    if t.dtype != torch.float64 or not t.is_cuda:
+        raise TypeError(f"{name} must be a float64 tensor on the GPU, got {t.dtype} on {t.device}")
+    if t.numel() != n:
+        raise ValueError(f"{name} has {t.numel()} entries, the pattern needs {n}")
+
+
 def _solve(solver, values, b):
     L = lib()
+    _check_tensor(values, "values", solver.pattern.nnz)
+    _check_tensor(b, "b", solver.pattern.n)
     torch.cuda.synchronize()
     A = solver._csr(values.contiguous())
-    bw = b.detach().clone().contiguous()
     M = C.c_void_p()
-    check(L.deco_cpr_create(C.byref(cpr_params(solver.prec, solver.precision)), A, _ptr(bw), C.byref(M)))
-    x = torch.empty_like(bw)
-    info = SolInfo()
-    sp = sol_params(solver.sol)
-    check(L.deco_solve(C.byref(sp), A, M, _ptr(bw), _ptr(x), C.byref(info)))
+    try:
+        bw = b.detach().clone().contiguous()
+        check(L.deco_cpr_create(C.byref(cpr_params(solver.prec, solver.precision)), A, _ptr(bw), C.byref(M)))
+        x = torch.empty_like(bw)
+        info = SolInfo()
+        sp = sol_params(solver.sol)
+        check(L.deco_solve(C.byref(sp), A, M, _ptr(bw), _ptr(x), C.byref(info)))
+    except Exception:
+        if M.value:
+            L.deco_prec_destroy(M.value)
+        L.deco_csr_destroy(A)
+        raise
     return A, M.value, x, info
 
 
@@ -110,9 +125,11 @@ class CPRSolve(torch.autograd.Function):
         Pat = C.c_void_p()
         zeros = torch.zeros(p.nnz, dtype=torch.float64, device=g.device)
         check(L.deco_csr_create_dev(p.n, p.n, p.nnz, _ptr(p.rowptr), _ptr(p.cols), _ptr(zeros), C.byref(Pat)))
-        dA = torch.empty(p.nnz, dtype=torch.float64, device=g.device)
-        check(L.deco_grad_A_pattern(Pat.value, _ptr(lam), _ptr(x), _ptr(dA)))
-        L.deco_csr_destroy(Pat.value)
+        try:
+            dA = torch.empty(p.nnz, dtype=torch.float64, device=g.device)
+            check(L.deco_grad_A_pattern(Pat.value, _ptr(lam), _ptr(x), _ptr(dA)))
+        finally:
+            L.deco_csr_destroy(Pat.value)
         ctx.handles = None
         solver.last_adjoint = info
         return dA, lam, None

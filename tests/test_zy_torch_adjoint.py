@@ -55,3 +55,25 @@ def test_adjoint_gradients_fd_and_direct(gpu):
         fd = (Lp - Lm) / (2 * h)
         assert abs(fd - dA[k]) <= 1e-5 * max(np.abs(full).max(), 1e-300), (k, fd, dA[k])
     del Scsr
+
+
+def test_adjoint_rejects_mistyped_tensors(gpu):
+    """The binding reads values and b as float64 device arrays of the pattern's sizes: other
+    dtypes, host tensors and wrong lengths are errors before any library call."""
+    import torch
+    from deco_amd.torch_adjoint import SparsePattern, CPRSolver, cpr_solve
+
+    g = gen_system(4)
+    pat = SparsePattern(g["n"], g["r"], g["c"])
+    solver = CPRSolver(pat, small_params(2), {"type": "rgmres", "reltol": "1e-8", "restart": "30",
+                                              "itmax": "100", "verbosity": "0"})
+    vals = torch.tensor(g["v"], dtype=torch.float64, device="cuda")
+    b = torch.tensor(g["b"], dtype=torch.float64, device="cuda")
+    with pytest.raises(TypeError):
+        cpr_solve(vals.float(), b, solver)
+    with pytest.raises(TypeError):
+        cpr_solve(vals, b.cpu(), solver)
+    with pytest.raises(ValueError):
+        cpr_solve(vals[:-1], b, solver)
+    x = cpr_solve(vals, b, solver)
+    assert x.shape == b.shape and solver.last.flag == 0
