@@ -610,6 +610,7 @@ void deco_csr::invalidate() {
   vf_valid = false;
   rb_valid = false;
   sell_ok = -1;
+  c16_ok = -1;
   delete b32;
   b32 = nullptr;
 }
@@ -893,6 +894,73 @@ const int2* deco_csr::row_chunks() const {
     rb_valid = true;
   }
   return rb.p;
+}
+
+namespace deco {
+// per stream chunk: its smallest column (the base of its 16-bit offsets), and ovf set when its
+// columns span more than 65535
+__global__ __launch_bounds__(256) void k_chunk_span(const int2* __restrict__ desc, const int* __restrict__ c,
+                                                    int* __restrict__ cbase, int* __restrict__ ovf) {
+  __shared__ int slo[4], shi[4];
+  const int j0 = desc[blockIdx.x].y, j1 = desc[blockIdx.x + 1].y;
+  int lo = INT_MAX, hi = INT_MIN;
+  for (int j = j0 + (int)threadIdx.x; j < j1; j += 256) {
+    lo = min(lo, c[j]);
+    hi = max(hi, c[j]);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    lo = min(lo, __shfl_xor(lo, off));
+    hi = max(hi, __shfl_xor(hi, off));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    slo[threadIdx.x >> 6] = lo;
+    shi[threadIdx.x >> 6] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; ++w) {
+      lo = min(lo, slo[w]);
+      hi = max(hi, shi[w]);
+    }
+    lo = min(lo, slo[0]);
+    hi = max(hi, shi[0]);
+    cbase[blockIdx.x] = j1 > j0 ? lo : 0;
+    if (j1 > j0 && (int64_t)hi - lo > 65535) atomicOr(ovf, 1);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_chunk_c16(const int2* __restrict__ desc, const int* __restrict__ c,
+                                                   const int* __restrict__ cbase, unsigned short* __restrict__ c16) {
+  const int j0 = desc[blockIdx.x].y, j1 = desc[blockIdx.x + 1].y, b = cbase[blockIdx.x];
+  for (int j = j0 + (int)threadIdx.x; j < j1; j += 256) c16[j] = (unsigned short)(c[j] - b);
+}
+}  // namespace deco
+
+bool deco_csr::cols16() const {
+  using namespace deco;
+  if (c16_ok < 0) {
+    c16_ok = 0;
+    if (g_sell_offsets16 && n > 0 && nnz > 0) {
+      const int2* desc = row_chunks();
+      cbase.ensure(nchunks);
+      DBuf<int> ovf(1);
+      ovf.zero();
+      hipLaunchKernelGGL(k_chunk_span, dim3(nchunks), dim3(256), 0, stream(), desc, (const int*)c.p, cbase.p, ovf.p);
+      DECO_CHECK_LAUNCH();
+      int o = 0;
+      copy_d2h(&o, ovf.p, 1);
+      if (o == 0) {
+        c16.ensure(nnz);
+        hipLaunchKernelGGL(k_chunk_c16, dim3(nchunks), dim3(256), 0, stream(), desc, (const int*)c.p,
+                           (const int*)cbase.p, c16.p);
+        DECO_CHECK_LAUNCH();
+        c16_ok = 1;
+      } else {
+        cbase.release();
+      }
+    }
+  }
+  return c16_ok == 1;
 }
 
 const float* deco_csr::values_f32() {

@@ -59,6 +59,13 @@ struct deco_csr {
   // whether the rows fit the sliced layout within the padding threshold (decided on first call)
   bool sell_scalar() const;
   mutable signed char sell_ok = -1;
+  // Columns as 16-bit offsets from a per-chunk base (the smallest column of each stream chunk),
+  // for the stream products of the AMG levels: built on first call when every chunk's columns
+  // span at most 65535, else false (32-bit columns). Dropped by invalidate().
+  bool cols16() const;
+  mutable deco::DBuf<unsigned short> c16;
+  mutable deco::DBuf<int> cbase;
+  mutable signed char c16_ok = -1;
   void invalidate();
 };
 
@@ -223,11 +230,14 @@ struct PtrGather {
   __device__ __forceinline__ XT operator()(int j) const { return x[j]; }
 };
 
-template <class VT, class XG, class E>
+// CT int: c holds the columns. CT unsigned short: c holds 16-bit offsets from cbase[chunk]
+// (deco_csr::cols16).
+template <class VT, class XG, class E, class CT = int>
 __global__ __launch_bounds__(stream_block<VT>()) void k_spmv_stream(const int2* __restrict__ desc,
                                                                    const int* __restrict__ r,
-                                                                   const int* __restrict__ c,
-                                                                   const VT* __restrict__ v, XG x, E epi) {
+                                                                   const CT* __restrict__ c,
+                                                                   const VT* __restrict__ v, XG x, E epi,
+                                                                   const int* __restrict__ cbase = nullptr) {
   using XT = typename XG::type;
   using AT = acc_t<VT, XT>;
   constexpr int BLK = stream_block<VT>();
@@ -238,6 +248,8 @@ __global__ __launch_bounds__(stream_block<VT>()) void k_spmv_stream(const int2* 
   const int2 d0 = desc[ch], d1 = desc[ch + 1];   // (first row, its first entry) of chunks ch, ch+1
   const int r0 = d0.x, r1 = d1.x;
   if (r0 >= r1) return;
+  int cb = 0;
+  if constexpr (!std::is_same<CT, int>::value) cb = cbase[ch];
   const int j0 = d0.y, j1 = d1.y;
   const int cnt = j1 - j0;
   const int tid = threadIdx.x;
@@ -267,7 +279,7 @@ __global__ __launch_bounds__(stream_block<VT>()) void k_spmv_stream(const int2* 
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int j = min(j0 + tid + u * BLK, jl);
-        cc[u] = __builtin_nontemporal_load(c + j);
+        cc[u] = cb + (int)__builtin_nontemporal_load(c + j);
         vv[u] = __builtin_nontemporal_load(v + j);
       }
 #pragma unroll
@@ -278,7 +290,7 @@ __global__ __launch_bounds__(stream_block<VT>()) void k_spmv_stream(const int2* 
         if (t < cnt) prod[t] = (AT)vv[u] * (AT)xx[u];
       }
       for (int t = tid + kStreamCap; t < cnt; t += BLK)
-        prod[t] = (AT)__builtin_nontemporal_load(v + j0 + t) * (AT)x(__builtin_nontemporal_load(c + j0 + t));
+        prod[t] = (AT)__builtin_nontemporal_load(v + j0 + t) * (AT)x(cb + (int)__builtin_nontemporal_load(c + j0 + t));
     }
     __syncthreads();
     if (row < r1) {   // a lane group is active or inactive as a whole
@@ -300,7 +312,7 @@ __global__ __launch_bounds__(stream_block<VT>()) void k_spmv_stream(const int2* 
     for (int rw = r0; rw < r1; ++rw) {
       const int b = r[rw], e = r[rw + 1];
       AT acc = AT(0);
-      for (int j = b + tid; j < e; j += BLK) acc += (AT)v[j] * (AT)x(c[j]);
+      for (int j = b + tid; j < e; j += BLK) acc += (AT)v[j] * (AT)x(cb + (int)c[j]);
       acc = block_sum<BLK>(acc, red);
       if (tid == 0) epi(rw, acc, epi.pre(rw));
     }
@@ -312,7 +324,21 @@ void spmv_stream_g(const deco_csr* A, const VT* v, XG x, E epi) {
   if (A->n <= 0) return;
   const int2* desc = A->row_chunks();
   hipLaunchKernelGGL((k_spmv_stream<VT, XG, E>), dim3(A->nchunks), dim3(stream_block<VT>()), 0, stream(), desc,
-                     A->r.p, A->c.p, v, x, epi);
+                     A->r.p, A->c.p, v, x, epi, (const int*)nullptr);
+  DECO_CHECK_LAUNCH();
+}
+
+// the same on the 16-bit chunk columns when A has them (deco_csr::cols16), else as above
+template <class VT, class XG, class E>
+void spmv_stream_c16(const deco_csr* A, const VT* v, XG x, E epi) {
+  if (A->n <= 0) return;
+  if (!A->cols16()) {
+    spmv_stream_g(A, v, x, epi);
+    return;
+  }
+  const int2* desc = A->row_chunks();
+  hipLaunchKernelGGL((k_spmv_stream<VT, XG, E, unsigned short>), dim3(A->nchunks), dim3(stream_block<VT>()), 0,
+                     stream(), desc, A->r.p, (const unsigned short*)A->c16.p, v, x, epi, (const int*)A->cbase.p);
   DECO_CHECK_LAUNCH();
 }
 
@@ -638,8 +664,8 @@ void bsr_stream_v(const deco_bsr* B, const VT* bv, const XT* x, E epi) {
 }
 
 // Scalar product y = A x (epilogue e) with VT values: through A's sliced 1 x 1 shadow when its
-// padding is within the threshold (built on first use), else the CSR-stream kernel. Each row
-// sums in column order in the sliced layout.
+// padding is within the threshold (built on first use), else the CSR-stream kernel on 16-bit
+// chunk columns where they fit. Each row sums in column order in the sliced layout.
 template <class VT, class XT, class E>
 void spmv_auto(const deco_csr* A, const XT* x, E epi) {
   const deco_bsr* B = !A->sell_scalar() ? nullptr
@@ -649,8 +675,8 @@ void spmv_auto(const deco_csr* A, const XT* x, E epi) {
     else bsr_stream_v<float>(B, B->bv.p, x, epi);
     return;
   }
-  if constexpr (std::is_same<VT, double>::value) spmv_stream_f(A, A->v.p, x, epi);
-  else spmv_stream_f(A, const_cast<deco_csr*>(A)->values_f32(), x, epi);
+  if constexpr (std::is_same<VT, double>::value) spmv_stream_c16(A, A->v.p, PtrGather<XT>{x}, epi);
+  else spmv_stream_c16(A, const_cast<deco_csr*>(A)->values_f32(), PtrGather<XT>{x}, epi);
 }
 
 // product with the FP32 block values

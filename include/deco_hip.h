@@ -214,7 +214,9 @@ int deco_read_bandwidth(size_t bytes, int iters, double* gbps);
 int deco_set_sell_padding(double max_fraction);
 /* Whether sliced block shadows made from now on store their block columns as 16-bit offsets
  * from the block row (default 1; taken when every offset fits and the block rows do not
- * outnumber the block columns, else 32-bit columns). No reference counterpart. */
+ * outnumber the block columns, else 32-bit columns), and whether the stream products of the
+ * AMG levels read their columns as 16-bit offsets from each row chunk's smallest column (taken
+ * when every chunk spans at most 65535 columns). No reference counterpart. */
 int deco_set_sell_offsets16(int enable);
 /* The R x C block shadow of A (created if needed): its layout (0 chunked, 1 sliced, 2 sliced with
  * 16-bit block-column offsets), its number of blocks, and the blocks it stores (padding

@@ -307,3 +307,46 @@ def test_cpr_setup_options_bitexact(gpu, cfg, opts):
     Mo.set_local(Go)
     x = np.random.default_rng(13).uniform(-1, 1, g["n"])
     assert rel(M.apply(x).numpy(), Mo.apply(x)) <= 1e-12
+
+
+def test_amg_chunk_columns16_bit_identical(gpu):
+    """The AMG levels' stream products on 16-bit chunk-relative columns give the V-cycle of the
+    32-bit columns bit for bit, in both precisions."""
+    g = gen_system(1)
+    tree = {"amg_SoC_alpha": "0.5", "amg_coarse_type": "10", "amg_relax_type": "18",
+            "amg_prol_trunc_fac": "1e-3", "amg_A_drop_tol": "1e-4", "amg_max_coarse": "50"}
+    x = np.random.default_rng(14).uniform(-1, 1, g["n"] // 2)
+    out = {}
+    for off16 in (True, False):
+        deco_amd.set_sell_offsets16(off16)
+        try:
+            Ap = dev_csr(g).local_matrix(2, 0)
+            G = deco_amd.AMG(tree, Ap)
+            y = G.apply(x).numpy()
+            G.set_precision(deco_amd.PREC_FP32)
+            out[off16] = (y, G.apply(x).numpy())
+        finally:
+            deco_amd.set_sell_offsets16(True)
+    assert np.array_equal(out[True][0], out[False][0]) and np.array_equal(out[True][1], out[False][1])
+
+
+def test_amg_wide_chunk_span_falls_back(gpu):
+    """Couplings 80,000 rows apart make the restriction's row chunks span more than 65535
+    columns: those products keep 32-bit columns, and the V-cycle still matches the oracle's."""
+    import scipy.sparse as sp
+    m = 300
+    T = sp.diags([-1.0, 2.0, -1.0], [-1, 0, 1], shape=(m, m))
+    L = (sp.kron(sp.eye(m), T) + sp.kron(T, sp.eye(m))).tocsr()
+    n = L.shape[0]
+    i = np.arange(0, 5000, 97)
+    far = sp.csr_matrix((np.full(2 * len(i), -1.5), (np.concatenate([i, i + 80000]), np.concatenate([i + 80000, i]))),
+                        shape=(n, n))
+    S = (L + far + sp.diags(np.asarray(abs(far).sum(1)).ravel())).tocsr()
+    S.sort_indices()
+    tree = {"amg_SoC_alpha": "0.25", "amg_coarse_type": "10", "amg_relax_type": "18",
+            "amg_prol_trunc_fac": "1e-3", "amg_A_drop_tol": "0", "amg_max_coarse": "300"}
+    G = deco_amd.AMG(tree, deco_amd.CSR.from_scipy(S))
+    Go, levels, _ = oracle_amg(G, 18)
+    assert len(levels) >= 2
+    x = np.random.default_rng(15).uniform(-1, 1, n)
+    assert rel(G.apply(x).numpy(), Go.apply(x)) <= 1e-12
