@@ -130,6 +130,8 @@ SIGNATURES = {
     "deco_amg_num_levels": (i, [vp, ip]),
     "deco_amg_level": (i, [vp, i, pp, pp, pp, pp, pp]),
     "deco_amg_coarse_inverse": (i, [vp, ip, pp]),
+    "deco_amg_dense_tail": (i, [vp, ip]),
+    "deco_set_amg_dense_tail": (i, [i]),
     "deco_amg_interp_host": (i, [C.POINTER(CprParams), i, vp, vp, vp, vp, ip, ip, pp, pp, pp]),
     "deco_host_free": (None, [vp]),
     "deco_workspace_release": (i, []),

@@ -502,6 +502,12 @@ class AMG(Prec):
             check(L.deco_memcpy_d2h(_vp(ci), inv.value, 8 * ci.size))
         return levels, ci.reshape(nc.value, nc.value)
 
+    def dense_tail(self):
+        """Rows of the dense operator that replaces the last level's sub-cycle (0: none)."""
+        rows = C.c_int()
+        check(lib().deco_amg_dense_tail(self.h, C.byref(rows)))
+        return rows.value
+
 
 class CPR(Prec):
     """CPR<int,int,double,FSAI|Jacobi,AMG>(param, A, b) (precond/CPR.hpp:11-172).
@@ -581,6 +587,12 @@ def set_sell_offsets16(enable):
     """Sliced block shadows made from now on store their block columns as 16-bit offsets from
     the block row when every offset fits (default), else (or when disabled) 32-bit columns."""
     check(lib().deco_set_sell_offsets16(1 if enable else 0))
+
+
+def set_amg_dense_tail(max_rows):
+    """AMG hierarchies set up from now on replace the sub-cycle below their last level with one
+    dense operator when that level has at most max_rows rows (default 2048; 0 disables)."""
+    check(lib().deco_set_amg_dense_tail(int(max_rows)))
 
 
 # ---------------------------------------------------------------------------- profiler ---

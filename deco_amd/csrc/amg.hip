@@ -200,21 +200,30 @@ static deco_csr* up_operator(const deco_csr* P, const deco_csr* Q, const double*
   return S;
 }
 
-// coarse solve x = C b, C row-major with leading dimension nc (the inverse for the forward
-// cycle, its transpose for the transposed one): one wavefront per row, lanes stride the
-// columns, fixed-order butterfly sum; input/output in T, FP64 arithmetic
-template <class T>
-__global__ __launch_bounds__(256) void k_coarse(int nc, const double* __restrict__ C, const T* __restrict__ b,
-                                                T* __restrict__ x) {
+// x = C b with C row-major n x n in MT (the coarsest inverse for the forward cycle, its
+// transpose for the transposed one; or the last level's dense sub-cycle operator): one wavefront
+// per row, lanes stride the columns, fixed-order butterfly sum; input/output in T, FP64
+// arithmetic; with add_to set, also add_to[i*stride] += x_i
+template <class T, class MT>
+__global__ __launch_bounds__(256) void k_dense(int n, const MT* __restrict__ C, const T* __restrict__ b,
+                                               T* __restrict__ x, T* __restrict__ add_to, int stride) {
   const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  if (row >= nc) return;
-  const double* cr = C + (size_t)row * nc;
+  if (row >= n) return;
+  const MT* cr = C + (size_t)row * n;
   double s = 0.0;
-  for (int k = lane; k < nc; k += 64) s += cr[k] * (double)b[k];
+#pragma unroll 4
+  for (int k = lane; k < n; k += 64) s += (double)cr[k] * (double)b[k];
   s = wave_sum(s);
-  if (lane == 0) x[row] = (T)s;
+  if (lane == 0) {
+    const T y = (T)s;
+    x[row] = y;
+    if (add_to) add_to[(size_t)row * stride] += y;
+  }
 }
+
+// largest last level (rows) whose sub-cycle is formed as one dense operator at setup
+static int g_dense_tail_rows = 2048;
 
 struct AMG : deco_prec {
   struct Level {
@@ -238,6 +247,12 @@ struct AMG : deco_prec {
   DBuf<double> xc, bc;
   DBuf<float> xcf, bcf;
   DBuf<float> r32, x32;       // FP32 copies of a standalone apply's input and output
+  // the last level's sub-cycle (zero-guess sweep, residual, restriction, coarse solve, up-sweep)
+  // as one dense operator Z, row-major, and its transpose for the transposed cycle (Jacobi-type
+  // smoothers, last level of at most g_dense_tail_rows rows); tail = its rows, 0 when absent
+  int tail = 0;
+  DBuf<double> Z64, Z64t;
+  DBuf<float> Z32, Z32t;
   int relax = 18;
   double weight = 1.0;
   bool have_t = false;
@@ -370,8 +385,60 @@ struct AMG : deco_prec {
     Cinv.upload(inv.data(), inv.size());
     CT.upload(invT.data(), invT.size());
     xc.alloc(nc); bc.alloc(nc); xcf.alloc(nc); bcf.alloc(nc);
+    ph_ci.end();
+    if (relax != 17 && !L.empty() && L.back().A->n <= g_dense_tail_rows) form_tail(inv);
     sync();
   }
+
+  static void download(const deco_csr* A, amg::HostCSR& H) {
+    H.n = A->n;
+    H.m = A->m;
+    H.r.resize(A->n + 1);
+    H.c.resize(A->nnz);
+    H.v.resize(A->nnz);
+    A->r.download(H.r.data(), A->n + 1);
+    A->c.download(H.c.data(), A->nnz);
+    A->v.download(H.v.data(), A->nnz);
+  }
+
+  // Z = 2 D^{-1} - D^{-1} A D^{-1} + S C R (I - A D^{-1}) of the last level, formed on the host in
+  // FP64 and stored in FP64 and FP32 with its transpose (R = P^T, so the transposed sub-cycle with
+  // A^T, S^T's counterpart P - D^{-1} A^T P and C^T is Z^T)
+  void form_tail(const std::vector<double>& inv) {
+    ProfPhase ph("dense tail");
+    Level& lv = L.back();
+    const int nl = lv.A->n;
+    amg::HostCSR A, R, S;
+    download(lv.A, A);
+    download(lv.R, R);
+    download(lv.S, S);
+    std::vector<double> d(nl), Z, Zt((size_t)nl * nl);
+    lv.d.download(d.data(), nl);
+    amg::dense_tail(A, d, R, S, inv, Z);
+    for (int i = 0; i < nl; ++i)
+      for (int k = 0; k < nl; ++k) Zt[(size_t)k * nl + i] = Z[(size_t)i * nl + k];
+    std::vector<float> z32(Z.begin(), Z.end()), zt32(Zt.begin(), Zt.end());
+    Z64.upload(Z.data(), Z.size());
+    Z64t.upload(Zt.data(), Zt.size());
+    Z32.upload(z32.data(), z32.size());
+    Z32t.upload(zt32.data(), zt32.size());
+    sync();   // the host arrays go out of scope here
+    tail = nl;
+  }
+
+  template <class MT>
+  void dense(int m, const MT* C, const float* b, float* x, float* add_to, int stride) {
+    hipLaunchKernelGGL((k_dense<float, MT>), dim3((m + 3) / 4), dim3(256), 0, stream(), m, C, b, x, add_to, stride);
+    DECO_CHECK_LAUNCH();
+  }
+  template <class MT>
+  void dense(int m, const MT* C, const double* b, double* x, double* add_to, int stride) {
+    hipLaunchKernelGGL((k_dense<double, MT>), dim3((m + 3) / 4), dim3(256), 0, stream(), m, C, b, x, add_to, stride);
+    DECO_CHECK_LAUNCH();
+  }
+  // the dense tail operator in the cycle's precision
+  const double* tail_op(bool tr, const double*) const { return tr ? Z64t.p : Z64.p; }
+  const float* tail_op(bool tr, const float*) const { return tr ? Z32t.p : Z32.p; }
 
   void release() {
     for (auto& lv : L) {
@@ -406,9 +473,11 @@ struct AMG : deco_prec {
   void cycle(size_t l, const T* b, T* x, bool tr, bool x_ready = false, T* add_to = nullptr, int stride = 0) {
     if (l == L.size()) {
       if (nc == 0) return;
-      hipLaunchKernelGGL(k_coarse<T>, dim3((nc + 3) / 4), dim3(256), 0, stream(), nc,
-                         (const double*)(tr ? CT.p : Cinv.p), b, x);
-      DECO_CHECK_LAUNCH();
+      dense(nc, (const double*)(tr ? CT.p : Cinv.p), b, x, (T*)nullptr, 0);
+      return;
+    }
+    if (tail && l + 1 == L.size()) {   // the whole sub-cycle of the last level as one product
+      dense(tail, tail_op(tr, b), b, x, add_to, stride);
       return;
     }
     Level& lv = L[l];
@@ -432,7 +501,7 @@ struct AMG : deco_prec {
     const bool last = (l + 1 == L.size());
     T* bn = last ? BC<T>() : B<T>(L[l + 1]);
     T* xn = last ? XC<T>() : X<T>(L[l + 1]);
-    const bool fuse = !last && relax != 17;
+    const bool fuse = !last && relax != 17 && !(tail && l + 2 == L.size());
     if (fuse)
       spmv_auto<T>(lv.R, (const T*)r, EpiRestrictJacobi0<T>{bn, D<T>(L[l + 1]), xn});
     else
@@ -470,9 +539,7 @@ struct AMG : deco_prec {
   void run(const T* r, T* x, bool tr) {
     if (tr) ensure_t();
     if (L.empty()) {
-      hipLaunchKernelGGL(k_coarse<T>, dim3((nc + 3) / 4), dim3(256), 0, stream(), nc,
-                         (const double*)(tr ? CT.p : Cinv.p), r, x);
-      DECO_CHECK_LAUNCH();
+      dense(nc, (const double*)(tr ? CT.p : Cinv.p), r, x, (T*)nullptr, 0);
       return;
     }
     cycle<T>(0, r, x, tr);
@@ -586,6 +653,20 @@ int deco_amg_level(const deco_prec* M, int level, const deco_csr** A, const deco
     if (R) *R = lv.R;
     if (d) *d = lv.d.p;
     if (cf) *cf = lv.cf.p;
+  });
+}
+
+int deco_set_amg_dense_tail(int max_rows) {
+  return guard([&] {
+    if (max_rows < 0 || max_rows > 8192) fail(DECO_ERR_ARG, "max_rows %d outside [0, 8192]", max_rows);
+    g_dense_tail_rows = max_rows;
+  });
+}
+
+int deco_amg_dense_tail(const deco_prec* M, int* rows) {
+  return guard([&] {
+    if (!rows) fail(DECO_ERR_ARG, "null argument");
+    *rows = as_amg(M)->tail;
   });
 }
 

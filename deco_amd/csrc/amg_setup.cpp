@@ -381,5 +381,46 @@ bool dense_inverse(const HostCSR& A, std::vector<double>& inv) {
   return true;
 }
 
+void dense_tail(const HostCSR& A, const std::vector<double>& d, const HostCSR& R, const HostCSR& S,
+                const std::vector<double>& C, std::vector<double>& Z) {
+  const int n = A.n, nc = R.n;
+  // G = R (I - A D^{-1}): the restriction of the residual left by the zero-guess sweep
+  std::vector<double> G((size_t)nc * n, 0.0);
+#pragma omp parallel for schedule(dynamic, 8)
+  for (int i = 0; i < nc; ++i) {
+    double* g = &G[(size_t)i * n];
+    for (int e = R.r[i]; e < R.r[i + 1]; ++e) {
+      const int k = R.c[e];
+      const double w = R.v[e];
+      g[k] += w;
+      for (int f = A.r[k]; f < A.r[k + 1]; ++f) g[A.c[f]] -= w * (A.v[f] / d[A.c[f]]);
+    }
+  }
+  // H = C G: the coarse correction
+  std::vector<double> H((size_t)nc * n, 0.0);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < nc; ++i) {
+    double* h = &H[(size_t)i * n];
+    for (int k = 0; k < nc; ++k) {
+      const double c = C[(size_t)i * nc + k];
+      const double* g = &G[(size_t)k * n];
+      for (int j = 0; j < n; ++j) h[j] += c * g[j];
+    }
+  }
+  // Z = S H + 2 D^{-1} - D^{-1} A D^{-1}
+  Z.assign((size_t)n * n, 0.0);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int i = 0; i < n; ++i) {
+    double* z = &Z[(size_t)i * n];
+    for (int e = S.r[i]; e < S.r[i + 1]; ++e) {
+      const double s = S.v[e];
+      const double* h = &H[(size_t)S.c[e] * n];
+      for (int j = 0; j < n; ++j) z[j] += s * h[j];
+    }
+    for (int f = A.r[i]; f < A.r[i + 1]; ++f) z[A.c[f]] -= A.v[f] / (d[i] * d[A.c[f]]);
+    z[i] += 2.0 / d[i];
+  }
+}
+
 }  // namespace amg
 }  // namespace deco

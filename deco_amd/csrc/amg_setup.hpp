@@ -54,5 +54,12 @@ HostCSR interp_extpi(const HostCSR& A, const std::vector<char>& strong, const st
 // Dense inverse of a small matrix by LU with partial pivoting; returns false if singular.
 bool dense_inverse(const HostCSR& A, std::vector<double>& inv_rowmajor);
 
+// The sub-cycle below a level with Jacobi-type smoothing whose next level is the coarsest, as
+// one dense n x n operator (row-major): x0 = D^{-1} b, r = b - A x0, x = x0 + D^{-1} r + S C R r
+// with S = P - D^{-1} A P and C the coarsest inverse (row-major, R.n x R.n), that is
+// Z = 2 D^{-1} - D^{-1} A D^{-1} + S C R (I - A D^{-1}).
+void dense_tail(const HostCSR& A, const std::vector<double>& d, const HostCSR& R, const HostCSR& S,
+                const std::vector<double>& C, std::vector<double>& Z);
+
 }  // namespace amg
 }  // namespace deco

@@ -286,6 +286,16 @@ int deco_amg_level(const deco_prec* M, int level, const deco_csr** A, const deco
                    const deco_csr** R, const double** smoother_dev, const int** cf_dev);
 /* Dense FP64 inverse of the coarsest matrix (row-major nc*nc) and its size. */
 int deco_amg_coarse_inverse(const deco_prec* M, int* nc, const double** inv_dev);
+/* Rows of the AMG's dense tail operator (0 when the hierarchy has none): the last level's
+ * sub-cycle formed as one dense matrix at setup. No reference counterpart. */
+int deco_amg_dense_tail(const deco_prec* M, int* rows);
+/* AMG hierarchies set up from now on form the sub-cycle below their last level with
+ * interpolation (zero-guess Jacobi-type sweep, residual, restriction, coarsest solve, up-sweep)
+ * as one dense operator Z = 2D^-1 - D^-1 A D^-1 + (P - D^-1 A P) C P^T (I - A D^-1), applied by
+ * one product per cycle, when that level has at most max_rows rows (default 2048, at most 8192;
+ * 0 keeps the sparse sub-cycle; the F-C-F smoother 17 always does). Equal to the sparse
+ * sub-cycle up to rounding. No reference counterpart (hypre's cycle runs level by level). */
+int deco_set_amg_dense_tail(int max_rows);
 /* Host-only: one level of the AMG setup (strength, C/F splitting, extended+i interpolation and
  * truncation) on host CSR arrays, as deco_amg_create runs it per level. cf[i] = 1 (C) / -1 (F);
  * P (n x nc) is returned in malloc'd arrays released with deco_host_free. No device is used. */

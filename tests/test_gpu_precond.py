@@ -350,3 +350,56 @@ def test_amg_wide_chunk_span_falls_back(gpu):
     assert len(levels) >= 2
     x = np.random.default_rng(15).uniform(-1, 1, n)
     assert rel(G.apply(x).numpy(), Go.apply(x)) <= 1e-12
+
+
+@pytest.mark.parametrize("relax", [18, 0])
+def test_amg_dense_tail_matches_sparse_subcycle(gpu, relax):
+    """The last level's sub-cycle formed as one dense operator at setup, against the same
+    hierarchy with the sparse sub-cycle (dense tail disabled) and the oracle's level-by-level
+    V-cycle, forward and transposed, in both precisions."""
+    g = gen_system(1)
+    A = dev_csr(g)
+    Ap = A.local_matrix(2, 0)
+    tree = {"amg_SoC_alpha": "0.5", "amg_coarse_type": "10", "amg_relax_type": str(relax),
+            "amg_prol_trunc_fac": "1e-3", "amg_A_drop_tol": "1e-4", "amg_max_coarse": "50"}
+    deco_amd.set_amg_dense_tail(0)
+    try:
+        G0 = deco_amd.AMG(tree, Ap)
+    finally:
+        deco_amd.set_amg_dense_tail(2048)
+    G1 = deco_amd.AMG(tree, Ap)
+    Go, levels, _ = oracle_amg(G1, relax)
+    assert len(levels) >= 2
+    assert G0.dense_tail() == 0 and G1.dense_tail() == levels[-1]["n"] > 0
+    x = np.random.default_rng(7).uniform(-1, 1, Ap.n)
+    y0, y1 = G0.apply(x).numpy(), G1.apply(x).numpy()
+    assert rel(y1, y0) <= 1e-13 and not np.array_equal(y1, y0)   # the dense product did run
+    assert rel(y1, Go.apply(x)) <= 1e-12
+    t0, t1 = G0.apply_t(x).numpy(), G1.apply_t(x).numpy()
+    assert rel(t1, t0) <= 1e-13
+    assert rel(t1, Go.apply_t(x)) <= 1e-12
+    for G in (G0, G1):
+        G.set_precision(deco_amd.PREC_FP32)
+    assert rel(G1.apply(x).numpy(), G0.apply(x).numpy()) <= 1e-5
+    assert rel(G1.apply_t(x).numpy(), G0.apply_t(x).numpy()) <= 1e-5
+
+
+@pytest.mark.parametrize("seq", [0, 3])
+def test_cpr_dense_tail_is_the_whole_pressure_cycle(gpu, seq):
+    """One level with interpolation (2048 pressure rows, two AMG levels): the dense tail is the
+    whole pressure V-cycle, and in the fused CPR stage it also adds its result into the iterate."""
+    g = gen_system(4)
+    tree = small_params(2, seq=str(seq), **{"prec_loc.amg_max_levels": "2"})
+    A = dev_csr(g)
+    M = deco_amd.CPR(tree, A, None)
+    Go, levels, _ = oracle_amg(M.amg(), 18)
+    assert len(levels) == 1 and M.amg().dense_tail() == levels[0]["n"]
+    Mo = oracle.CPR(deco_amd.cpr_params(tree), to_mat(g), None)
+    Mo.set_local(Go)
+    x = np.random.default_rng(seq).uniform(-1, 1, g["n"])
+    assert rel(M.apply(x).numpy(), Mo.apply(x)) <= 1e-12
+    if seq == 3:                                          # the transposed CPR's sequence
+        assert rel(M.apply_t(x).numpy(), Mo.apply_t(x)) <= 1e-11
+    M.set_precision(deco_amd.PREC_FP32)
+    assert rel(M.apply(x).numpy(), Mo.apply(x)) <= 2e-4
+    M.set_precision(deco_amd.PREC_FP64)
