@@ -75,3 +75,18 @@ def test_profile_tree_report_format():
         deco_amd.profile_enable(True)
         deco_amd.profile_stop()
     deco_amd.profile_enable(False)
+
+
+def test_global_layout_switches_validate_without_a_device():
+    """The library-wide switches are plain host settings: in-range values are accepted (and
+    restored), out-of-range ones rejected with DECO_ERR_ARG, with no device present."""
+    import deco_amd
+    deco_amd.set_amg_dense_tail(0)
+    deco_amd.set_amg_dense_tail(8192)
+    deco_amd.set_amg_dense_tail(2048)
+    for bad in (-1, 8193):
+        with pytest.raises(deco_amd._lib.DecoError) as e:
+            deco_amd.set_amg_dense_tail(bad)
+        assert deco_amd._lib.ERRORS[e.value.code] == "DECO_ERR_ARG"
+    deco_amd.set_sell_offsets16(False)
+    deco_amd.set_sell_offsets16(True)
