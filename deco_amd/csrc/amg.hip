@@ -658,7 +658,7 @@ int deco_amg_level(const deco_prec* M, int level, const deco_csr** A, const deco
 
 int deco_set_amg_dense_tail(int max_rows) {
   return guard([&] {
-    if (max_rows < 0 || max_rows > 8192) fail(DECO_ERR_ARG, "max_rows %d outside [0, 8192]", max_rows);
+    if (max_rows < 0 || max_rows > 4096) fail(DECO_ERR_ARG, "max_rows %d outside [0, 4096]", max_rows);
     g_dense_tail_rows = max_rows;
   });
 }

@@ -292,9 +292,11 @@ int deco_amg_dense_tail(const deco_prec* M, int* rows);
 /* AMG hierarchies set up from now on form the sub-cycle below their last level with
  * interpolation (zero-guess Jacobi-type sweep, residual, restriction, coarsest solve, up-sweep)
  * as one dense operator Z = 2D^-1 - D^-1 A D^-1 + (P - D^-1 A P) C P^T (I - A D^-1), applied by
- * one product per cycle, when that level has at most max_rows rows (default 2048, at most 8192;
- * 0 keeps the sparse sub-cycle; the F-C-F smoother 17 always does). Equal to the sparse
- * sub-cycle up to rounding. No reference counterpart (hypre's cycle runs level by level). */
+ * one product per cycle, when that level has at most max_rows rows (default 2048, at most 4096;
+ * 0 keeps the sparse sub-cycle; the F-C-F smoother 17 always does). Forming Z costs
+ * O(n_c^2 n) host flops and 24 n^2 bytes of device memory (FP64 and FP32, forward and
+ * transposed). Equal to the sparse sub-cycle up to rounding. No reference counterpart (hypre's
+ * cycle runs level by level). */
 int deco_set_amg_dense_tail(int max_rows);
 /* Host-only: one level of the AMG setup (strength, C/F splitting, extended+i interpolation and
  * truncation) on host CSR arrays, as deco_amg_create runs it per level. cf[i] = 1 (C) / -1 (F);
