@@ -56,9 +56,15 @@ struct Context {
   int num_cu = 0;
   size_t lds_per_block = 0;
   // pinned host staging for scalar readbacks (coherent: kernels may write it through
-  // pinned_dev, its device address, and the host reads it after an event behind them)
+  // pinned_dev, its device address, and the host reads it after an event behind them or once
+  // the matching tags below are set)
   double* pinned = nullptr;
   double* pinned_dev = nullptr;
+  // per-value sequence tags beside it (coherent pinned): a kernel stores tags[v] = seq after
+  // value v and a system-scope fence, so the host may read value v once its tag reads seq
+  long long* tags = nullptr;
+  long long* tags_dev = nullptr;
+  long long tag_seq = 0;
   // library buffers come from the device's stream-ordered pool (allocation and release queued
   // on the library stream: no device-wide synchronisation, freed blocks reused)
   bool pooled = false;

@@ -50,6 +50,11 @@ static void init_device(int device) {
     DECO_HIP(hipHostMalloc(&g_ctx.pinned, 4096 * sizeof(double), hipHostMallocCoherent));
     DECO_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&g_ctx.pinned_dev), g_ctx.pinned, 0));
   }
+  if (!g_ctx.tags) {
+    DECO_HIP(hipHostMalloc(reinterpret_cast<void**>(&g_ctx.tags), 4096 * sizeof(long long), hipHostMallocCoherent));
+    std::memset(g_ctx.tags, 0, 4096 * sizeof(long long));
+    DECO_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&g_ctx.tags_dev), g_ctx.tags, 0));
+  }
   g_ctx.device = device;
   g_ctx.num_cu = prop.multiProcessorCount;
   g_ctx.lds_per_block = prop.sharedMemPerBlock;

@@ -134,9 +134,12 @@ __global__ __launch_bounds__(kArnBlock) void k_arn_dots(int n, int j, const doub
 
 // out[v] = sum over the nb block partials of value v: one workgroup per value, every lane a
 // strided subset (loads issued together), then a fixed-order workgroup sum. The value also goes
-// to host (pinned host memory, read by the host after an event behind this kernel).
+// to host (coherent pinned host memory), followed by a system-scope fence and tag[v] = seq, so
+// the host reads it as soon as the tag shows seq (no event in the stream between this kernel
+// and the update sweep behind it).
 __global__ __launch_bounds__(256) void k_arn_reduce(int nb, const double* __restrict__ part, double* __restrict__ out,
-                                                   double* __restrict__ host) {
+                                                   double* __restrict__ host, long long* __restrict__ tag,
+                                                   long long seq) {
   __shared__ double sm[4];
   const double* p = part + (size_t)blockIdx.x * nb;
   double s = 0.0;
@@ -147,6 +150,30 @@ __global__ __launch_bounds__(256) void k_arn_reduce(int nb, const double* __rest
     out[blockIdx.x] = s;
     host[blockIdx.x] = s;
     __threadfence_system();
+    tag[blockIdx.x] = seq;
+  }
+}
+
+// Waits until the reduction with sequence number seq has published values [0, nv) to the pinned
+// host buffer. While spinning it polls the stream now and then: a stream error is raised, and a
+// drained stream without the tags is an error rather than an endless wait.
+static void wait_tags(int nv, long long seq) {
+  const long long* t = ctx().tags;
+  auto arrived = [&] {
+    for (int i = 0; i < nv; ++i)
+      if (__atomic_load_n(&t[i], __ATOMIC_ACQUIRE) != seq) return false;
+    return true;
+  };
+  for (unsigned spin = 1;; ++spin) {
+    if (arrived()) return;
+    if ((spin & 1023) == 0) {
+      const hipError_t e = hipStreamQuery(stream());
+      if (e == hipSuccess) {
+        if (arrived()) return;
+        fail(DECO_ERR_DEVICE, "reduction values missing after the stream drained");
+      }
+      if (e != hipErrorNotReady) DECO_HIP(e);
+    }
   }
 }
 
@@ -541,7 +568,8 @@ static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, cons
     } else { A.apply(uu, z.p); M.apply(z.p, w.p); }
     ++napp; ++nspmv;
   };
-  Event ev(hipEventDisableTiming);
+  long long red_seq = 0;   // sequence number and value count of the last queued reduction
+  int red_nv = 0;
   while (iter < itmax) {
     std::fill(Hr.begin(), Hr.end(), 0.0);
     std::fill(Rm.begin(), Rm.end(), 0.0);
@@ -553,17 +581,19 @@ static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, cons
     double rn1 = -1.0, rn2 = -1.0;   // the last two residual estimates of this cycle
     // one reduction of step j: s = Q^T u, |u|^2 (and z = Q^T w, u.w with w); the update sweep
     // is queued behind it when w is there. The reduction writes the values to red (pinned host
-    // memory) itself, so the event behind it releases the host while the update runs; the host
-    // has read red before it queues the next reduction.
+    // memory) itself, each followed by its sequence tag, so the host reads them while the update
+    // runs; the host has read red before it queues the next reduction.
     auto reduce_step = [&](const double* uu, const double* wp, double* un) {
       const int nv = wp ? 2 * j + 2 : j + 1;
       hipLaunchKernelGGL(k_arn_dots, dim3(nbk), dim3(kArnBlock), sizeof(double) * 4 * (2 * j + 2), stream(), n, j,
                          (const double*)V.p, uu, wp, part.p, nbk);
       DECO_CHECK_LAUNCH();
+      if (nv > 4096) fail(DECO_ERR_ARG, "restart %d too large for the reduction buffers", m);
+      red_seq = ++ctx().tag_seq;
+      red_nv = nv;
       hipLaunchKernelGGL(k_arn_reduce, dim3(nv), dim3(256), 0, stream(), nbk, (const double*)part.p, redd.p,
-                         ctx().pinned_dev);
+                         ctx().pinned_dev, ctx().tags_dev, red_seq);
       DECO_CHECK_LAUNCH();
-      DECO_HIP(hipEventRecord(ev.e, stream()));
       if (wp) {   // queued before the host decision; unused if the cycle ends here
         hipLaunchKernelGGL(k_arn_update, dim3(nbk), dim3(kArnBlock), 0, stream(), n, j, (const double*)V.p, uu, wp,
                            (const double*)redd.p, V.p + (size_t)j * n, un);
@@ -612,7 +642,7 @@ static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, cons
       bool finalized = false;
       if (!last && !have_w && j > 0 && rn1 > 0.0 && rn2 > 0.0 && rn1 * (rn1 / rn2) < absTol) {
         reduce_step(u, nullptr, un);
-        DECO_HIP(hipEventSynchronize(ev.e));
+        wait_tags(red_nv, red_seq);
         rho = rho_of(false, gam);
         if (finalize(rho, false)) break;
         finalized = true;
@@ -631,7 +661,7 @@ static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, cons
       const bool likely_stop = rn1 > 0.0 && rn2 > 0.0 && rn1 * (rn1 / rn2) < absTol;
       have_w = wp && !next_last && !likely_stop;
       if (have_w) apply_op(j + 1, un);
-      DECO_HIP(hipEventSynchronize(ev.e));
+      wait_tags(red_nv, red_seq);
       rho = rho_of(wp != nullptr, gam);
       if (j == 0) {
         g[0] = rho;
