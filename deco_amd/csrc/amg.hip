@@ -131,7 +131,7 @@ struct EpiRestrictJacobi0 {
 
 // Up-sweep of a level with a Jacobi-type smoother as one product: with t = x + P e and the
 // residual r = b - A x kept from the down-sweep, t + (b - A t)/d = x + r/d + S e for
-// S = P - D^{-1} (A P). Writes x (and add_to[o*stride] += x when set).
+// S = P - D^{-1} (A P). Writes x, or with add_to set adds it to add_to[o*stride] instead.
 template <class T>
 struct EpiUp {
   T* __restrict__ x;
@@ -146,8 +146,8 @@ struct EpiUp {
   template <class AT>
   __device__ __forceinline__ void operator()(int o, AT acc, const Pre& p) const {
     const T y = (T)(p.x + p.r / p.d) + (T)acc;
-    x[o] = y;
-    if (add_to) add_to[(size_t)o * stride] = p.a + y;
+    if (add_to) add_to[(size_t)o * stride] = p.a + y;   // the caller's iterate takes the result
+    else x[o] = y;
   }
 };
 
@@ -203,7 +203,7 @@ static deco_csr* up_operator(const deco_csr* P, const deco_csr* Q, const double*
 // x = C b with C row-major n x n in MT (the coarsest inverse for the forward cycle, its
 // transpose for the transposed one; or the last level's dense sub-cycle operator): one wavefront
 // per row, lanes stride the columns, fixed-order butterfly sum; input/output in T, FP64
-// arithmetic; with add_to set, also add_to[i*stride] += x_i
+// arithmetic; with add_to set, add_to[i*stride] += x_i instead of storing x
 template <class T, class MT>
 __global__ __launch_bounds__(256) void k_dense(int n, const MT* __restrict__ C, const T* __restrict__ b,
                                                T* __restrict__ x, T* __restrict__ add_to, int stride) {
@@ -217,8 +217,8 @@ __global__ __launch_bounds__(256) void k_dense(int n, const MT* __restrict__ C, 
   s = wave_sum(s);
   if (lane == 0) {
     const T y = (T)s;
-    x[row] = y;
     if (add_to) add_to[(size_t)row * stride] += y;
+    else x[row] = y;
   }
 }
 
@@ -468,7 +468,8 @@ struct AMG : deco_prec {
   }
 
   // one V-cycle on level l with right-hand side b, result in x (x != b); x_ready: x already
-  // holds the zero-guess sweep; add_to: the last sweep also adds the result to add_to[stride*i]
+  // holds the zero-guess sweep; add_to: the last sweep adds the result to add_to[stride*i] (with
+  // a Jacobi-type smoother instead of storing it in x, which then holds the zero-guess sweep)
   template <class T>
   void cycle(size_t l, const T* b, T* x, bool tr, bool x_ready = false, T* add_to = nullptr, int stride = 0) {
     if (l == L.size()) {

@@ -38,7 +38,8 @@ struct deco_prec {
   // Pressure stage of CPR with two steps folded in (AMG only; false: not available, nothing
   // done). d0 receives the diagonal of the first zero-guess sweep x = b/d, which the caller's
   // residual epilogue then applies itself; stage() runs the rest of the cycle on (b, x) and adds
-  // the result to add_to[stride*i] from its last kernel.
+  // the result to add_to[stride*i] from its last kernel (x is work space: it does not receive
+  // the result).
   virtual bool stage_diag(const double** d0) { (void)d0; return false; }
   virtual bool stage_diag32(const float** d0) { (void)d0; return false; }
   virtual void stage(const double* b, double* x, double* add_to, int stride) {
