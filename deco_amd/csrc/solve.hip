@@ -84,6 +84,8 @@ constexpr int kArnRpt = 8;
 constexpr int kArnBlock = 256;
 constexpr int kArnTile = kArnRpt * kArnBlock;
 constexpr int kDcgs2MaxRestart = 1000;   // LDS of the dots kernel: (2 restart + 2) * 4 doubles
+// the reduction's values and tags live in pinned host buffers of 4096 entries each
+static_assert(2 * kDcgs2MaxRestart + 2 <= 4096, "DCGS2 reduction exceeds the pinned buffers");
 
 __global__ __launch_bounds__(kArnBlock) void k_arn_dots(int n, int j, const double* __restrict__ Q,
                                                        const double* __restrict__ u, const double* __restrict__ w,
@@ -588,7 +590,6 @@ static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, cons
       hipLaunchKernelGGL(k_arn_dots, dim3(nbk), dim3(kArnBlock), sizeof(double) * 4 * (2 * j + 2), stream(), n, j,
                          (const double*)V.p, uu, wp, part.p, nbk);
       DECO_CHECK_LAUNCH();
-      if (nv > 4096) fail(DECO_ERR_ARG, "restart %d too large for the reduction buffers", m);
       red_seq = ++ctx().tag_seq;
       red_nv = nv;
       hipLaunchKernelGGL(k_arn_reduce, dim3(nv), dim3(256), 0, stream(), nbk, (const double*)part.p, redd.p,
