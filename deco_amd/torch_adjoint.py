@@ -8,7 +8,12 @@ transform (lambda = W^T mu), and returns
     dL/dA_ij = -lambda_i x_j   on A's sparsity pattern (device sampled kernel)
     dL/db    =  lambda
 
-Tensors must be float64 on the GPU; the library works on their device memory directly.
+Tensors must be float64 on the GPU; the library works on their device memory directly. The
+library runs on its own stream: every tensor it reads is made (copied, cleared) on torch's stream
+before a torch.cuda.synchronize(), and the library's calls return after their own work is done.
+
+A forward or transposed solve that stops without reaching the tolerance (SOL::flag != 0,
+solver/SOL.hpp:10-13) raises NotConverged: its x or gradients are not returned.
 """
 import ctypes as C
 
@@ -21,6 +26,16 @@ from .core import cpr_params, sol_params, lib, PTree
 
 def _ptr(t):
     return C.c_void_p(t.data_ptr())
+
+
+class NotConverged(RuntimeError):
+    """A solve stopped at itmax before the relative residual reached reltol. `info` holds the
+    solve's SolInfo (iterations, residuals, flag)."""
+
+    def __init__(self, what, info):
+        super().__init__(f"{what} did not converge: flag {info.flag} after {info.iter} iterations, "
+                         f"relative residual {info.res_rel:.3e}")
+        self.info = info
 
 
 class SparsePattern:
@@ -79,16 +94,20 @@ def _solve(solver, values, b):
     L = lib()
     _check_tensor(values, "values", solver.pattern.nnz)
     _check_tensor(b, "b", solver.pattern.n)
+    # copies on torch's stream, finished before the library's stream reads them
+    vals = values.detach().contiguous()
+    bw = b.detach().clone().contiguous()
     torch.cuda.synchronize()
-    A = solver._csr(values.contiguous())
+    A = solver._csr(vals)
     M = C.c_void_p()
     try:
-        bw = b.detach().clone().contiguous()
         check(L.deco_cpr_create(C.byref(cpr_params(solver.prec, solver.precision)), A, _ptr(bw), C.byref(M)))
         x = torch.empty_like(bw)
         info = SolInfo()
         sp = sol_params(solver.sol)
         check(L.deco_solve(C.byref(sp), A, M, _ptr(bw), _ptr(x), C.byref(info)))
+        if info.flag != 0:
+            raise NotConverged("forward CPR solve", info)
     except Exception:
         if M.value:
             L.deco_prec_destroy(M.value)
@@ -114,24 +133,27 @@ class CPRSolve(torch.autograd.Function):
         A, M = ctx.handles.A, ctx.handles.M
         L = lib()
         solver = ctx.solver
+        p = solver.pattern
+        # every tensor the library reads is written on torch's stream before the synchronize
         g = grad_x.contiguous().to(torch.float64)
-        torch.cuda.synchronize()
+        zeros = torch.zeros(p.nnz, dtype=torch.float64, device=g.device)
         mu = torch.empty_like(g)
+        lam = torch.empty_like(g)
+        dA = torch.empty(p.nnz, dtype=torch.float64, device=g.device)
+        torch.cuda.synchronize()
         info = SolInfo()
         check(L.deco_solve_transpose(C.byref(sol_params(solver.sol)), A, M, _ptr(g), _ptr(mu), C.byref(info)))
-        lam = torch.empty_like(g)
+        solver.last_adjoint = info
+        if info.flag != 0:
+            raise NotConverged("transposed CPR solve", info)
         check(L.deco_cpr_transform_transpose(M, _ptr(mu), _ptr(lam)))
-        p = solver.pattern
         Pat = C.c_void_p()
-        zeros = torch.zeros(p.nnz, dtype=torch.float64, device=g.device)
         check(L.deco_csr_create_dev(p.n, p.n, p.nnz, _ptr(p.rowptr), _ptr(p.cols), _ptr(zeros), C.byref(Pat)))
         try:
-            dA = torch.empty(p.nnz, dtype=torch.float64, device=g.device)
             check(L.deco_grad_A_pattern(Pat.value, _ptr(lam), _ptr(x), _ptr(dA)))
         finally:
             L.deco_csr_destroy(Pat.value)
         ctx.handles = None
-        solver.last_adjoint = info
         return dA, lam, None
 
 

@@ -77,3 +77,27 @@ def test_adjoint_rejects_mistyped_tensors(gpu):
         cpr_solve(vals[:-1], b, solver)
     x = cpr_solve(vals, b, solver)
     assert x.shape == b.shape and solver.last.flag == 0
+
+
+def test_adjoint_raises_when_a_solve_does_not_converge(gpu):
+    """A forward solve that stops at itmax raises instead of returning x; so does a transposed
+    solve in the backward pass that stops at itmax (the forward one converged)."""
+    import torch
+    from deco_amd.torch_adjoint import SparsePattern, CPRSolver, cpr_solve, NotConverged
+
+    g = gen_system(4)
+    pat = SparsePattern(g["n"], g["r"], g["c"])
+    vals = torch.tensor(g["v"], dtype=torch.float64, device="cuda", requires_grad=True)
+    b = torch.tensor(g["b"], dtype=torch.float64, device="cuda")
+    tight = {"type": "rgmres", "reltol": "1e-12", "restart": "30", "itmax": "1", "verbosity": "0"}
+    solver = CPRSolver(pat, small_params(2), tight)
+    with pytest.raises(NotConverged) as e:
+        cpr_solve(vals, b, solver)
+    assert e.value.info.flag == 1 and e.value.info.iter == 1
+    solver = CPRSolver(pat, small_params(2), dict(tight, itmax="300"))
+    x = cpr_solve(vals, b, solver)
+    assert solver.last.flag == 0
+    solver.sol["itmax"] = "1"                       # the backward's transposed solve stops early
+    with pytest.raises(NotConverged) as e:
+        x.sum().backward()
+    assert e.value.info.flag == 1 and solver.last_adjoint.flag == 1
