@@ -77,14 +77,17 @@ def cpu_solve_baseline(g, M, prec_tree, sol, gpu_iters, args, log):
     bo = g["b"].copy()
     t0 = time.perf_counter()
     Mo = oracle.CPR(deco_amd.cpr_params(deco_amd.PTree(prec_tree)), Ao, bo)
-    levels, cinv = M.amg().export()
-    lv = [dict(A=oracle.Mat(l["n"], l["n"], *l["A"]), P=oracle.Mat(l["n"], l["nc"], *l["P"]),
-               R=oracle.Mat(l["nc"], l["n"], *l["R"]), d=l["d"], cf=l["cf"]) for l in levels]
-    Mo.set_local(oracle.amg(lv, 0, cinv))
+    # the oracle's own AMG setup (R, Galerkin products, drop rule, smoother diagonals, coarsest
+    # inverse) on the device hierarchy's interpolation operators and C/F splittings
+    levels, _ = M.amg().export()
+    p = deco_amd.cpr_params(deco_amd.PTree(prec_tree))
+    Mo.set_local(oracle.amg(Mo.pressure_matrix(), [oracle.Mat(l["n"], l["nc"], *l["P"]) for l in levels],
+                            [l["cf"] for l in levels], p.amg_relax_type, p.amg_A_drop_tol, p.amg_relax_weight))
     setup = time.perf_counter() - t0
     threads = oracle.num_threads()
     k = args.cpu_iters
-    desc = f"oracle CPR-{sol['type']}({sol['restart']}) on the same decoupled system and AMG hierarchy"
+    desc = (f"oracle CPR-{sol['type']}({sol['restart']}) on the same decoupled system, AMG on the device's "
+            f"interpolation operators")
     if k > 0:
         sp = deco_amd.sol_params(dict(sol, itmax=str(k)))
         t1 = time.perf_counter()

@@ -502,6 +502,15 @@ class AMG(Prec):
             check(L.deco_memcpy_d2h(_vp(ci), inv.value, 8 * ci.size))
         return levels, ci.reshape(nc.value, nc.value)
 
+    def coarse_matrix(self):
+        """Host copy (r, c, v) of the coarsest matrix."""
+        L = lib()
+        nl = C.c_int()
+        check(L.deco_amg_num_levels(self.h, C.byref(nl)))
+        a = C.c_void_p()
+        check(L.deco_amg_level(self.h, nl.value, C.byref(a), None, None, None, None))
+        return _view(a.value, self).download()
+
     def dense_tail(self):
         """Rows of the dense operator that replaces the last level's sub-cycle (0: none)."""
         rows = C.c_int()

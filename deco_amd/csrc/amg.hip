@@ -8,10 +8,18 @@
 // inverted densely in FP64 on the host and uploaded.
 //
 // Apply: one V-cycle from a zero initial guess (hypre_AMG.hpp:28-33), one sweep down and up.
-// Smoothers: relax 18 = l1-Jacobi (diagonal sum_j |a_ij|), 0/7 = Jacobi with amg_relax_weight,
-// -1 = damped Jacobi (weight 2/3) in place of hypre's hybrid Gauss-Seidel, 17 = F-C-F
-// Jacobi half-sweeps. Relaxation is fused into the stream SpMV epilogue. FP32 mode runs the
-// levels on f32 shadows and the coarsest solve in FP64.
+// Smoothers by amg_relax_type. The reference passes a type to hypre only when it is positive
+// (hypre_AMG.tpp:50-51), so -1 and 0 both mean hypre's default: Gauss-Seidel forward on the way
+// down and backward on the way up (hypre's l1 hybrid types 13/14). A sequential sweep has no
+// wavefront form, so the Gauss-Seidel types run as two-stage Gauss-Seidel (the form hypre itself
+// uses on GPUs, its types 11/12): x += g_s with rho = b - A x, g_0 = rho/d,
+// g_k = (rho - T g_{k-1})/d, T the strict lower (forward) or upper (backward) part, d = a_ii.
+//   -1, 0   forward down, backward up, s = 1      11, 13   forward both ways, s = 1
+//   12      forward both ways, s = 2              14       backward both ways, s = 1
+//   7       Jacobi with weight amg_relax_weight (1 when unset: weighted Jacobi is this key)
+//   17      F-C-F Jacobi half-sweeps              18       l1-Jacobi (diagonal sum_j |a_ij|)
+// Relaxation is fused into the stream SpMV epilogue. FP32 mode runs the levels on f32 shadows and
+// the coarsest solve in FP64.
 #include "prec.hpp"
 #include "amg_setup.hpp"
 #include <vector>
@@ -63,6 +71,82 @@ static deco_csr* drop_small(const deco_csr* A, double tol) {
   }
   return B;
 }
+
+// strict lower (lower != 0) or strict upper part of A, rows kept sorted
+__global__ void k_strict_count(int n, const int* __restrict__ r, const int* __restrict__ c, int lower,
+                               int* __restrict__ cnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int k = 0;
+  for (int j = r[i]; j < r[i + 1]; ++j) k += lower ? (c[j] < i) : (c[j] > i);
+  cnt[i] = k;
+}
+__global__ void k_strict_fill(int n, const int* __restrict__ r, const int* __restrict__ c,
+                              const double* __restrict__ v, int lower, const int* __restrict__ rn,
+                              int* __restrict__ cn, double* __restrict__ vn) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int o = rn[i];
+  for (int j = r[i]; j < r[i + 1]; ++j)
+    if (lower ? c[j] < i : c[j] > i) { cn[o] = c[j]; vn[o] = v[j]; ++o; }
+}
+
+static deco_csr* strict_part(const deco_csr* A, bool lower) {
+  deco_csr* B = csr_new(A->n, A->m);
+  DBuf<int> cnt(A->n + 1);
+  if (A->n) {
+    hipLaunchKernelGGL(k_strict_count, dim3(grid_for(A->n, 256)), dim3(256), 0, stream(), A->n, A->r.p, A->c.p,
+                       lower ? 1 : 0, cnt.p);
+    DECO_CHECK_LAUNCH();
+  }
+  build_row_ptr(B, cnt.p, A->n);
+  if (A->n) {
+    hipLaunchKernelGGL(k_strict_fill, dim3(grid_for(A->n, 256)), dim3(256), 0, stream(), A->n, A->r.p, A->c.p,
+                       A->v.p, lower ? 1 : 0, (const int*)B->r.p, B->c.p, B->v.p);
+    DECO_CHECK_LAUNCH();
+  }
+  return B;
+}
+
+// Residual of a two-stage Gauss-Seidel sweep: rho[o] = b[o] - (A x)[o] and g0[o] = rho[o]/d[o]
+template <class T>
+struct EpiResG {
+  const T* __restrict__ b;
+  const T* __restrict__ d;
+  T* __restrict__ rho;
+  T* __restrict__ g0;
+  struct Pre { T b, d; };
+  __device__ __forceinline__ Pre pre(int o) const { return Pre{b[o], d[o]}; }
+  template <class AT>
+  __device__ __forceinline__ void operator()(int o, AT acc, const Pre& p) const {
+    const T r = (T)((AT)p.b - acc);
+    rho[o] = r;
+    g0[o] = r / p.d;
+  }
+};
+
+// One inner step of a two-stage Gauss-Seidel sweep, the product being (T g)[o]:
+// y = x[o] + (rho[o] - (T g)[o])/d[o] (x NULL: 0 + ...); stored in out, or with add_to set added to
+// add_to[o*stride] (the finest level's last sweep of the fused CPR stage)
+template <class T>
+struct EpiTwoStage {
+  const T* __restrict__ x;
+  const T* __restrict__ rho;
+  const T* __restrict__ d;
+  T* __restrict__ out;
+  T* __restrict__ add_to;
+  int stride;
+  struct Pre { T x, rho, d, a; };
+  __device__ __forceinline__ Pre pre(int o) const {
+    return Pre{x ? x[o] : (T)0, rho[o], d[o], add_to ? add_to[(size_t)o * stride] : (T)0};
+  }
+  template <class AT>
+  __device__ __forceinline__ void operator()(int o, AT acc, const Pre& p) const {
+    const T y = p.x + (T)((AT)p.rho - acc) / p.d;
+    if (add_to) add_to[(size_t)o * stride] = p.a + y;
+    else out[o] = y;
+  }
+};
 
 // smoother diagonal: l1 (sum |a_ij|) or the diagonal entry divided by the Jacobi weight
 __global__ void k_smoother_diag(int n, const int* __restrict__ r, const int* __restrict__ c,
@@ -234,11 +318,15 @@ struct AMG : deco_prec {
     deco_csr* AP = nullptr;   // A P from the Galerkin product, until S is formed
     deco_csr* S = nullptr;    // up-sweep operator P - D^{-1} A P (Jacobi-type smoothers)
     deco_csr* St = nullptr;   // the same with A^T, for the transposed cycle
+    deco_csr* Lo = nullptr;   // strict lower / upper parts of A (two-stage Gauss-Seidel)
+    deco_csr* Up = nullptr;
+    deco_csr* Lot = nullptr;  // the same of A^T
+    deco_csr* Upt = nullptr;
     DBuf<double> d;
     DBuf<float> df;
     DBuf<int> cf;
-    DBuf<double> x, b, t, r;
-    DBuf<float> xf, bf, tf, rf;
+    DBuf<double> x, b, t, r, u;
+    DBuf<float> xf, bf, tf, rf, uf;
   };
   std::vector<Level> L;       // levels 0..nl-1 with interpolation; L.back() may be the coarsest
   deco_csr* Ac = nullptr;     // coarsest matrix
@@ -253,19 +341,27 @@ struct AMG : deco_prec {
   int tail = 0;
   DBuf<double> Z64, Z64t;
   DBuf<float> Z32, Z32t;
-  int relax = 18;
+  int relax = 18;              // 18 l1-Jacobi, 0 Jacobi (weighted), 17 F-C-F, 11 two-stage Gauss-Seidel
   double weight = 1.0;
+  int dir_down = 1, dir_up = -1, inner = 1;   // two-stage: +1 forward, -1 backward; inner steps
   bool have_t = false;
+  bool jacobi_type() const { return relax == 18 || relax == 0; }
 
   AMG(const deco_cpr_params& p, const deco_csr* A0) {
     kind = deco_prec::AMG;
     n = A0->n;
-    relax = p.amg_relax_type;
-    if (relax != -1 && relax != 0 && relax != 7 && relax != 17 && relax != 18)
-      fail(DECO_ERR_UNSUPPORTED, "amg_relax_type %d is not supported (use -1, 0, 7, 17 or 18)", p.amg_relax_type);
-    // -1 (hypre's CPU default, hybrid Gauss-Seidel) runs as damped Jacobi with weight 2/3
-    weight = p.amg_relax_weight > 0.0 ? p.amg_relax_weight : (relax == -1 ? 2.0 / 3.0 : 1.0);
-    if (relax == -1 || relax == 7) relax = 0;
+    switch (p.amg_relax_type) {
+      case 18: relax = 18; break;
+      case 17: relax = 17; break;
+      case 7: relax = 0; weight = p.amg_relax_weight > 0.0 ? p.amg_relax_weight : 1.0; break;
+      case -1: case 0: relax = 11; dir_down = 1; dir_up = -1; inner = 1; break;
+      case 11: case 13: relax = 11; dir_down = 1; dir_up = 1; inner = 1; break;
+      case 12: relax = 11; dir_down = 1; dir_up = 1; inner = 2; break;
+      case 14: relax = 11; dir_down = -1; dir_up = -1; inner = 1; break;
+      default:
+        fail(DECO_ERR_UNSUPPORTED, "amg_relax_type %d is not supported (use -1, 0, 7, 11-14, 17 or 18)",
+             p.amg_relax_type);
+    }
     if (p.amg_agg_levs > 0) fail(DECO_ERR_UNSUPPORTED, "aggressive coarsening (amg_agg_levs > 0) is not supported");
     if (A0->n != A0->m) fail(DECO_ERR_ARG, "AMG matrix is not square");
     const int max_coarse = p.amg_max_coarse > 0 ? p.amg_max_coarse : 300;
@@ -349,14 +445,20 @@ struct AMG : deco_prec {
       const int nl = lv.A->n;
       lv.d.alloc(nl);
       hipLaunchKernelGGL(k_smoother_diag, dim3(grid_for(nl, 256)), dim3(256), 0, stream(), nl, lv.A->r.p, lv.A->c.p,
-                         lv.A->v.p, l1, relax == 17 ? 1.0 : weight, lv.d.p);
+                         lv.A->v.p, l1, relax == 0 ? weight : 1.0, lv.d.p);
       DECO_CHECK_LAUNCH();
       lv.df.alloc(nl);
       convert(lv.d.p, lv.df.p, nl);
       lv.x.alloc(nl); lv.b.alloc(nl); lv.t.alloc(nl); lv.r.alloc(nl);
       lv.xf.alloc(nl); lv.bf.alloc(nl); lv.tf.alloc(nl); lv.rf.alloc(nl);
+      if (relax == 11) {
+        lv.Lo = strict_part(lv.A, true);
+        lv.Up = strict_part(lv.A, false);
+        lv.u.alloc(nl);
+        lv.uf.alloc(nl);
+      }
       // the up-sweep's prolongation and Jacobi-type sweep as one operator
-      if (relax != 17) lv.S = up_operator(lv.P, lv.AP, lv.d.p);
+      if (jacobi_type()) lv.S = up_operator(lv.P, lv.AP, lv.d.p);
       delete lv.AP;
       lv.AP = nullptr;
     }
@@ -401,20 +503,64 @@ struct AMG : deco_prec {
     A->v.download(H.v.data(), A->nnz);
   }
 
-  // Z = 2 D^{-1} - D^{-1} A D^{-1} + S C R (I - A D^{-1}) of the last level, formed on the host in
-  // FP64 and stored in FP64 and FP32 with its transpose (R = P^T, so the transposed sub-cycle with
-  // A^T, S^T's counterpart P - D^{-1} A^T P and C^T is Z^T)
+  // the smoother of the last level as a stationary step's matrix: D^{-1} (Jacobi-type), or the
+  // two-stage Neumann series sum_k N^k D^{-1}, N = -D^{-1} T, T the strict lower (dir > 0) or
+  // upper part, k = 0..inner
+  amg::HostCSR smoother_matrix(const amg::HostCSR& A, const std::vector<double>& d, int dir) const {
+    const int nl = A.n;
+    amg::HostCSR X;
+    X.n = X.m = nl;
+    X.r.resize(nl + 1);
+    for (int i = 0; i <= nl; ++i) X.r[i] = i;
+    X.c.resize(nl);
+    X.v.resize(nl);
+    for (int i = 0; i < nl; ++i) { X.c[i] = i; X.v[i] = 1.0 / d[i]; }
+    if (relax != 11) return X;
+    amg::HostCSR N;
+    N.n = N.m = nl;
+    N.r.assign(nl + 1, 0);
+    for (int i = 0; i < nl; ++i) {
+      for (int e = A.r[i]; e < A.r[i + 1]; ++e)
+        if (dir > 0 ? A.c[e] < i : A.c[e] > i) { N.c.push_back(A.c[e]); N.v.push_back(-A.v[e] / d[i]); }
+      N.r[i + 1] = (int)N.c.size();
+    }
+    amg::HostCSR term = X;
+    for (int k = 0; k < inner; ++k) {
+      term = amg::host_spgemm(N, term);
+      amg::HostCSR sum;                                // X + term, patterns merged
+      sum.n = sum.m = nl;
+      sum.r.assign(nl + 1, 0);
+      for (int i = 0; i < nl; ++i) {
+        int a = X.r[i], b = term.r[i];
+        const int ae = X.r[i + 1], be = term.r[i + 1];
+        while (a < ae || b < be) {
+          if (b >= be || (a < ae && X.c[a] < term.c[b])) { sum.c.push_back(X.c[a]); sum.v.push_back(X.v[a]); ++a; }
+          else if (a >= ae || term.c[b] < X.c[a]) { sum.c.push_back(term.c[b]); sum.v.push_back(term.v[b]); ++b; }
+          else { sum.c.push_back(X.c[a]); sum.v.push_back(X.v[a] + term.v[b]); ++a; ++b; }
+        }
+        sum.r[i + 1] = (int)sum.c.size();
+      }
+      X = std::move(sum);
+    }
+    return X;
+  }
+
+  // Z = Xpre + Xpost - Xpost A Xpre + (P - Xpost A P) C (R - R A Xpre) of the last level, formed on
+  // the host in FP64 and stored in FP64 and FP32 with its transpose (the transposed sub-cycle is
+  // Z^T: R = P^T, and the transposed cycle's smoothers are the transposes of the other side's)
   void form_tail(const std::vector<double>& inv) {
     ProfPhase ph("dense tail");
     Level& lv = L.back();
     const int nl = lv.A->n;
-    amg::HostCSR A, R, S;
+    amg::HostCSR A, P, R;
     download(lv.A, A);
+    download(lv.P, P);
     download(lv.R, R);
-    download(lv.S, S);
     std::vector<double> d(nl), Z, Zt((size_t)nl * nl);
     lv.d.download(d.data(), nl);
-    amg::dense_tail(A, d, R, S, inv, Z);
+    const amg::HostCSR Xpre = smoother_matrix(A, d, dir_down);
+    const amg::HostCSR Xpost = smoother_matrix(A, d, dir_up);
+    amg::dense_tail(A, Xpre, Xpost, P, R, inv, Z);
     for (int i = 0; i < nl; ++i)
       for (int k = 0; k < nl; ++k) Zt[(size_t)k * nl + i] = Z[(size_t)i * nl + k];
     std::vector<float> z32(Z.begin(), Z.end()), zt32(Zt.begin(), Zt.end());
@@ -443,7 +589,9 @@ struct AMG : deco_prec {
   void release() {
     for (auto& lv : L) {
       delete lv.A; delete lv.At; delete lv.P; delete lv.R; delete lv.AP; delete lv.S; delete lv.St;
+      delete lv.Lo; delete lv.Up; delete lv.Lot; delete lv.Upt;
       lv.A = lv.At = lv.P = lv.R = lv.AP = lv.S = lv.St = nullptr;
+      lv.Lo = lv.Up = lv.Lot = lv.Upt = nullptr;
     }
     L.clear();
     delete Ac;
@@ -456,6 +604,7 @@ struct AMG : deco_prec {
   template <class T> static T* B(Level& l);
   template <class T> static T* Tm(Level& l);
   template <class T> static T* Rr(Level& l);
+  template <class T> static T* Uu(Level& l);
   template <class T> static const T* D(Level& l);
   template <class T> static const T* vals(deco_csr* A);
   template <class T> T* XC();
@@ -481,6 +630,10 @@ struct AMG : deco_prec {
       dense(tail, tail_op(tr, b), b, x, add_to, stride);
       return;
     }
+    if (relax == 11) {
+      two_stage_level<T>(l, b, x, tr, x_ready, add_to, stride);
+      return;
+    }
     Level& lv = L[l];
     const deco_csr* A = tr ? lv.At : lv.A;
     const int nl = A->n;
@@ -499,15 +652,7 @@ struct AMG : deco_prec {
     // residual and restriction (the restriction also does the next level's zero-guess sweep
     // for the Jacobi-type smoothers)
     spmv_auto<T>(A, x, EpiStd<EPI_RES, T, T>{r, b});
-    const bool last = (l + 1 == L.size());
-    T* bn = last ? BC<T>() : B<T>(L[l + 1]);
-    T* xn = last ? XC<T>() : X<T>(L[l + 1]);
-    const bool fuse = !last && relax != 17 && !(tail && l + 2 == L.size());
-    if (fuse)
-      spmv_auto<T>(lv.R, (const T*)r, EpiRestrictJacobi0<T>{bn, D<T>(L[l + 1]), xn});
-    else
-      spmv_auto<T>(lv.R, (const T*)r, EpiStd<EPI_SET, T, T>{bn, nullptr});
-    cycle<T>(l + 1, bn, xn, tr, fuse);
+    T* xn = coarse_correction<T>(l, r, tr);
     if (deco_csr* S = tr ? lv.St : lv.S) {   // x <- x + r/d + S xn: prolongation and sweep in one product
       spmv_auto<T>(S, (const T*)xn, EpiUp<T>{x, (const T*)r, D<T>(lv), add_to, stride});
       return;
@@ -523,11 +668,77 @@ struct AMG : deco_prec {
     }
   }
 
+  // restriction of the residual r of level l, the cycle below, and the coarse solution's
+  // location; the restriction also writes the next level's zero-guess sweep (g_0 = b/d) unless
+  // that level is the coarsest, the dense tail or an F-C-F level
+  template <class T>
+  T* coarse_correction(size_t l, const T* r, bool tr) {
+    Level& lv = L[l];
+    const bool last = (l + 1 == L.size());
+    T* bn = last ? BC<T>() : B<T>(L[l + 1]);
+    T* xn = last ? XC<T>() : X<T>(L[l + 1]);
+    const bool fuse = !last && relax != 17 && !(tail && l + 2 == L.size());
+    if (fuse)
+      spmv_auto<T>(lv.R, r, EpiRestrictJacobi0<T>{bn, D<T>(L[l + 1]), xn});
+    else
+      spmv_auto<T>(lv.R, r, EpiStd<EPI_SET, T, T>{bn, nullptr});
+    cycle<T>(l + 1, bn, xn, tr, fuse);
+    return xn;
+  }
+
+  // strict part of the (transposed) level matrix for a two-stage sweep of direction dir
+  const deco_csr* strict(const Level& lv, bool tr, int dir) const {
+    return dir > 0 ? (tr ? lv.Lot : lv.Lo) : (tr ? lv.Upt : lv.Up);
+  }
+
+  // Level l with two-stage Gauss-Seidel sweeps. Down from a zero guess (rho = b): g_0 = b/d is in
+  // x (x_ready, or written here), the inner steps leave the sweep's result in t. Up: x = t + P e,
+  // rho = b - A x and g_0 = rho/d (into r and u), the inner steps add g_s to x (or, with add_to
+  // set, x + g_s to add_to[stride*i]). The transposed cycle runs on A^T with the other side's
+  // directions reversed.
+  template <class T>
+  void two_stage_level(size_t l, const T* b, T* x, bool tr, bool x_ready, T* add_to, int stride) {
+    Level& lv = L[l];
+    const deco_csr* A = tr ? lv.At : lv.A;
+    const int nl = A->n;
+    T* t = Tm<T>(lv);
+    T* r = Rr<T>(lv);
+    T* u = Uu<T>(lv);
+    const T* d = D<T>(lv);
+    const int down = tr ? -dir_up : dir_down, up = tr ? -dir_down : dir_up;
+    if (!x_ready) {
+      hipLaunchKernelGGL(k_relax_zero<T>, dim3(grid_for(nl, 256)), dim3(256), 0, stream(), nl, b, d, lv.cf.p, -1, x);
+      DECO_CHECK_LAUNCH();
+    }
+    const deco_csr* Td = strict(lv, tr, down);
+    if (inner == 1) {
+      spmv_auto<T>(Td, (const T*)x, EpiTwoStage<T>{nullptr, b, d, t, nullptr, 0});
+    } else {
+      spmv_auto<T>(Td, (const T*)x, EpiTwoStage<T>{nullptr, b, d, u, nullptr, 0});
+      spmv_auto<T>(Td, (const T*)u, EpiTwoStage<T>{nullptr, b, d, t, nullptr, 0});
+    }
+    spmv_auto<T>(A, (const T*)t, EpiStd<EPI_RES, T, T>{r, b});
+    const T* xn = coarse_correction<T>(l, r, tr);
+    spmv_auto<T>(lv.P, xn, EpiStd<EPI_ADDZ, T, T>{x, t});
+    spmv_auto<T>(A, (const T*)x, EpiResG<T>{b, d, r, u});
+    const deco_csr* Tu = strict(lv, tr, up);
+    if (inner == 1) {
+      spmv_auto<T>(Tu, (const T*)u, EpiTwoStage<T>{x, r, d, x, add_to, stride});
+    } else {
+      spmv_auto<T>(Tu, (const T*)u, EpiTwoStage<T>{nullptr, r, d, t, nullptr, 0});
+      spmv_auto<T>(Tu, (const T*)t, EpiTwoStage<T>{x, r, d, x, add_to, stride});
+    }
+  }
+
   void ensure_t() {
     if (have_t) return;
     for (auto& lv : L) {
       lv.At = csr_transpose(lv.A);
-      if (relax != 17) {
+      if (relax == 11) {
+        lv.Lot = strict_part(lv.At, true);
+        lv.Upt = strict_part(lv.At, false);
+      }
+      if (jacobi_type()) {
         deco_csr* Q = spgemm(lv.At, lv.P);
         lv.St = up_operator(lv.P, Q, lv.d.p);
         delete Q;
@@ -599,6 +810,8 @@ template <> double* AMG::Tm<double>(Level& l) { return l.t.p; }
 template <> float* AMG::Tm<float>(Level& l) { return l.tf.p; }
 template <> double* AMG::Rr<double>(Level& l) { return l.r.p; }
 template <> float* AMG::Rr<float>(Level& l) { return l.rf.p; }
+template <> double* AMG::Uu<double>(Level& l) { return l.u.p; }
+template <> float* AMG::Uu<float>(Level& l) { return l.uf.p; }
 template <> const double* AMG::D<double>(Level& l) { return l.d.p; }
 template <> const float* AMG::D<float>(Level& l) { return l.df.p; }
 template <> const double* AMG::vals<double>(deco_csr* A) { return A->v.p; }

@@ -381,20 +381,49 @@ bool dense_inverse(const HostCSR& A, std::vector<double>& inv) {
   return true;
 }
 
-void dense_tail(const HostCSR& A, const std::vector<double>& d, const HostCSR& R, const HostCSR& S,
+HostCSR host_spgemm(const HostCSR& A, const HostCSR& B) {
+  HostCSR C;
+  C.n = A.n;
+  C.m = B.m;
+  C.r.assign(A.n + 1, 0);
+  std::vector<double> acc(B.m, 0.0);
+  std::vector<char> on(B.m, 0);
+  std::vector<int> cols;
+  for (int i = 0; i < A.n; ++i) {
+    cols.clear();
+    for (int e = A.r[i]; e < A.r[i + 1]; ++e) {
+      const int k = A.c[e];
+      const double a = A.v[e];
+      for (int f = B.r[k]; f < B.r[k + 1]; ++f) {
+        const int j = B.c[f];
+        if (!on[j]) { on[j] = 1; cols.push_back(j); acc[j] = 0.0; }
+        acc[j] += a * B.v[f];
+      }
+    }
+    std::sort(cols.begin(), cols.end());
+    for (int j : cols) {
+      C.c.push_back(j);
+      C.v.push_back(acc[j]);
+      on[j] = 0;
+    }
+    C.r[i + 1] = (int)C.c.size();
+  }
+  return C;
+}
+
+void dense_tail(const HostCSR& A, const HostCSR& Xpre, const HostCSR& Xpost, const HostCSR& P, const HostCSR& R,
                 const std::vector<double>& C, std::vector<double>& Z) {
   const int n = A.n, nc = R.n;
-  // G = R (I - A D^{-1}): the restriction of the residual left by the zero-guess sweep
+  const HostCSR AX = host_spgemm(A, Xpre);        // A Xpre
+  const HostCSR RAX = host_spgemm(R, AX);         // R A Xpre
+  const HostCSR XAX = host_spgemm(Xpost, AX);     // Xpost A Xpre
+  const HostCSR XAP = host_spgemm(Xpost, host_spgemm(A, P));   // Xpost A P
+  // G = R - R A Xpre (nc x n): the restriction of the residual the pre-smoother leaves
   std::vector<double> G((size_t)nc * n, 0.0);
-#pragma omp parallel for schedule(dynamic, 8)
   for (int i = 0; i < nc; ++i) {
     double* g = &G[(size_t)i * n];
-    for (int e = R.r[i]; e < R.r[i + 1]; ++e) {
-      const int k = R.c[e];
-      const double w = R.v[e];
-      g[k] += w;
-      for (int f = A.r[k]; f < A.r[k + 1]; ++f) g[A.c[f]] -= w * (A.v[f] / d[A.c[f]]);
-    }
+    for (int e = R.r[i]; e < R.r[i + 1]; ++e) g[R.c[e]] += R.v[e];
+    for (int e = RAX.r[i]; e < RAX.r[i + 1]; ++e) g[RAX.c[e]] -= RAX.v[e];
   }
   // H = C G: the coarse correction
   std::vector<double> H((size_t)nc * n, 0.0);
@@ -407,18 +436,24 @@ void dense_tail(const HostCSR& A, const std::vector<double>& d, const HostCSR& R
       for (int j = 0; j < n; ++j) h[j] += c * g[j];
     }
   }
-  // Z = S H + 2 D^{-1} - D^{-1} A D^{-1}
+  // Z = (P - Xpost A P) H + Xpre + Xpost - Xpost A Xpre
   Z.assign((size_t)n * n, 0.0);
 #pragma omp parallel for schedule(dynamic, 16)
   for (int i = 0; i < n; ++i) {
     double* z = &Z[(size_t)i * n];
-    for (int e = S.r[i]; e < S.r[i + 1]; ++e) {
-      const double s = S.v[e];
-      const double* h = &H[(size_t)S.c[e] * n];
+    for (int e = P.r[i]; e < P.r[i + 1]; ++e) {
+      const double s = P.v[e];
+      const double* h = &H[(size_t)P.c[e] * n];
       for (int j = 0; j < n; ++j) z[j] += s * h[j];
     }
-    for (int f = A.r[i]; f < A.r[i + 1]; ++f) z[A.c[f]] -= A.v[f] / (d[i] * d[A.c[f]]);
-    z[i] += 2.0 / d[i];
+    for (int e = XAP.r[i]; e < XAP.r[i + 1]; ++e) {
+      const double s = XAP.v[e];
+      const double* h = &H[(size_t)XAP.c[e] * n];
+      for (int j = 0; j < n; ++j) z[j] -= s * h[j];
+    }
+    for (int e = Xpre.r[i]; e < Xpre.r[i + 1]; ++e) z[Xpre.c[e]] += Xpre.v[e];
+    for (int e = Xpost.r[i]; e < Xpost.r[i + 1]; ++e) z[Xpost.c[e]] += Xpost.v[e];
+    for (int e = XAX.r[i]; e < XAX.r[i + 1]; ++e) z[XAX.c[e]] -= XAX.v[e];
   }
 }
 

@@ -96,9 +96,12 @@ typedef struct deco_cpr_params {
 
   double loc_tol_rel;       /* prec.prec_loc.tol_rel (filter_proper of Ap)      */
   int    amg_coarse_type;   /* prec.prec_loc.amg_coarse_type: 8 PMIS, 10 HMIS   */
-  int    amg_relax_type;    /* prec.prec_loc.amg_relax_type: -1, 0, 7, 17, 18
-                               (-1: damped Jacobi, the parallel stand-in for hypre's
-                               hybrid Gauss-Seidel default)                    */
+  int    amg_relax_type;    /* prec.prec_loc.amg_relax_type. As in the reference only a
+                               positive value names a type (hypre_AMG.tpp:50-51): -1 and 0 mean
+                               hypre's default, Gauss-Seidel forward down / backward up, run as
+                               two-stage Gauss-Seidel (one inner step); 11/13 forward, 12 forward
+                               with two inner steps, 14 backward; 7 Jacobi (weight
+                               amg_relax_weight); 17 F-C-F Jacobi; 18 l1-Jacobi */
   int    amg_agg_levs;      /* prec.prec_loc.amg_agg_levs (only 0 supported)    */
   int    amg_agg_path;      /* prec.prec_loc.amg_agg_path (unused when 0 levels)*/
   double amg_A_drop_tol;    /* prec.prec_loc.amg_A_drop_tol                     */
@@ -110,7 +113,7 @@ typedef struct deco_cpr_params {
   int    amg_P_max_elmts;   /* default 4                                        */
   double amg_max_row_sum;   /* default 0.9                                      */
   int    precision;         /* DECO_PREC_FP64 / DECO_PREC_FP32                  */
-  double amg_relax_weight;  /* Jacobi weight; 0 = default (1 for relax 0/7, 2/3 for -1) */
+  double amg_relax_weight;  /* weight of the Jacobi smoother (type 7); 0 = 1             */
   int    layout;            /* DECO_LAYOUT_*; 0 (BLOCK) in a zero-initialised struct */
 } deco_cpr_params;
 

@@ -59,7 +59,10 @@ def lib():
             "orc_jacobi_create": (i, [i, P(OrcCSR), P(vp)]),
             "orc_dense_create": (i, [P(OrcCSR), P(vp)]),
             "orc_identity_create": (i, [i, P(vp)]),
-            "orc_amg_create": (i, [i, P(OrcCSR), P(OrcCSR), P(OrcCSR), P(vp), P(vp), i, i, vp, P(vp)]),
+            "orc_amg_build": (i, [P(OrcCSR), i, P(OrcCSR), P(vp), i, d, d, P(vp)]),
+            "orc_amg_num_levels": (i, [vp, P(i), P(i)]),
+            "orc_amg_level": (i, [vp, i, P(OrcCSR), P(OrcCSR), vp]),
+            "orc_amg_coarse_inverse": (i, [vp, vp]),
             "orc_cpr_create": (i, [P(CprParams), P(OrcCSR), vp, P(vp)]),
             "orc_cpr_pressure_matrix": (i, [vp, P(OrcCSR)]),
             "orc_cpr_set_local": (i, [vp, vp]),
@@ -310,22 +313,49 @@ def identity(n):
     return Prec(h.value, n)
 
 
-def amg(levels, relax_type, coarse_inv):
-    """levels: list of dicts with A, P, R (Mat), d (float64), cf (int32); coarse_inv (nc x nc)."""
-    L = len(levels)
-    As = (OrcCSR * max(L, 1))(*[lv["A"].view() for lv in levels])
-    Ps = (OrcCSR * max(L, 1))(*[lv["P"].view() for lv in levels])
-    Rs = (OrcCSR * max(L, 1))(*[lv["R"].view() for lv in levels])
-    ds = [np.ascontiguousarray(lv["d"], np.float64) for lv in levels]
-    cfs = [np.ascontiguousarray(lv["cf"], np.int32) for lv in levels]
-    dp = (C.c_void_p * max(L, 1))(*[_vp(a) for a in ds])
-    cp = (C.c_void_p * max(L, 1))(*[_vp(a) for a in cfs])
-    ci = np.ascontiguousarray(coarse_inv, np.float64)
-    nc = ci.shape[0]
-    h = C.c_void_p()
-    _check(lib().orc_amg_create(L, As, Ps, Rs, dp, cp, relax_type, nc, _vp(ci), C.byref(h)))
-    n = levels[0]["A"].n if L else nc
-    return Prec(h.value, n)
+class AMG(Prec):
+    """The oracle's AMG pressure stage: from the finest matrix A0 (Mat) and the interpolation
+    operators P_l (Mat, n_l x n_{l+1}) and C/F markers cf_l of levels 0..L-1 it forms R_l = P_l^T,
+    the Galerkin products with the drop rule, the smoother diagonals and the coarsest inverse
+    itself (orc_amg_build); apply = one V-cycle from zero."""
+
+    def __init__(self, A0, Ps, cfs, relax_type, drop_tol, relax_weight=0.0):
+        L = len(Ps)
+        self._cfs = [np.ascontiguousarray(cf, np.int32) for cf in cfs]
+        Pv = (OrcCSR * max(L, 1))(*[P.view() for P in Ps])
+        cp = (C.c_void_p * max(L, 1))(*[_vp(a) for a in self._cfs])
+        h = C.c_void_p()
+        _check(lib().orc_amg_build(C.byref(A0.view()), L, Pv, cp, int(relax_type), float(relax_weight),
+                                   float(drop_tol), C.byref(h)))
+        super().__init__(h.value, A0.n)
+        nl, nc = C.c_int(), C.c_int()
+        _check(lib().orc_amg_num_levels(self.h, C.byref(nl), C.byref(nc)))
+        self.nlevels, self.nc = nl.value, nc.value
+
+    def level(self, l):
+        """(A_l, R_l, d_l) of level l < nlevels; the coarsest matrix for l == nlevels."""
+        A, R = OrcCSR(), OrcCSR()
+        if l == self.nlevels:
+            _check(lib().orc_amg_level(self.h, l, C.byref(A), None, None))
+            return Mat.take(A), None, None
+        nl = self.level_rows(l)
+        d = np.empty(nl)
+        _check(lib().orc_amg_level(self.h, l, C.byref(A), C.byref(R), _vp(d)))
+        return Mat.take(A), Mat.take(R), d
+
+    def level_rows(self, l):
+        A = OrcCSR()
+        _check(lib().orc_amg_level(self.h, l, C.byref(A), None, None))
+        return Mat.take(A).n
+
+    def coarse_inverse(self):
+        ci = np.empty(self.nc * self.nc)
+        _check(lib().orc_amg_coarse_inverse(self.h, _vp(ci)))
+        return ci.reshape(self.nc, self.nc)
+
+
+def amg(A0, Ps, cfs, relax_type, drop_tol, relax_weight=0.0):
+    return AMG(A0, Ps, cfs, relax_type, drop_tol, relax_weight)
 
 
 class CPR(Prec):

@@ -13,9 +13,12 @@
 //       (FL A)[i,P_i] = 0, (A FU^T)[P_i,i] = 0, unit diagonal, closed forms)
 //   rgmres/lgmres/bicgstab/lbicgstab/pcg                                  pinned (exact on tiny
 //       systems, Arnoldi relation, scipy solutions)
-//   AMG V-cycle on an imported hierarchy                                  pinned (two-grid
-//       identities, exact coarse solve); the hierarchy itself follows no reference code
-//       (hypre is absent): parity with hypre BoomerAMG is unpinned.
+//   AMG: R = P^T, Galerkin RAP, drop rule, smoother diagonals, coarsest   pinned (scipy RAP,
+//       inverse and the V-cycle with every smoother (l1-Jacobi, Jacobi,    numpy inverse, dense
+//       F-C-F, two-stage Gauss-Seidel) on imported P_l and C/F splittings    multi-level closed
+//                                                                          forms)
+//       The interpolation and coarsening follow no reference code (hypre is absent): parity of
+//       the hierarchy with hypre BoomerAMG is unpinned.
 #include "oracle.h"
 #include <omp.h>
 #include <cmath>
@@ -564,18 +567,151 @@ struct Dense : Prec {
   }
 };
 
-// AMG V-cycle on an imported hierarchy (one V-cycle from x = 0, hypre_AMG.hpp:25-37).
+// AMG pressure stage (the role of hypre_AMG, precond/hypre_AMG/hypre_AMG.hpp:16-37 and
+// hypre_AMG.tpp:44-67). hypre (v2.31.0) is absent, so the C/F splittings and interpolation
+// operators P_l are imported (they are the product's host setup, which follows no reference
+// code); everything else of the hierarchy is formed here from the finest matrix: R_l = P_l^T,
+// the Galerkin product A_{l+1} = R_l (A_l P_l) with MxM, the drop rule of amg_A_drop_tol, the
+// smoother diagonals and the coarsest inverse (dgetf2/dgetrs on unit columns).
+//
+// Apply: one V-cycle from x = 0 (hypre_AMG.hpp:28-33, hypre_AMG.tpp:67), one sweep down and up.
+// Smoothers by amg_relax_type (the reference passes a type to hypre only when it is positive,
+// hypre_AMG.tpp:50-51, so -1 and 0 both mean hypre's default, Gauss-Seidel down and up):
+//   -1, 0  two-stage Gauss-Seidel: forward (strict lower part) down, backward (strict upper) up
+//   11, 13 two-stage Gauss-Seidel, forward both ways;  14: backward both ways
+//   12     two-stage Gauss-Seidel with two inner iterations, forward both ways
+//   7      Jacobi with weight amg_relax_weight (1 when unset)
+//   17     F-C-F Jacobi half-sweeps;  18: l1-Jacobi
+// A two-stage sweep with s inner iterations on x (Neumann series of (D + T)^{-1}, T the strict
+// lower or upper part): rho = b - A x, g_0 = rho / d, g_k = (rho - T g_{k-1}) / d, x += g_s.
 struct AMG : Prec {
   struct Level {
     CSR A, At, P, R;
+    CSR Lo, Up, Lot, Upt;      // strict lower / upper parts of A and of A^T (two-stage sweeps)
     std::vector<double> d;
     std::vector<int> cf;
-    std::vector<double> x, b, r, t;
+    std::vector<double> x, b, r, t, g;
   };
+  enum Kind { L1JAC, JAC, FCF, TWOSTAGE };
   std::vector<Level> L;
-  int relax = 18, nc = 0;
+  CSR Ac;                      // coarsest matrix
+  Kind kind = L1JAC;
+  double weight = 1.0;
+  int dir_down = 1, dir_up = -1, inner = 1;   // two-stage: +1 forward, -1 backward
+  int nc = 0;
   std::vector<double> Cinv, xc, bc;
   bool have_t = false;
+
+  void set_relax(int relax, double w) {
+    weight = 1.0;
+    switch (relax) {
+      case 18: kind = L1JAC; break;
+      case 7: kind = JAC; weight = w > 0.0 ? w : 1.0; break;
+      case 17: kind = FCF; break;
+      case -1: case 0: kind = TWOSTAGE; dir_down = 1; dir_up = -1; inner = 1; break;
+      case 11: case 13: kind = TWOSTAGE; dir_down = 1; dir_up = 1; inner = 1; break;
+      case 12: kind = TWOSTAGE; dir_down = 1; dir_up = 1; inner = 2; break;
+      case 14: kind = TWOSTAGE; dir_down = -1; dir_up = -1; inner = 1; break;
+      default: throw std::runtime_error("amg_relax_type " + std::to_string(relax) + " not supported");
+    }
+  }
+
+  // keep a_ij if j == i or |a_ij| >= tol * max_k |a_ik| (hypre's A-drop with the row's max norm)
+  static CSR drop_small(const CSR& A, double tol) {
+    CSR B;
+    B.n = A.n;
+    B.m = A.m;
+    B.r.assign(A.n + 1, 0);
+    for (int i = 0; i < A.n; ++i) {
+      double mx = 0.0;
+      for (int j = A.r[i]; j < A.r[i + 1]; ++j) mx = std::max(mx, std::fabs(A.v[j]));
+      const double thr = tol * mx;
+      for (int j = A.r[i]; j < A.r[i + 1]; ++j)
+        if (A.c[j] == i || std::fabs(A.v[j]) >= thr) {
+          B.c.push_back(A.c[j]);
+          B.v.push_back(A.v[j]);
+        }
+      B.r[i + 1] = B.nnz();
+    }
+    return B;
+  }
+  // strict lower (lower = true) or strict upper part, rows kept sorted
+  static CSR strict_part(const CSR& A, bool lower) {
+    CSR B;
+    B.n = A.n;
+    B.m = A.m;
+    B.r.assign(A.n + 1, 0);
+    for (int i = 0; i < A.n; ++i) {
+      for (int j = A.r[i]; j < A.r[i + 1]; ++j)
+        if (lower ? A.c[j] < i : A.c[j] > i) {
+          B.c.push_back(A.c[j]);
+          B.v.push_back(A.v[j]);
+        }
+      B.r[i + 1] = B.nnz();
+    }
+    return B;
+  }
+  void smoother_diag(Level& lv) const {
+    const CSR& A = lv.A;
+    lv.d.assign(A.n, 0.0);
+    for (int i = 0; i < A.n; ++i) {
+      double s = 0.0;
+      for (int j = A.r[i]; j < A.r[i + 1]; ++j) {
+        if (kind == L1JAC) s += std::fabs(A.v[j]);
+        else if (A.c[j] == i) s += A.v[j];
+      }
+      lv.d[i] = kind == L1JAC ? s : s / weight;
+    }
+  }
+
+  // hierarchy from the finest matrix and the imported P_l, cf_l
+  void build(const CSR& A0, int nlev, const orc_csr* P, const int* const* cf, double drop_tol) {
+    n = A0.n;
+    CSR A = A0;
+    L.resize(nlev);
+    for (int l = 0; l < nlev; ++l) {
+      Level& lv = L[l];
+      lv.P = from_c(&P[l]);
+      if (lv.P.n != A.n) throw std::runtime_error("P_" + std::to_string(l) + " does not match A_l's rows");
+      lv.cf.assign(cf[l], cf[l] + A.n);
+      lv.R = transpose(lv.P);
+      CSR AP = spgemm(A, lv.P);
+      CSR RAP = spgemm(lv.R, AP);
+      lv.A = std::move(A);
+      A = drop_tol > 0.0 ? drop_small(RAP, drop_tol) : std::move(RAP);
+    }
+    Ac = std::move(A);
+    nc = Ac.n;
+    for (auto& lv : L) {
+      smoother_diag(lv);
+      if (kind == TWOSTAGE) {
+        lv.Lo = strict_part(lv.A, true);
+        lv.Up = strict_part(lv.A, false);
+      }
+      const int nl = lv.A.n;
+      lv.x.assign(nl, 0.0);
+      lv.b.assign(nl, 0.0);
+      lv.r.assign(nl, 0.0);
+      lv.t.assign(nl, 0.0);
+      lv.g.assign(nl, 0.0);
+    }
+    // coarsest inverse, row-major: column k solves Ac y = e_k
+    std::vector<double> lu((size_t)nc * nc, 0.0);
+    for (int i = 0; i < nc; ++i)
+      for (int j = Ac.r[i]; j < Ac.r[i + 1]; ++j) lu[i + (size_t)Ac.c[j] * nc] = Ac.v[j];
+    std::vector<int> ipiv(nc);
+    if (nc && lu_factor(nc, lu.data(), ipiv.data()) != 0) throw std::runtime_error("singular coarsest matrix");
+    Cinv.assign((size_t)nc * nc, 0.0);
+    std::vector<double> col(nc);
+    for (int k = 0; k < nc; ++k) {
+      std::fill(col.begin(), col.end(), 0.0);
+      col[k] = 1.0;
+      lu_solve(nc, lu.data(), ipiv.data(), 0, col.data());
+      for (int i = 0; i < nc; ++i) Cinv[(size_t)i * nc + k] = col[i];
+    }
+    xc.assign(nc, 0.0);
+    bc.assign(nc, 0.0);
+  }
 
   // One relaxation sweep x <- x + D^{-1}(b - A x) on the rows of `subset`
   // (-1 all, 1 C points, 0 F points); out of place through t.
@@ -591,9 +727,38 @@ struct AMG : Prec {
     }
     lv.x.swap(lv.t);
   }
-  void relax_sweep(Level& lv, const CSR& A) {
-    if (relax == 17) { half_sweep(lv, A, 0); half_sweep(lv, A, 1); half_sweep(lv, A, 0); }
-    else half_sweep(lv, A, -1);
+  // two-stage Gauss-Seidel sweep on x with T (strict lower or upper part of A)
+  void two_stage(Level& lv, const CSR& A, const CSR& T) {
+    const int nn = A.n;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nn; ++i) {
+      double s = 0.0;
+      for (int j = A.r[i]; j < A.r[i + 1]; ++j) s += A.v[j] * lv.x[A.c[j]];
+      lv.r[i] = lv.b[i] - s;                 // rho
+      lv.g[i] = lv.r[i] / lv.d[i];           // g_0
+    }
+    for (int k = 0; k < inner; ++k) {
+#pragma omp parallel for schedule(static)
+      for (int i = 0; i < nn; ++i) {
+        double s = 0.0;
+        for (int j = T.r[i]; j < T.r[i + 1]; ++j) s += T.v[j] * lv.g[T.c[j]];
+        lv.t[i] = (lv.r[i] - s) / lv.d[i];   // g_{k+1}
+      }
+      lv.g.swap(lv.t);
+    }
+    for (int i = 0; i < nn; ++i) lv.x[i] += lv.g[i];
+  }
+  // pre (down = true) or post smoother; the transposed cycle is the cycle on A^T with the
+  // sweep directions of the other side reversed (its pre-smoother is the transpose of the
+  // forward cycle's post-smoother)
+  void smooth(Level& lv, bool tr, bool down) {
+    const CSR& A = tr ? lv.At : lv.A;
+    if (kind == FCF) { half_sweep(lv, A, 0); half_sweep(lv, A, 1); half_sweep(lv, A, 0); return; }
+    if (kind != TWOSTAGE) { half_sweep(lv, A, -1); return; }
+    int dir = down ? dir_down : dir_up;
+    if (tr) dir = -(down ? dir_up : dir_down);
+    const CSR& T = dir > 0 ? (tr ? lv.Lot : lv.Lo) : (tr ? lv.Upt : lv.Up);
+    two_stage(lv, A, T);
   }
   void cycle(size_t l, bool tr) {
     if (l == L.size()) {   // coarsest: dense inverse, row-major
@@ -607,7 +772,7 @@ struct AMG : Prec {
     Level& lv = L[l];
     const CSR& A = tr ? lv.At : lv.A;
     std::fill(lv.x.begin(), lv.x.end(), 0.0);
-    relax_sweep(lv, A);
+    smooth(lv, tr, true);
     // r = b - A x
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < A.n; ++i) {
@@ -625,7 +790,7 @@ struct AMG : Prec {
       for (int j = lv.P.r[i]; j < lv.P.r[i + 1]; ++j) s += lv.P.v[j] * xnext[lv.P.c[j]];
       lv.x[i] += s;
     }
-    relax_sweep(lv, A);
+    smooth(lv, tr, false);
   }
   void run(const double* r, double* x, bool tr) {
     if (L.empty()) {
@@ -635,7 +800,13 @@ struct AMG : Prec {
       return;
     }
     if (tr && !have_t) {
-      for (auto& lv : L) lv.At = transpose(lv.A);
+      for (auto& lv : L) {
+        lv.At = transpose(lv.A);
+        if (kind == TWOSTAGE) {
+          lv.Lot = strict_part(lv.At, true);
+          lv.Upt = strict_part(lv.At, false);
+        }
+      }
       have_t = true;
     }
     std::memcpy(L[0].b.data(), r, sizeof(double) * n);
@@ -1414,30 +1585,47 @@ int orc_dense_create(const orc_csr* A, orc_prec** out) {
 int orc_identity_create(int n, orc_prec** out) {
   return guard([&] { *out = new orc_prec{std::unique_ptr<Prec>(new Identity(n))}; });
 }
-int orc_amg_create(int nlev, const orc_csr* A, const orc_csr* P, const orc_csr* R, const double* const* d,
-                   const int* const* cf, int relax_type, int nc, const double* coarse_inv, orc_prec** out) {
+int orc_amg_build(const orc_csr* A0, int nlev, const orc_csr* P, const int* const* cf, int relax_type,
+                  double relax_weight, double drop_tol, orc_prec** out) {
   return guard([&] {
+    if (!A0 || nlev < 0 || (nlev > 0 && (!P || !cf)) || !out) throw std::invalid_argument("null argument");
     auto* G = new AMG();
-    G->relax = relax_type;
-    G->nc = nc;
-    G->n = nlev > 0 ? A[0].n : nc;
-    G->L.resize(nlev);
-    for (int l = 0; l < nlev; ++l) {
-      auto& lv = G->L[l];
-      lv.A = from_c(&A[l]);
-      lv.P = from_c(&P[l]);
-      lv.R = from_c(&R[l]);
-      lv.d.assign(d[l], d[l] + lv.A.n);
-      lv.cf.assign(cf[l], cf[l] + lv.A.n);
-      lv.x.assign(lv.A.n, 0.0);
-      lv.b.assign(lv.A.n, 0.0);
-      lv.r.assign(lv.A.n, 0.0);
-      lv.t.assign(lv.A.n, 0.0);
+    std::unique_ptr<Prec> own(G);
+    G->set_relax(relax_type, relax_weight);
+    G->build(from_c(A0), nlev, P, cf, drop_tol);
+    *out = new orc_prec{std::move(own)};
+  });
+}
+static AMG* as_amg(const orc_prec* M) {
+  auto* G = dynamic_cast<AMG*>(M->impl.get());
+  if (!G) throw std::invalid_argument("not an AMG preconditioner");
+  return G;
+}
+int orc_amg_num_levels(const orc_prec* M, int* nlev, int* nc) {
+  return guard([&] {
+    AMG* G = as_amg(M);
+    if (nlev) *nlev = (int)G->L.size();
+    if (nc) *nc = G->nc;
+  });
+}
+int orc_amg_level(const orc_prec* M, int level, orc_csr* A, orc_csr* R, double* d) {
+  return guard([&] {
+    AMG* G = as_amg(M);
+    if (level < 0 || level > (int)G->L.size()) throw std::invalid_argument("level out of range");
+    if (level == (int)G->L.size()) {
+      if (A) to_c(G->Ac, A);
+      return;
     }
-    G->Cinv.assign(coarse_inv, coarse_inv + (size_t)nc * nc);
-    G->xc.assign(nc, 0.0);
-    G->bc.assign(nc, 0.0);
-    *out = new orc_prec{std::unique_ptr<Prec>(G)};
+    const auto& lv = G->L[level];
+    if (A) to_c(lv.A, A);
+    if (R) to_c(lv.R, R);
+    if (d) std::memcpy(d, lv.d.data(), sizeof(double) * lv.d.size());
+  });
+}
+int orc_amg_coarse_inverse(const orc_prec* M, double* inv) {
+  return guard([&] {
+    AMG* G = as_amg(M);
+    if (inv && G->nc) std::memcpy(inv, G->Cinv.data(), sizeof(double) * G->Cinv.size());
   });
 }
 

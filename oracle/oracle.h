@@ -59,12 +59,17 @@ int orc_fsai_create(const deco_cpr_params* p, const orc_csr* A, orc_prec** out);
 int orc_jacobi_create(int block_size, const orc_csr* A, orc_prec** out);
 int orc_dense_create(const orc_csr* A, orc_prec** out);         /* exact LU solve (tiny A) */
 int orc_identity_create(int n, orc_prec** out);
-/* AMG V-cycle on an imported hierarchy: levels 0..L-1 have A_l, P_l (n_l x n_{l+1}),
- * R_l (= P_l^T), smoother diagonal d_l and C/F markers cf_l (1 = C); the coarsest level L
- * has its matrix A_L and the dense row-major inverse. relax_type as in deco_cpr_params. */
-int orc_amg_create(int nlev, const orc_csr* A, const orc_csr* P, const orc_csr* R,
-                   const double* const* d, const int* const* cf, int relax_type,
-                   int nc, const double* coarse_inv, orc_prec** out);
+/* AMG pressure stage (role of precond/hypre_AMG): from the finest matrix A0 and the imported
+ * interpolation operators P_l (n_l x n_{l+1}) and C/F markers cf_l (1 = C) of levels
+ * 0..nlev-1, the oracle forms R_l = P_l^T, A_{l+1} = drop(R_l (A_l P_l), drop_tol), the smoother
+ * diagonals of relax_type (relax_weight for type 7) and the coarsest inverse itself. Apply = one
+ * V-cycle from zero. */
+int orc_amg_build(const orc_csr* A0, int nlev, const orc_csr* P, const int* const* cf, int relax_type,
+                  double relax_weight, double drop_tol, orc_prec** out);
+int orc_amg_num_levels(const orc_prec* M, int* nlev, int* nc);
+/* copies of level `level`'s A, R and smoother diagonal (level nlev: the coarsest A only) */
+int orc_amg_level(const orc_prec* M, int level, orc_csr* A, orc_csr* R, double* d);
+int orc_amg_coarse_inverse(const orc_prec* M, double* inv);   /* row-major nc x nc */
 /* CPR setup: decouples A and b in place (precond/CPR.hpp:42-172); the local (pressure)
  * stage is attached afterwards from the extracted pressure matrix. */
 int orc_cpr_create(const deco_cpr_params* p, orc_csr* A, double* b, orc_prec** out);

@@ -90,3 +90,63 @@ def zero_diag_system(bs):
         v[k] = v[k + 1] = v[k2 - 1] = 0.0
     g["v"] = v
     return g, cases
+
+
+def amg_params(p):
+    """deco_cpr_params of a CPR "prec" tree (PTree) or of a "prec_loc" dict (standalone AMG)."""
+    import deco_amd
+    if isinstance(p, dict):
+        return deco_amd.cpr_params(deco_amd.PTree({"block_size": "1", "prec_loc": p}))
+    if isinstance(p, deco_amd.PTree):
+        return deco_amd.cpr_params(p)
+    return p
+
+
+def oracle_amg(G, A0, params, check=True):
+    """The oracle's AMG stage on A0 (oracle.Mat, the stage's input matrix) built from only the
+    device hierarchy G's interpolation operators P_l and C/F splittings: R_l, the Galerkin
+    products, the drop rule, the smoother diagonals and the coarsest inverse are the oracle's own.
+    With check, the device's setup products are compared with them: every level matrix (pattern
+    and values), R_l and the smoother diagonals bit for bit, the coarsest inverse to 1e-12, and
+    the drop rule against a scipy Galerkin product of the device's own level matrices (dropped
+    entries below amg_A_drop_tol times the row maximum, kept ones at or above it)."""
+    import oracle
+    import scipy.sparse as sp
+    p = amg_params(params)
+    levels, cinv = G.export()
+    Ps = [oracle.Mat(l["n"], l["nc"], *l["P"]) for l in levels]
+    Go = oracle.amg(A0, Ps, [l["cf"] for l in levels], p.amg_relax_type, p.amg_A_drop_tol, p.amg_relax_weight)
+    if not check:
+        return Go, levels, cinv
+    assert Go.nlevels == len(levels)
+    mats = [l["A"] for l in levels] + [G.coarse_matrix()]
+    for l in range(len(levels) + 1):
+        Ao, Ro, do = Go.level(l)
+        r, c, v = mats[l]
+        assert np.array_equal(r, Ao.r) and np.array_equal(c, Ao.c[:Ao.nnz]), f"level {l} pattern"
+        assert np.array_equal(v, Ao.v[:Ao.nnz]), f"level {l} values"
+        if l < len(levels):
+            rr, rc, rv = levels[l]["R"]
+            assert np.array_equal(rr, Ro.r) and np.array_equal(rc, Ro.c[:Ro.nnz]) and np.array_equal(rv, Ro.v[:Ro.nnz])
+            assert np.array_equal(levels[l]["d"], do), f"level {l} smoother diagonal"
+        if l > 0 and p.amg_A_drop_tol > 0:
+            lv = levels[l - 1]
+            Al = sp.csr_matrix((lv["A"][2], lv["A"][1], lv["A"][0]), shape=(lv["n"], lv["n"]))
+            P = sp.csr_matrix((lv["P"][2], lv["P"][1], lv["P"][0]), shape=(lv["n"], lv["nc"]))
+            full = (P.T.tocsr() @ Al @ P).tocsr()
+            full.sort_indices()
+            nr = full.shape[0]
+            rows = np.repeat(np.arange(nr), np.diff(full.indptr))
+            mx = np.zeros(nr)
+            np.maximum.at(mx, rows, np.abs(full.data))
+            kept = np.isin(rows.astype(np.int64) * nr + full.indices,
+                           np.repeat(np.arange(nr), np.diff(r)).astype(np.int64) * nr + c)
+            thr = p.amg_A_drop_tol * mx[rows]
+            off = full.indices != rows
+            assert np.all(np.abs(full.data[~kept]) < thr[~kept] * (1 + 1e-10)), f"level {l}: a dropped entry is large"
+            assert np.all(off[~kept]), f"level {l}: a diagonal was dropped"
+            big = kept & off
+            assert np.all(np.abs(full.data[big]) >= thr[big] * (1 - 1e-10)), f"level {l}: a kept entry is small"
+    ci = Go.coarse_inverse()
+    assert np.linalg.norm(cinv - ci) <= 1e-12 * np.linalg.norm(ci)
+    return Go, levels, cinv

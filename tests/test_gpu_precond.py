@@ -5,7 +5,7 @@ import pytest
 
 import deco_amd
 import oracle
-from conftest import gen_system, to_mat, small_params
+from conftest import gen_system, to_mat, small_params, oracle_amg
 
 pytestmark = pytest.mark.gpu
 
@@ -21,13 +21,6 @@ def dev_csr(g):
 def same_pattern(gpu_csr, mat):
     r, c, v = gpu_csr.download()
     return np.array_equal(r, mat.r) and np.array_equal(c, mat.c[:mat.nnz]), v
-
-
-def oracle_amg(G, relax):
-    levels, cinv = G.export()
-    lv = [dict(A=oracle.Mat(l["n"], l["n"], *l["A"]), P=oracle.Mat(l["n"], l["nc"], *l["P"]),
-               R=oracle.Mat(l["nc"], l["n"], *l["R"]), d=l["d"], cf=l["cf"]) for l in levels]
-    return oracle.amg(lv, relax, cinv), levels, cinv
 
 
 @pytest.mark.parametrize("cfg", [1, 4])
@@ -104,25 +97,16 @@ def test_cpr_setup_bitexact_and_apply_parity(gpu, cfg):
     ok, vp = same_pattern(Ap, Apo)
     assert ok and np.array_equal(vp, Apo.v)
     assert np.array_equal(M.left_transform(), Mo.left_transform(bs))
-    # AMG V-cycle on the same hierarchy
+    # AMG: the device hierarchy's setup products against the oracle's own (from P_l and the C/F
+    # splittings only), and the V-cycle
     G = M.amg()
-    Go, levels, cinv = oracle_amg(G, 18)
+    Go, levels, cinv = oracle_amg(G, Apo, tree)
     assert len(levels) >= 1
     xp = np.random.default_rng(2).uniform(-1, 1, Ap.n)
     assert rel(G.apply(xp).numpy(), Go.apply(xp)) <= 1e-13
     assert rel(G.apply_t(xp).numpy(), Go.apply_t(xp)) <= 1e-12
-    # hierarchy invariants: Galerkin product, coarse inverse
-    import scipy.sparse as sp
-    for lv, nxt in zip(levels, levels[1:] + [None]):
-        P = sp.csr_matrix((lv["P"][2], lv["P"][1], lv["P"][0]), shape=(lv["n"], lv["nc"]))
-        Al = sp.csr_matrix((lv["A"][2], lv["A"][1], lv["A"][0]), shape=(lv["n"], lv["n"]))
-        RAP = (P.T @ Al @ P).toarray()
-        if nxt is not None:
-            Ac = sp.csr_matrix((nxt["A"][2], nxt["A"][1], nxt["A"][0]), shape=(nxt["n"], nxt["n"])).toarray()
-            kept = Ac != 0
-            assert np.allclose(Ac[kept], RAP[kept], rtol=1e-12, atol=1e-14 * np.abs(RAP).max())
-    # CPR apply on the same hierarchy
-    Go2, _, _ = oracle_amg(G, 18)
+    # CPR apply with the oracle's AMG
+    Go2, _, _ = oracle_amg(G, Apo, tree, check=False)
     Mo.set_local(Go2)
     x = np.random.default_rng(3).uniform(-1, 1, g["n"])
     assert rel(M.apply(x).numpy(), Mo.apply(x)) <= 1e-12
@@ -133,18 +117,19 @@ def test_cpr_setup_bitexact_and_apply_parity(gpu, cfg):
     M.set_precision(deco_amd.PREC_FP64)
 
 
-@pytest.mark.parametrize("seq,relax", [(0, 18), (1, 18), (2, 18), (3, 17), (0, 17)])
+@pytest.mark.parametrize("seq,relax", [(0, 18), (1, 18), (2, 18), (3, 17), (0, 17), (3, -1), (2, -1), (3, 12)])
 def test_cpr_sequences(gpu, seq, relax):
-    """CPR sequences 0-3 against the oracle. With a Jacobi-type smoother (18) the pressure stage
-    runs fused (the residual epilogue does the first sweep, the last sweep adds into x); with
-    F-C-F half-sweeps (17) it runs as separate kernels. FP32 against FP64 on the device."""
+    """CPR sequences 0-3 against the oracle. With a Jacobi-type or two-stage Gauss-Seidel smoother
+    (18, -1, 12) the pressure stage runs fused (the residual epilogue writes the first sweep's
+    b/d, the last sweep adds into x); with F-C-F half-sweeps (17) it runs as separate kernels.
+    FP32 against FP64 on the device."""
     g = gen_system(4)
     tree = small_params(2, seq=str(seq), **{"prec_loc.amg_relax_type": relax})
     A = dev_csr(g)
     M = deco_amd.CPR(tree, A, None)
     Mo_A = to_mat(g)
     Mo = oracle.CPR(deco_amd.cpr_params(tree), Mo_A, None)
-    Go, _, _ = oracle_amg(M.amg(), relax)
+    Go, _, _ = oracle_amg(M.amg(), Mo.pressure_matrix(), tree)
     Mo.set_local(Go)
     x = np.random.default_rng(seq).uniform(-1, 1, g["n"])
     y = M.apply(x).numpy()
@@ -155,20 +140,27 @@ def test_cpr_sequences(gpu, seq, relax):
     assert rel(y32, y) <= 2e-4
 
 
-@pytest.mark.parametrize("relax,coarse", [(17, 10), (18, 8), (0, 10)])
-def test_amg_variants(gpu, relax, coarse):
+@pytest.mark.parametrize("relax,coarse,weight", [(17, 10, 0), (18, 8, 0), (7, 10, 0), (7, 10, 0.6), (-1, 10, 0),
+                                                (0, 8, 0), (11, 10, 0), (12, 10, 0), (14, 10, 0)])
+def test_amg_variants(gpu, relax, coarse, weight):
+    """Every smoother (F-C-F, l1-Jacobi, weighted Jacobi, the two-stage Gauss-Seidel forms of -1/0
+    and 11-14) and both coarsenings: the device hierarchy against the oracle's own setup, and the
+    V-cycle and its transpose against the oracle's."""
     g = gen_system(1)
     A = dev_csr(g)
     Ap = A.local_matrix(2, 0)
     tree = {"amg_SoC_alpha": "0.5", "amg_coarse_type": str(coarse), "amg_relax_type": str(relax),
             "amg_prol_trunc_fac": "1e-3", "amg_A_drop_tol": "1e-4", "amg_max_coarse": "50"}
+    if weight:
+        tree["amg_relax_weight"] = str(weight)
     G = deco_amd.AMG(tree, Ap)
-    Go, levels, _ = oracle_amg(G, relax)
+    Go, levels, _ = oracle_amg(G, oracle.local_matrix(to_mat(g), 2, 0), tree)
     assert len(levels) >= 2
     x = np.random.default_rng(4).uniform(-1, 1, Ap.n)
     y = G.apply(x).numpy()
     # for the Jacobi-type smoothers the device's up-sweep is one product with P - D^-1 A P (and
-    # P - D^-1 A^T P transposed), the oracle's the two steps of the reference form
+    # P - D^-1 A^T P transposed), the oracle's the two steps of the reference form; the last
+    # level's sub-cycle is one dense product on the device (the F-C-F smoother aside)
     assert rel(y, Go.apply(x)) <= 1e-12
     assert rel(G.apply_t(x).numpy(), Go.apply_t(x)) <= 1e-12
     G.set_precision(deco_amd.PREC_FP32)
@@ -216,7 +208,7 @@ def test_cpr_product_layouts(gpu, cfg):
     A = dev_csr(g)
     M = deco_amd.CPR(small_params(bs, layout="csr"), A, None)
     Mo = oracle.CPR(deco_amd.cpr_params(small_params(bs)), to_mat(g), None)
-    Go, _, _ = oracle_amg(M.amg(), 18)
+    Go, _, _ = oracle_amg(M.amg(), Mo.pressure_matrix(), small_params(bs))
     Mo.set_local(Go)
     x = np.random.default_rng(11).uniform(-1, 1, g["n"])
     yo, yto = Mo.apply(x), Mo.apply_t(x)
@@ -303,7 +295,7 @@ def test_cpr_setup_options_bitexact(gpu, cfg, opts):
     ok, vp = same_pattern(M.pressure_matrix(), Mo.pressure_matrix())
     assert ok and np.array_equal(vp, Mo.pressure_matrix().v)
     assert np.array_equal(M.left_transform(), Mo.left_transform(bs))
-    Go, _, _ = oracle_amg(M.amg(), 18)
+    Go, _, _ = oracle_amg(M.amg(), Mo.pressure_matrix(), tree)
     Mo.set_local(Go)
     x = np.random.default_rng(13).uniform(-1, 1, g["n"])
     assert rel(M.apply(x).numpy(), Mo.apply(x)) <= 1e-12
@@ -346,13 +338,13 @@ def test_amg_wide_chunk_span_falls_back(gpu):
     tree = {"amg_SoC_alpha": "0.25", "amg_coarse_type": "10", "amg_relax_type": "18",
             "amg_prol_trunc_fac": "1e-3", "amg_A_drop_tol": "0", "amg_max_coarse": "300"}
     G = deco_amd.AMG(tree, deco_amd.CSR.from_scipy(S))
-    Go, levels, _ = oracle_amg(G, 18)
+    Go, levels, _ = oracle_amg(G, oracle.Mat.from_scipy(S), tree)
     assert len(levels) >= 2
     x = np.random.default_rng(15).uniform(-1, 1, n)
     assert rel(G.apply(x).numpy(), Go.apply(x)) <= 1e-12
 
 
-@pytest.mark.parametrize("relax", [18, 0])
+@pytest.mark.parametrize("relax", [18, 7, -1, 12, 14])
 def test_amg_dense_tail_matches_sparse_subcycle(gpu, relax):
     """The last level's sub-cycle formed as one dense operator at setup, against the same
     hierarchy with the sparse sub-cycle (dense tail disabled) and the oracle's level-by-level
@@ -368,7 +360,7 @@ def test_amg_dense_tail_matches_sparse_subcycle(gpu, relax):
     finally:
         deco_amd.set_amg_dense_tail(2048)
     G1 = deco_amd.AMG(tree, Ap)
-    Go, levels, _ = oracle_amg(G1, relax)
+    Go, levels, _ = oracle_amg(G1, oracle.local_matrix(to_mat(g), 2, 0), tree)
     assert len(levels) >= 2
     assert G0.dense_tail() == 0 and G1.dense_tail() == levels[-1]["n"] > 0
     x = np.random.default_rng(7).uniform(-1, 1, Ap.n)
@@ -392,9 +384,9 @@ def test_cpr_dense_tail_is_the_whole_pressure_cycle(gpu, seq):
     tree = small_params(2, seq=str(seq), **{"prec_loc.amg_max_levels": "2"})
     A = dev_csr(g)
     M = deco_amd.CPR(tree, A, None)
-    Go, levels, _ = oracle_amg(M.amg(), 18)
-    assert len(levels) == 1 and M.amg().dense_tail() == levels[0]["n"]
     Mo = oracle.CPR(deco_amd.cpr_params(tree), to_mat(g), None)
+    Go, levels, _ = oracle_amg(M.amg(), Mo.pressure_matrix(), tree)
+    assert len(levels) == 1 and M.amg().dense_tail() == levels[0]["n"]
     Mo.set_local(Go)
     x = np.random.default_rng(seq).uniform(-1, 1, g["n"])
     assert rel(M.apply(x).numpy(), Mo.apply(x)) <= 1e-12

@@ -5,8 +5,8 @@ import pytest
 
 import deco_amd
 import oracle
-from conftest import gen_system, to_mat, small_params
-from test_gpu_precond import oracle_amg, dev_csr
+from conftest import gen_system, to_mat, small_params, oracle_amg
+from test_gpu_precond import dev_csr
 
 pytestmark = pytest.mark.gpu
 
@@ -23,8 +23,7 @@ def setup_pair(cfg, tree):
     Ao = to_mat(g)
     bo = g["b"].copy()
     Mo = oracle.CPR(deco_amd.cpr_params(tree), Ao, bo)
-    Go, _, _ = oracle_amg(M.amg(), deco_amd.cpr_params(tree).amg_relax_type if
-                          deco_amd.cpr_params(tree).amg_relax_type != -1 else 18)
+    Go, _, _ = oracle_amg(M.amg(), Mo.pressure_matrix(), tree)
     Mo.set_local(Go)
     return g, A, b, M, Ao, bo, Mo
 

@@ -130,31 +130,6 @@ def laplacian_1d_blocks(n):
     return sp.diags([off, main, off], [-1, 0, 1]).tocsr()
 
 
-def test_vcycle_equals_two_grid_operator():
-    # two-grid cycle with a Jacobi-type smoother: G = S + (I - S A)(S + P Ac^-1 R (I - A S)),
-    # S = D^-1, checked as a dense operator
-    n = 40
-    S = (laplacian_1d_blocks(n) + sp.eye(n) * 0.01).tocsr()
-    A = oracle.Mat.from_scipy(S)
-    g = deco_amd  # host interpolation from the product (host-only function)
-    cf, nc, (Pr, Pc, Pv) = g.amg_interp_host({"amg_SoC_alpha": "0.25", "amg_coarse_type": "10",
-                                               "amg_prol_trunc_fac": "0", "amg_P_max_elmts": "0"},
-                                              n, A.r, A.c, A.v)
-    P = sp.csr_matrix((Pv, Pc, Pr), shape=(n, nc))
-    R = P.T.tocsr()
-    Ac = (R @ S @ P).toarray()
-    d = np.abs(S).sum(axis=1).A1        # l1-Jacobi diagonal
-    lv = dict(A=A, P=oracle.Mat.from_scipy(P), R=oracle.Mat.from_scipy(R), d=d, cf=cf)
-    G = oracle.amg([lv], 18, np.linalg.inv(Ac))
-    Dn = S.toarray()
-    Sm = np.diag(1.0 / d)
-    I = np.eye(n)
-    Gd = Sm + (I - Sm @ Dn) @ (Sm + P.toarray() @ np.linalg.inv(Ac) @ R.toarray() @ (I - Dn @ Sm))
-    x = np.random.default_rng(2).standard_normal(n)
-    assert np.allclose(G.apply(x), Gd @ x, rtol=1e-12, atol=1e-13)
-    assert np.allclose(G.apply_t(x), Gd.T @ x, rtol=1e-12, atol=1e-13)
-
-
 def test_host_interpolation_structure():
     n = 30
     S = laplacian_1d_blocks(n)
