@@ -121,6 +121,49 @@ def cpu_solve_baseline(g, M, prec_tree, sol, gpu_iters, args, log):
                       f"per-iteration {dt / max(info.iter, 1) * 1e3:.1f} ms; one run on 4 threads {dt4:.2f}s"}
 
 
+def adjoint_small(deco_amd, args, log):
+    """Forward CPR-GMRES(30) solve of b = A x* and the adjoint of L = w.x (w ~ N(0,1)): the
+    transposed solve, lambda = W^T mu and dL/dA on A's pattern, to rtol 1e-8 on the 32x32x16
+    two-unknown system, both preconditioner precisions; medians of args.steps runs."""
+    g = deco_amd.generate(1)
+    n = g["n"]
+    A = deco_amd.CSR(n, n, g["r"], g["c"], g["v"])
+    b = deco_amd.DeviceArray.from_numpy(g["b"])
+    M = deco_amd.CPR(deco_amd.PTree(dict(PREC_TREE, block_size=str(g["bs"]))), A, b)
+    sol = {"type": "rgmres", "reltol": "1e-8", "restart": "30", "itmax": "500", "verbosity": "0"}
+    w = deco_amd.DeviceArray.from_numpy(np.random.default_rng(1004).standard_normal(n))
+    out = {"system": f"two-phase 32x32x16, n={n}, nnz={g['nnz']}"}
+
+    def med(fn):
+        fn()
+        deco_amd.lib().deco_sync()
+        ts = []
+        for _ in range(max(args.steps, 3)):
+            t1 = time.perf_counter()
+            r = fn()
+            deco_amd.lib().deco_sync()
+            ts.append(time.perf_counter() - t1)
+        return float(np.median(ts)) * 1e3, r
+
+    for prec, key in ((deco_amd.PREC_FP64, "fp64"), (deco_amd.PREC_FP32, "fp32_precond")):
+        M.set_precision(prec)
+        fw_ms, Sf = med(lambda: deco_amd.SOL(sol, A, M, b))
+        fwv_ms, Sv = med(lambda: deco_amd.SOL(sol, A, M, w))
+
+        def adjoint():
+            St = deco_amd.SOL(sol, A, M, w, transpose=True)
+            lam = M.transform_transpose(St.x)
+            deco_amd.grad_A_pattern(A, lam, Sf.x)
+            return St
+        adj_ms, St = med(adjoint)
+        out[key] = {"forward_ms": fw_ms, "forward_iter": Sf.iter, "adjoint_ms": adj_ms, "adjoint_iter": St.iter,
+                    "ratio_to_forward": adj_ms / fw_ms, "forward_same_rhs_ms": fwv_ms,
+                    "forward_same_rhs_iter": Sv.iter, "ratio_to_forward_same_rhs": adj_ms / fwv_ms}
+    M.set_precision(deco_amd.PREC_FP64)
+    log(f"adjoint (32x32x16): {out}")
+    return out
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -222,6 +265,11 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=0,
                     help="GMRES iterations in the CPU baseline sample (0: the whole solve)")
     ap.add_argument("--cpu-reps", type=int, default=3, help="repetitions of the whole CPU solve (median)")
+    ap.add_argument("--relax", type=int, default=None,
+                    help="amg_relax_type of the headline solves (default: the reference parameter set's -1)")
+    ap.add_argument("--relax-weight", type=float, default=0.0, help="amg_relax_weight (relax 7)")
+    ap.add_argument("--no-smoother-compare", action="store_true",
+                    help="skip the second CPR setup with weighted Jacobi (relax 7, weight 2/3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: the solve path runs on one GPU (the multi-GPU row partition is not implemented); "
@@ -311,6 +359,10 @@ def main():
     tree = dict(PREC_TREE, block_size=str(g["bs"]))
     for k, sub in WORKLOAD_PREC.get(args.cfg, {}).items():
         tree[k] = dict(tree[k], **sub)
+    if args.relax is not None:
+        tree["prec_loc"] = dict(tree["prec_loc"], amg_relax_type=str(args.relax))
+    if args.relax_weight > 0:
+        tree["prec_loc"] = dict(tree["prec_loc"], amg_relax_weight=str(args.relax_weight))
     prec_tree = deco_amd.PTree(tree)
     sol = {"type": method, "reltol": "1e-8", "restart": restart, "itmax": "500", "verbosity": "0"}
     b_dev = deco_amd.DeviceArray.from_numpy(g["b"])
@@ -323,58 +375,109 @@ def main():
     tight = deco_amd.SOL(dict(sol, type="rgmres", restart="50", reltol="1e-12", itmax="2000"), A, M, b_dev)
     xt = tight.x.numpy()
     log(f"FP64 reference solve (rtol 1e-12): iter {tight.iter} res {tight.res_rel:.2e}")
+    import scipy.sparse as sps
+    A0 = sps.csr_matrix((g["v"], g["c"], g["r"]), shape=(n, n))   # the system before decoupling
+    err = lambda x, ref: float(np.linalg.norm(x - ref) / np.linalg.norm(ref))
 
-    def run(precision):
-        M.set_precision(precision)
+    def run(Mp, Ap, bp, precision, flexible=0, xref=None, xtight=None):
+        """warm-up, then args.steps timed solves; the last solve's x and, from one more solve with
+        the library's byte count on, the solve's algorithmic bytes"""
+        Mp.set_precision(precision)
+        sp_ = dict(sol, flexible=str(flexible))
         for _ in range(args.warmup):
-            deco_amd.SOL(sol, A, M, b_dev)
+            deco_amd.SOL(sp_, Ap, Mp, bp)
         deco_amd.lib().deco_sync()
         times, last = [], None
         t1 = time.perf_counter()
         for _ in range(args.steps):
-            last = deco_amd.SOL(sol, A, M, b_dev)
+            last = deco_amd.SOL(sp_, Ap, Mp, bp)
             times.append(last.info.time_ms)
         deco_amd.lib().deco_sync()
         wall = (time.perf_counter() - t1) / args.steps
-        return last, dict(ms=float(np.median(times)), wall_ms=wall * 1e3, iter=last.iter, res_rel=last.res_rel,
-                          flag=last.flag, n_prec=last.info.n_prec)
-
-    S64, r64 = run(deco_amd.PREC_FP64)
-    x64 = S64.x.numpy()
-    S32, r32 = run(deco_amd.PREC_FP32)
-    x32 = S32.x.numpy()
-    M.set_precision(deco_amd.PREC_FP64)
-    err = lambda x, ref: float(np.linalg.norm(x - ref) / np.linalg.norm(ref))
-    import scipy.sparse as sps
-    A0 = sps.csr_matrix((g["v"], g["c"], g["r"]), shape=(n, n))   # the system before decoupling
-    for r, x in ((r64, x64), (r32, x32)):
-        r["err_vs_fp64_same_rtol"] = err(x, x64)
-        r["err_vs_fp64_rtol1e-12"] = err(x, xt)
+        deco_amd.count_bytes(True)
+        deco_amd.SOL(sp_, Ap, Mp, bp)
+        nbytes, nlaunch = deco_amd.counted_bytes()
+        deco_amd.count_bytes(False)
+        ms = float(np.median(times))
+        x = last.x.numpy()
+        r = dict(ms=ms, wall_ms=wall * 1e3, iter=last.iter, res_rel=last.res_rel, flag=last.flag,
+                 n_prec=last.info.n_prec, alg_bytes=nbytes, launches=nlaunch,
+                 alg_GBps=nbytes / (ms * 1e-3) / 1e9, frac_of_peak=nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS)
+        if xref is not None:
+            r["err_vs_fp64_same_rtol"] = err(x, xref)
+        if xtight is not None:
+            r["err_vs_fp64_rtol1e-12"] = err(x, xtight)
         r["err_vs_xstar"] = err(x, g["xstar"])
         r["res_rel_original"] = float(np.linalg.norm(g["b"] - A0 @ x) / np.linalg.norm(g["b"]))
-    del A0
-    log(f"FP64-only: {r64}")
+        return last, r
+
+    S64, r64 = run(M, A, b_dev, deco_amd.PREC_FP64, 0, None, xt)
+    x64 = S64.x.numpy()
+    r64["err_vs_fp64_same_rtol"] = 0.0
+    _, r64f = run(M, A, b_dev, deco_amd.PREC_FP64, 1, x64, xt)
+    S32, r32 = run(M, A, b_dev, deco_amd.PREC_FP32, 0, x64, xt)
+    M.set_precision(deco_amd.PREC_FP64)
+    log(f"FP64-only (reference GMRES form): {r64}")
+    log(f"FP64-only (flexible): {r64f}")
     log(f"FP32-precond: {r32}")
     # Both precisions run the preconditioner's products on cell-block shadows. The same pair of
     # solves on the CSR arrays (the reference's storage and summation order) shows what the
     # precision gains there.
     M.set_layout(deco_amd.LAYOUT_CSR)
-    Sa, ra = run(deco_amd.PREC_FP64)
+    Sa, ra = run(M, A, b_dev, deco_amd.PREC_FP64, 1)
     xa = Sa.x.numpy()
-    Sb, rb = run(deco_amd.PREC_FP32)
+    Sb, rb = run(M, A, b_dev, deco_amd.PREC_FP32, 0)
     xb = Sb.x.numpy()
-    csr = {"fp64_ms": ra["wall_ms"], "fp64_iter": ra["iter"], "fp32_ms": rb["wall_ms"], "fp32_iter": rb["iter"],
-           "speedup_fp32_vs_fp64": ra["wall_ms"] / rb["wall_ms"], "fp64_err_vs_block_fp64": err(xa, x64),
-           "fp32_err_vs_fp64_same_rtol": err(xb, xa), "fp32_err_vs_fp64_rtol1e-12": err(xb, xt)}
+    csr = {"fp64_ms": ra["wall_ms"], "fp64_iter": ra["iter"], "fp64_n_prec": ra["n_prec"], "fp32_ms": rb["wall_ms"],
+           "fp32_iter": rb["iter"], "fp32_n_prec": rb["n_prec"], "speedup_fp32_vs_fp64": ra["wall_ms"] / rb["wall_ms"],
+           "fp64_err_vs_block_fp64": err(xa, x64), "fp32_err_vs_fp64_same_rtol": err(xb, xa),
+           "fp32_err_vs_fp64_rtol1e-12": err(xb, xt)}
     log(f"CSR storage: {csr}")
     M.set_layout(deco_amd.LAYOUT_BLOCK)
     M.set_precision(deco_amd.PREC_FP64)
+    relax = int(tree["prec_loc"]["amg_relax_type"])
     result.update({
         "value": r32["wall_ms"] / 1e3, "unit": "s", "higher_is_better": False, "dtype": "f64",
         "ms_per_step": r32["wall_ms"],
-        "solve": {"fp32_precond": r32, "fp64_only": r64, "speedup_fp32_vs_fp64": r64["wall_ms"] / r32["wall_ms"],
-                  "csr_storage": csr, "setup_s": setup_s, "fp64_rtol1e-12_iter": tight.iter},
+        "solve": {"fp32_precond": r32, "fp64_only": r64f, "fp64_only_reference_form": r64,
+                  "speedup_fp32_vs_fp64": r64f["wall_ms"] / r32["wall_ms"],
+                  "speedup_fp32_vs_fp64_reference_form": r64["wall_ms"] / r32["wall_ms"],
+                  "note": ("fp64_only runs flexible right GMRES, as the FP32-preconditioned solve does, so both "
+                           "make the same preconditioner applications; fp64_only_reference_form keeps "
+                           "rgmres.tpp's final M^-1(V y) application"),
+                  "amg_relax_type": relax, "csr_storage": csr, "setup_s": setup_s, "fp64_rtol1e-12_iter": tight.iter},
+        "solve_roofline": {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+                           "fp32_precond": {"alg_bytes": r32["alg_bytes"], "achieved": r32["alg_GBps"],
+                                            "frac": r32["frac_of_peak"]},
+                           "fp64_only": {"alg_bytes": r64f["alg_bytes"], "achieved": r64f["alg_GBps"],
+                                         "frac": r64f["frac_of_peak"]},
+                           "definition": ("library byte count of one whole solve (every product's stored values, "
+                                          "indices and row data, one pass over each vector it reads or writes; "
+                                          "Gram-Schmidt sweeps; vector kernels, copies and fills) / the solve's "
+                                          "device time")},
     })
+
+    # the same workload with the explicit weighted-Jacobi smoother (relax 7, weight 2/3) against
+    # the reference parameter set's default (-1: two-stage Gauss-Seidel here)
+    if not args.no_smoother_compare and relax != 7:
+        A2 = deco_amd.CSR(n, n, g["r"], g["c"], g["v"])
+        b2 = deco_amd.DeviceArray.from_numpy(g["b"])
+        t2 = dict(tree, prec_loc=dict(tree["prec_loc"], amg_relax_type="7", amg_relax_weight=str(2.0 / 3.0)))
+        t0 = time.perf_counter()
+        M2 = deco_amd.CPR(deco_amd.PTree(t2), A2, b2)
+        deco_amd.lib().deco_sync()
+        setup2 = time.perf_counter() - t0
+        _, j64 = run(M2, A2, b2, deco_amd.PREC_FP64, 1, x64, xt)
+        _, j32 = run(M2, A2, b2, deco_amd.PREC_FP32, 0, x64, xt)
+        result["solve"]["smoothers"] = {
+            str(relax): {"fp64_ms": r64f["wall_ms"], "fp64_iter": r64f["iter"], "fp32_ms": r32["wall_ms"],
+                         "fp32_iter": r32["iter"], "setup_s": setup_s},
+            "7 (weight 2/3)": {"fp64_ms": j64["wall_ms"], "fp64_iter": j64["iter"], "fp32_ms": j32["wall_ms"],
+                               "fp32_iter": j32["iter"], "setup_s": setup2,
+                               "speedup_fp32_vs_fp64": j64["wall_ms"] / j32["wall_ms"]}}
+        log(f"smoothers: {result['solve']['smoothers']}")
+        del M2, A2, b2
+    del A0
 
     # ---------------------------------------------------------------- adjoint
     # transposed CPR solve (W A)^T mu = w, lambda = W^T mu, dL/dA on A's pattern; and, because
@@ -411,6 +514,10 @@ def main():
                 "per_iteration_ratio": (adj_ms / max(St.iter, 1)) / (fw_ms / max(Sf.iter, 1))}
             log(f"adjoint {key}: {result['adjoint'][key]}")
         M.set_precision(deco_amd.PREC_FP64)
+
+    # adjoint-to-forward time on the 32x32x16 two-unknown system (the adjoint case's timing size)
+    if not args.no_adjoint:
+        result["adjoint_small"] = adjoint_small(deco_amd, args, log)
 
     # ---------------------------------------------------------------- CPU baseline
     if not args.no_cpu:

@@ -50,7 +50,7 @@ class CprParams(C.Structure):
 
 class SolParams(C.Structure):
     _fields_ = [("type", C.c_int), ("reltol", C.c_double), ("restart", C.c_int), ("itmax", C.c_int),
-                ("verbosity", C.c_int), ("use_x0", C.c_int), ("ortho", C.c_int)]
+                ("verbosity", C.c_int), ("use_x0", C.c_int), ("ortho", C.c_int), ("flexible", C.c_int)]
 
 
 class SolInfo(C.Structure):
@@ -141,6 +141,8 @@ SIGNATURES = {
     "deco_profile_stop": (i, []),
     "deco_profile_time": (i, [C.c_char_p, dp]),
     "deco_profile_report": (i, [C.c_char_p, sz, C.POINTER(C.c_size_t)]),
+    "deco_count_bytes": (i, [i]),
+    "deco_counted_bytes": (i, [C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
     "deco_solve": (i, [C.POINTER(SolParams), vp, vp, vp, vp, C.POINTER(SolInfo)]),
     "deco_solve_transpose": (i, [C.POINTER(SolParams), vp, vp, vp, vp, C.POINTER(SolInfo)]),
     "deco_grad_A_pattern": (i, [vp, vp, vp, vp]),

@@ -366,6 +366,7 @@ def sol_params(sol):
     if ortho not in _lib.ORTHO:
         raise ValueError("Invalid Gram-Schmidt variant: " + ortho)
     s.ortho = _lib.ORTHO[ortho]
+    s.flexible = sol.get("flexible", int, 0)
     return s
 
 
@@ -628,6 +629,19 @@ def profile_time(name):
     t = C.c_double()
     check(lib().deco_profile_time(name.encode(), C.byref(t)))
     return t.value
+
+
+def count_bytes(on=True):
+    """Start (or stop) the library's algorithmic byte count of the device work it queues; both
+    reset the count."""
+    check(lib().deco_count_bytes(1 if on else 0))
+
+
+def counted_bytes():
+    """(bytes, launches) counted since count_bytes()."""
+    b, n = C.c_double(), C.c_longlong()
+    check(lib().deco_counted_bytes(C.byref(b), C.byref(n)))
+    return b.value, n.value
 
 
 def profile_report():

@@ -381,6 +381,7 @@ class SOL {
     if (ortho == "dcgs2") sp.ortho = DECO_ORTHO_DCGS2;
     else if (ortho == "mgs") sp.ortho = DECO_ORTHO_MGS;
     else throw std::invalid_argument("Invalid Gram-Schmidt variant: " + ortho);
+    sp.flexible = param.template get<int>("flexible", 0);
     deco_prec* M = nullptr;
     if (precond) {
       auto* dp = precond.template target<deco_facade::DevicePrecond<V>>();

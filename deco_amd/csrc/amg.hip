@@ -123,6 +123,7 @@ struct EpiResG {
     rho[o] = r;
     g0[o] = r / p.d;
   }
+  size_t row_bytes() const { return 4 * sizeof(T); }
 };
 
 // One inner step of a two-stage Gauss-Seidel sweep, the product being (T g)[o]:
@@ -146,6 +147,7 @@ struct EpiTwoStage {
     if (add_to) add_to[(size_t)o * stride] = p.a + y;
     else out[o] = y;
   }
+  size_t row_bytes() const { return sizeof(T) * ((x ? 1 : 0) + 2 + (add_to ? 2 : 1)); }
 };
 
 // smoother diagonal: l1 (sum |a_ij|) or the diagonal entry divided by the Jacobi weight
@@ -194,6 +196,7 @@ struct EpiRelax {
     y[o] = yo;
     if (add_to) add_to[(size_t)o * stride] = p.a + yo;
   }
+  size_t row_bytes() const { return sizeof(T) * (4 + (add_to ? 2 : 0)) + (subset >= 0 ? sizeof(int) : 0); }
 };
 
 // Restriction b_c = R r fused with the next level's zero-guess Jacobi-type sweep: each coarse
@@ -211,6 +214,7 @@ struct EpiRestrictJacobi0 {
     bn[o] = bo;
     xn[o] = (T)0 + (bo - (T)0) / p.d;
   }
+  size_t row_bytes() const { return 3 * sizeof(T); }
 };
 
 // Up-sweep of a level with a Jacobi-type smoother as one product: with t = x + P e and the
@@ -233,6 +237,7 @@ struct EpiUp {
     if (add_to) add_to[(size_t)o * stride] = p.a + y;   // the caller's iterate takes the result
     else x[o] = y;
   }
+  size_t row_bytes() const { return sizeof(T) * (3 + (add_to ? 2 : 1)); }
 };
 
 // S = P - D^{-1} Q (Q = A P) on the union of the two row patterns (both sorted by column)
@@ -576,11 +581,13 @@ struct AMG : deco_prec {
   void dense(int m, const MT* C, const float* b, float* x, float* add_to, int stride) {
     hipLaunchKernelGGL((k_dense<float, MT>), dim3((m + 3) / 4), dim3(256), 0, stream(), m, C, b, x, add_to, stride);
     DECO_CHECK_LAUNCH();
+    count_bytes((double)m * m * sizeof(MT) + (double)m * sizeof(float) * (add_to ? 3 : 2));
   }
   template <class MT>
   void dense(int m, const MT* C, const double* b, double* x, double* add_to, int stride) {
     hipLaunchKernelGGL((k_dense<double, MT>), dim3((m + 3) / 4), dim3(256), 0, stream(), m, C, b, x, add_to, stride);
     DECO_CHECK_LAUNCH();
+    count_bytes((double)m * m * sizeof(MT) + (double)m * sizeof(double) * (add_to ? 3 : 2));
   }
   // the dense tail operator in the cycle's precision
   const double* tail_op(bool tr, const double*) const { return tr ? Z64t.p : Z64.p; }
@@ -643,11 +650,13 @@ struct AMG : deco_prec {
     if (relax == 17) {
       hipLaunchKernelGGL(k_relax_zero<T>, dim3(grid_for(nl, 256)), dim3(256), 0, stream(), nl, b, D<T>(lv), lv.cf.p, 0, x);
       DECO_CHECK_LAUNCH();
+      count_bytes((double)nl * (3 * sizeof(T) + sizeof(int)));
       relax_sweep<T>(lv, A, x, b, t, 1);
       relax_sweep<T>(lv, A, t, b, x, 0);
     } else if (!x_ready) {
       hipLaunchKernelGGL(k_relax_zero<T>, dim3(grid_for(nl, 256)), dim3(256), 0, stream(), nl, b, D<T>(lv), lv.cf.p, -1, x);
       DECO_CHECK_LAUNCH();
+      count_bytes((double)nl * (3 * sizeof(T)));
     }
     // residual and restriction (the restriction also does the next level's zero-guess sweep
     // for the Jacobi-type smoothers)
@@ -709,6 +718,7 @@ struct AMG : deco_prec {
     if (!x_ready) {
       hipLaunchKernelGGL(k_relax_zero<T>, dim3(grid_for(nl, 256)), dim3(256), 0, stream(), nl, b, d, lv.cf.p, -1, x);
       DECO_CHECK_LAUNCH();
+      count_bytes((double)nl * (3 * sizeof(T)));
     }
     const deco_csr* Td = strict(lv, tr, down);
     if (inner == 1) {

@@ -91,6 +91,23 @@ struct ProfPhase {
   ProfPhase& operator=(const ProfPhase&) = delete;
 };
 
+// Algorithmic bytes of the device work queued while counting is on (deco_count_bytes): each
+// launch adds what it must move at least once -- its matrix's stored values and indices, its row
+// data, one pass over every vector it reads or writes -- and a copy or fill its bytes.
+struct ByteCount {
+  bool on = false;
+  double bytes = 0.0;
+  long long launches = 0;
+};
+ByteCount& byte_count();
+inline void count_bytes(double b) {
+  ByteCount& c = byte_count();
+  if (c.on) {
+    c.bytes += b;
+    ++c.launches;
+  }
+}
+
 // ------------------------------------------------------------------ device buffer ------
 template <class T>
 struct DBuf {
@@ -140,6 +157,12 @@ struct DBuf {
 template <class T>
 void copy_d2d(T* dst, const T* src, size_t count) {
   if (count) DECO_HIP(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDeviceToDevice, stream()));
+  count_bytes(2.0 * sizeof(T) * count);
+}
+// clears bytes of device memory on the library stream (counted as written bytes)
+inline void zero_async(void* p, size_t bytes) {
+  if (bytes) DECO_HIP(hipMemsetAsync(p, 0, bytes, stream()));
+  count_bytes((double)bytes);
 }
 template <class T>
 void copy_h2d(T* dst, const T* src, size_t count) {

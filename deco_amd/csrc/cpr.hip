@@ -567,6 +567,7 @@ struct CPR : deco_prec {
       glo->apply(r, tmp.p);
       hipLaunchKernelGGL((k_addz<double, double>), dim3(gn()), dim3(256), 0, stream(), n, (const double*)x,
                          (const double*)tmp.p, x);
+      count_bytes(3.0 * n * 8);
       DECO_CHECK_LAUNCH();
     }
   }
@@ -618,14 +619,17 @@ struct CPR : deco_prec {
     const int seq = p.seq;
     if (seq == 1) {                                                   // CPR.hpp:176-200
       hipLaunchKernelGGL((k_gather_pressure<double, double>), dim3(gs()), dim3(256), 0, stream(), sn, bs, r, rt.p);
+      count_bytes((double)sn * 12);
       loc->apply(rt.p, xp.p);
-      DECO_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, stream()));
+      zero_async(x, sizeof(double) * n);
       hipLaunchKernelGGL((k_set_pressure<double, double>), dim3(gs()), dim3(256), 0, stream(), sn, bs, x,
                          (const double*)xp.p);
+      count_bytes((double)sn * 12);
       residual64(x, r, rt.p);
       glo->apply(rt.p, x);
       hipLaunchKernelGGL((k_add_pressure<double, double>), dim3(gs()), dim3(256), 0, stream(), sn, bs, x,
                          (const double*)xp.p);
+      count_bytes((double)sn * 20);
       DECO_CHECK_LAUNCH();
       return;
     }
@@ -646,6 +650,7 @@ struct CPR : deco_prec {
     if (!fused) {
       hipLaunchKernelGGL((k_add_pressure<double, double>), dim3(gs()), dim3(256), 0, stream(), sn, bs, x,
                          (const double*)xp.p);
+      count_bytes((double)sn * 20);
       DECO_CHECK_LAUNCH();
     }
     if (seq == 0) return;
@@ -668,15 +673,18 @@ struct CPR : deco_prec {
     const int seq = p.seq;
     if (seq == 1) {
       hipLaunchKernelGGL((k_gather_pressure<double, float>), dim3(gs()), dim3(256), 0, stream(), sn, bs, r, rt32.p);
+      count_bytes((double)sn * 12);
       loc->apply32(rt32.p, xp32.p);
-      DECO_HIP(hipMemsetAsync(x32.p, 0, sizeof(float) * n, stream()));
+      zero_async(x32.p, sizeof(float) * n);
       hipLaunchKernelGGL((k_set_pressure<float, float>), dim3(gs()), dim3(256), 0, stream(), sn, bs, x32.p,
                          (const float*)xp32.p);
+      count_bytes((double)sn * 12);
       residual32(x32.p, r, rt32.p);
       if (F) fsai_apply32_f(F, rt32.p, x32.p);
       else glo->apply32(rt32.p, x32.p);
       hipLaunchKernelGGL((k_add_pressure<float, float>), dim3(gs()), dim3(256), 0, stream(), sn, bs, x32.p,
                          (const float*)xp32.p);
+      count_bytes((double)sn * 20);
       convert((const float*)x32.p, xo, n);
       return;
     }
@@ -695,6 +703,7 @@ struct CPR : deco_prec {
     if (!fused) {
       hipLaunchKernelGGL((k_add_pressure<float, float>), dim3(gs()), dim3(256), 0, stream(), sn, bs, x32.p,
                          (const float*)xp32.p);
+      count_bytes((double)sn * 20);
       DECO_CHECK_LAUNCH();
     }
     if (seq == 0) { convert((const float*)x32.p, xo, n); return; }
@@ -705,6 +714,7 @@ struct CPR : deco_prec {
       glo->apply32(rt32.p, t32.p);
       hipLaunchKernelGGL((k_addz<float, float, TO>), dim3(gn()), dim3(256), 0, stream(), n, (const float*)x32.p,
                          (const float*)t32.p, xo);
+      count_bytes(3.0 * n * 8);
       DECO_CHECK_LAUNCH();
     }
   }
@@ -729,16 +739,19 @@ struct CPR : deco_prec {
       residual32_t(a, u, s32.p);                                       // s = u - A^T a
       hipLaunchKernelGGL((k_gather_pressure<float, float>), dim3(gs()), dim3(256), 0, stream(), sn, bs,
                          (const float*)s32.p, rt32.p);
+      count_bytes((double)sn * 12);
       DECO_CHECK_LAUNCH();
     }
     loc->apply32_t(rt32.p, xp32.p);                                    // e = P L^T R s
     ProfPhase ph_rest("CPR_rest");
-    DECO_HIP(hipMemsetAsync(e32.p, 0, sizeof(float) * n, stream()));
+    zero_async(e32.p, sizeof(float) * n);
     hipLaunchKernelGGL((k_set_pressure<float, float>), dim3(gs()), dim3(256), 0, stream(), sn, bs, e32.p,
                        (const float*)xp32.p);
+    count_bytes((double)sn * 12);
     DECO_CHECK_LAUNCH();
     residual32_t(e32.p, (const float*)s32.p, x32.p);                    // s - A^T e
-    hipLaunchKernelGGL(k_add_f, dim3(gn()), dim3(256), 0, stream(), n, (const float*)a, e32.p);   // e = a + e
+    hipLaunchKernelGGL(k_add_f, dim3(gn()), dim3(256), 0, stream(), n, (const float*)a, e32.p);
+    count_bytes(12.0 * n);   // e = a + e
     DECO_CHECK_LAUNCH();
     ph_rest.end();
     fsai_apply32_t_addz(F, x32.p, e32.p, xo);                          // out = a + e + G^T (s - A^T e)
@@ -756,13 +769,16 @@ struct CPR : deco_prec {
     a_product(At, a, EpiStd<EPI_RES, double, double>{s, u});
     hipLaunchKernelGGL((k_gather_pressure<double, double>), dim3(gs()), dim3(256), 0, stream(), sn, bs,
                        (const double*)s, rt.p);
+    count_bytes((double)sn * 12);
     loc->apply_t(rt.p, xp.p);
-    DECO_HIP(hipMemsetAsync(x2.p, 0, sizeof(double) * n, stream()));
+    zero_async(x2.p, sizeof(double) * n);
     hipLaunchKernelGGL((k_set_pressure<double, double>), dim3(gs()), dim3(256), 0, stream(), sn, bs, x2.p,
                        (const double*)xp.p);
+    count_bytes((double)sn * 12);
     a_product(At, x2.p, EpiStd<EPI_RES, double, double>{rt.p, s});   // rt (length n) = s - A^T e
     glo->apply_t(rt.p, x);
     hipLaunchKernelGGL(k_add3, dim3(gn()), dim3(256), 0, stream(), n, (const double*)a, (const double*)x2.p, x);
+    count_bytes(32.0 * n);
     DECO_CHECK_LAUNCH();
   }
 
@@ -840,6 +856,7 @@ int deco_cpr_transform_transpose(const deco_prec* M, const double* mu, double* l
     need(lam, c->n, "lambda");
     hipLaunchKernelGGL(k_transform_t, dim3(grid_for(c->sn, 128)), dim3(128), 0, stream(), c->sn, c->bs,
                        (const double*)c->W.p, mu, lam);
+    count_bytes((double)c->sn * c->bs * (c->bs + 2) * 8);
     DECO_CHECK_LAUNCH();
     sync();
   });

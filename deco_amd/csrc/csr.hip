@@ -750,6 +750,7 @@ const deco_bsr* deco_csr::block32(int R, int C) const {
   B->R = R;
   B->C = C;
   B->nb = n / R;
+  B->mb = m / C;
   const int nb = B->nb;
   DBuf<int> cnt(nb + 1);
   B->br.alloc(nb + 1);

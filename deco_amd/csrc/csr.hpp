@@ -16,6 +16,7 @@
 // chosen when that padding stays within the library's threshold (deco_set_sell_padding).
 struct deco_bsr {
   int R = 1, C = 1, nb = 0, nnzb = 0, nchunks = 0;
+  int mb = 0;                  // block columns
   deco::DBuf<int> br, bc;
   deco::DBuf<int2> rb;
   deco::DBuf<float> bv;
@@ -179,6 +180,12 @@ struct EpiStd {
     else if constexpr (EPI == EPI_RES) y[o] = (YT)((AT)p.a - acc);
     else y[o] = (YT)((AT)p.a + acc);                    // EPI_ACC, EPI_ADDZ
   }
+  // vector bytes per row (the byte count of deco_count_bytes)
+  size_t row_bytes() const {
+    if (EPI == EPI_SET) return sizeof(YT);
+    if (EPI == EPI_ACC) return 2 * sizeof(YT);
+    return sizeof(ZT) + sizeof(YT);
+  }
 };
 
 // y[o] = sum for the Krylov solvers' product with the system matrix: its own kernel
@@ -189,6 +196,7 @@ struct EpiOuter {
   __device__ __forceinline__ Pre pre(int) const { return Pre{}; }
   template <class AT>
   __device__ __forceinline__ void operator()(int o, AT acc, const Pre&) const { y[o] = (double)acc; }
+  size_t row_bytes() const { return sizeof(double); }
 };
 
 // y[o] = z[o*zs] - sum: residual of a row subset (rows zs*o of the full matrix)
@@ -201,6 +209,7 @@ struct EpiStrideRes {
   __device__ __forceinline__ Pre pre(int o) const { return Pre{z[(size_t)o * zs]}; }
   template <class AT>
   __device__ __forceinline__ void operator()(int o, AT acc, const Pre& p) const { y[o] = (YT)((AT)p.a - acc); }
+  size_t row_bytes() const { return sizeof(ZT) + sizeof(YT); }
 };
 
 // EpiStrideRes that also writes x0[o] = 0 + (y[o] - 0)/d[o]: the zero-guess Jacobi-type sweep on
@@ -220,6 +229,7 @@ struct EpiStrideResJ0 {
     y[o] = bo;
     x0[o] = (YT)0 + (bo - (YT)0) / p.d;
   }
+  size_t row_bytes() const { return sizeof(ZT) + 3 * sizeof(YT); }
 };
 
 // x access of the stream kernel: a plain vector, or a functor that computes x_j (fused steps)
@@ -326,6 +336,8 @@ void spmv_stream_g(const deco_csr* A, const VT* v, XG x, E epi) {
   hipLaunchKernelGGL((k_spmv_stream<VT, XG, E>), dim3(A->nchunks), dim3(stream_block<VT>()), 0, stream(), desc,
                      A->r.p, A->c.p, v, x, epi, (const int*)nullptr);
   DECO_CHECK_LAUNCH();
+  count_bytes((double)A->nnz * (sizeof(VT) + 4) + 4.0 * (A->n + 1) + 8.0 * (A->nchunks + 1) +
+              (double)sizeof(typename XG::type) * A->m + (double)A->n * epi.row_bytes());
 }
 
 // the same on the 16-bit chunk columns when A has them (deco_csr::cols16), else as above
@@ -340,6 +352,8 @@ void spmv_stream_c16(const deco_csr* A, const VT* v, XG x, E epi) {
   hipLaunchKernelGGL((k_spmv_stream<VT, XG, E, unsigned short>), dim3(A->nchunks), dim3(stream_block<VT>()), 0,
                      stream(), desc, A->r.p, (const unsigned short*)A->c16.p, v, x, epi, (const int*)A->cbase.p);
   DECO_CHECK_LAUNCH();
+  count_bytes((double)A->nnz * (sizeof(VT) + 2) + 4.0 * (A->n + 1) + 12.0 * A->nchunks + 8.0 +
+              (double)sizeof(typename XG::type) * A->m + (double)A->n * epi.row_bytes());
 }
 
 template <class VT, class XT, class E>
@@ -648,6 +662,8 @@ void bsr_stream_v(const deco_bsr* B, const VT* bv, const XT* x, E epi) {
     else fail(DECO_ERR_UNSUPPORTED, "block shape %d x %d", B->R, B->C);
 #undef DECO_SELL_LAUNCH
     DECO_CHECK_LAUNCH();
+    count_bytes((double)B->nnzb * (B->R * B->C * sizeof(VT) + (B->d16 ? 2 : 4)) + 8.0 * B->nslices +
+                (double)sizeof(XT) * B->mb * B->C + (double)B->nb * B->R * epi.row_bytes());
     return;
   }
 #define DECO_BSR_LAUNCH(RR, CC)                                                                                  \
@@ -661,6 +677,8 @@ void bsr_stream_v(const deco_bsr* B, const VT* bv, const XT* x, E epi) {
   else fail(DECO_ERR_UNSUPPORTED, "block shape %d x %d", B->R, B->C);
 #undef DECO_BSR_LAUNCH
   DECO_CHECK_LAUNCH();
+  count_bytes((double)B->nnzb * (B->R * B->C * sizeof(VT) + 4) + 4.0 * (B->nb + 1) + 8.0 * (B->nchunks + 1) +
+              (double)sizeof(XT) * B->mb * B->C + (double)B->nb * B->R * epi.row_bytes());
 }
 
 // Scalar product y = A x (epilogue e) with VT values: through A's sliced 1 x 1 shadow when its

@@ -65,6 +65,7 @@ void convert(const A* a, B* b, int64_t n) {
   if (n <= 0) return;
   hipLaunchKernelGGL((k_convert<A, B>), dim3(grid_for(n, 256)), dim3(256), 0, stream(), n, a, b);
   DECO_CHECK_LAUNCH();
+  count_bytes((double)n * (sizeof(A) + sizeof(B)));
 }
 
 struct Identity : deco_prec {

@@ -100,7 +100,31 @@ void prof_stop() {
 
 using namespace deco;
 
+namespace deco {
+ByteCount& byte_count() {
+  static ByteCount c;
+  return c;
+}
+}  // namespace deco
+
 extern "C" {
+
+int deco_count_bytes(int on) {
+  return guard([&] {
+    ByteCount& c = byte_count();
+    c.on = on != 0;
+    c.bytes = 0.0;
+    c.launches = 0;
+  });
+}
+
+int deco_counted_bytes(double* bytes, long long* launches) {
+  return guard([&] {
+    const ByteCount& c = byte_count();
+    if (bytes) *bytes = c.bytes;
+    if (launches) *launches = c.launches;
+  });
+}
 
 int deco_profile_enable(int on) {
   return guard([&] {

@@ -318,6 +318,7 @@ struct Solver {
   double dot(const double* a, const double* b) {
     hipLaunchKernelGGL((k_dots<1>), dim3(nb), dim3(kRedBlock), 0, stream(), n, a, b, nullptr, nullptr, rs.part.p);
     DECO_CHECK_LAUNCH();
+    count_bytes(16.0 * n + 8.0 * nb);
     double s;
     host_sums(1, &s);
     return s;
@@ -326,19 +327,23 @@ struct Solver {
   void dot2(const double* a0, const double* b0, const double* a1, const double* b1, double* out) {
     hipLaunchKernelGGL((k_dots<2>), dim3(nb), dim3(kRedBlock), 0, stream(), n, a0, b0, a1, b1, rs.part.p);
     DECO_CHECK_LAUNCH();
+    count_bytes(32.0 * n + 16.0 * nb);
     host_sums(2, out);
   }
   void scale(double a, const double* x, double* y) {
     hipLaunchKernelGGL(k_scale, dim3(g256()), dim3(256), 0, stream(), n, a, x, y);
     DECO_CHECK_LAUNCH();
+    count_bytes(16.0 * n);
   }
   void add(const double* z, double* x) {
     hipLaunchKernelGGL(k_add, dim3(g256()), dim3(256), 0, stream(), n, z, x);
     DECO_CHECK_LAUNCH();
+    count_bytes(24.0 * n);
   }
   void axpby(double a, const double* x, double b, double* y) {
     hipLaunchKernelGGL(k_axpby2, dim3(g256()), dim3(256), 0, stream(), n, a, x, b, y);
     DECO_CHECK_LAUNCH();
+    count_bytes(24.0 * n);
   }
 };
 
@@ -426,10 +431,10 @@ static void gmres(bool left, const deco_sol_params& sp, const AOp& A, const MOp&
   if (!left) {
     bnorm = S.nrm2(b);
     absTol = tol * bnorm;
-    if (!sp.use_x0) { DECO_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, stream())); copy_d2d(r.p, b, n); }
+    if (!sp.use_x0) { zero_async(x, sizeof(double) * n); copy_d2d(r.p, b, n); }
     else { A.residual(x, b, r.p); ++nspmv; }
   } else {
-    if (!sp.use_x0) { DECO_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, stream())); M.apply(b, r.p); ++napp; }
+    if (!sp.use_x0) { zero_async(x, sizeof(double) * n); M.apply(b, r.p); ++napp; }
     else { A.residual(x, b, z.p); ++nspmv; M.apply(z.p, r.p); ++napp; }
     bnorm = S.nrm2(r.p);
     absTol = tol * bnorm;
@@ -459,16 +464,19 @@ static void gmres(bool left, const deco_sol_params& sp, const AOp& A, const MOp&
       hipLaunchKernelGGL(k_mgs_step, dim3(nb), dim3(kRedBlock), 0, stream(), n, (const double*)nullptr, 0,
                          (const double*)nullptr, w.p, (const double*)V.p, slot[0], (double*)nullptr);
       DECO_CHECK_LAUNCH();
+      count_bytes(8.0 * n + 8.0 * nb);
       for (int i = 1; i <= j + 1; ++i) {
         const double* vprev = V.p + (size_t)(i - 1) * n;
         const double* vcur = (i <= j) ? V.p + (size_t)i * n : nullptr;
         hipLaunchKernelGGL(k_mgs_step, dim3(nb), dim3(kRedBlock), 0, stream(), n, (const double*)slot[(i - 1) & 1], nb,
                            vprev, w.p, vcur, slot[i & 1], hcol.p + (i - 1));
         DECO_CHECK_LAUNCH();
+        count_bytes((vcur ? 32.0 : 24.0) * n + 16.0 * nb);
       }
       hipLaunchKernelGGL(k_mgs_norm, dim3(nb), dim3(kRedBlock), 0, stream(), n, (const double*)slot[(j + 1) & 1], nb,
                          (const double*)w.p, V.p + (size_t)(j + 1) * n, hcol.p + (j + 1));
       DECO_CHECK_LAUNCH();
+      count_bytes(16.0 * n + 8.0 * nb);
       DECO_HIP(hipMemcpyAsync(hh.data(), hcol.p, sizeof(double) * (j + 2), hipMemcpyDeviceToHost, stream()));
       sync();
       for (int i = 0; i <= j + 1; ++i) Hm(i, j) = hh[i];
@@ -492,6 +500,7 @@ static void gmres(bool left, const deco_sol_params& sp, const AOp& A, const MOp&
       hipLaunchKernelGGL(k_combine<double>, dim3(S.g256()), dim3(256), 0, stream(), n, j, (const double*)V.p,
                          (int64_t)n, (const double*)gdev.p, w.p);
     DECO_CHECK_LAUNCH();
+    count_bytes((double)n * (j * (flex ? 4 : 8) + 8) + 8.0 * j);
     if (flex) {
       if (j > 0) S.add(w.p, x);
       A.residual(x, b, r.p); ++nspmv;
@@ -536,12 +545,18 @@ static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, cons
   const double tol = sp.reltol;
   const int m = sp.restart, itmax = sp.itmax;
   if (m <= 0) fail(DECO_ERR_ARG, "restart must be positive");
-  const bool flex = !left && M.M && M.M->precision == DECO_PREC_FP32;
+  // flexible directions: always with an FP32 preconditioner, and with an FP64 one when the
+  // caller asks (sp.flexible), kept in FP64 then; the same Krylov space in exact arithmetic,
+  // and no final M^{-1}(V y) application
+  const bool flex32 = !left && M.M && M.M->precision == DECO_PREC_FP32;
+  const bool flex64 = !left && M.M && !flex32 && sp.flexible;
+  const bool flex = flex32 || flex64;
   const int nbk = (int)std::max<int64_t>(1, ((int64_t)n + kArnTile - 1) / kArnTile);
   WsRef r = ws(0, n), z = ws(1, n), w = ws(2, n), ua = ws(7, n), ub = ws(8, n), V = ws(3, (size_t)n * (m + 1)),
         gdev = ws(4, m + 1);
   WsRef part = ws(9, (size_t)(2 * m + 2) * nbk), redd = ws(10, 2 * m + 2);
-  float* Z = flex ? flex_basis(n, m) : nullptr;
+  float* Z = flex32 ? flex_basis(n, m) : nullptr;
+  double* Z64 = flex64 ? ws(11, (size_t)n * m).p : nullptr;
   const size_t ld = (size_t)m + 1;
   std::vector<double> Hr(ld * m), Rm(ld * m), T(flex ? ld * m : 0), g(m + 1), cs(m), sn(m), hs(m + 1);
   double* red = ctx().pinned;
@@ -550,19 +565,23 @@ static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, cons
   if (!left) {
     bnorm = S.nrm2(b);
     absTol = tol * bnorm;
-    if (!sp.use_x0) { DECO_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, stream())); copy_d2d(r.p, b, n); }
+    if (!sp.use_x0) { zero_async(x, sizeof(double) * n); copy_d2d(r.p, b, n); }
     else { A.residual(x, b, r.p); ++nspmv; }
   } else {
-    if (!sp.use_x0) { DECO_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, stream())); M.apply(b, r.p); ++napp; }
+    if (!sp.use_x0) { zero_async(x, sizeof(double) * n); M.apply(b, r.p); ++napp; }
     else { A.residual(x, b, z.p); ++nspmv; M.apply(z.p, r.p); ++napp; }
     bnorm = S.nrm2(r.p);
     absTol = tol * bnorm;
   }
   // w = B u_j for step j (B = A M^{-1} right, M^{-1} A left); z / Z column j hold M^{-1} u_j
   auto apply_op = [&](int jj, const double* uu) {
-    if (!left && flex) {
+    if (flex32) {
       float* zj = Z + (size_t)jj * n;
       M.apply_f32(uu, zj);
+      A.apply(zj, w.p);
+    } else if (flex64) {
+      double* zj = Z64 + (size_t)jj * n;
+      M.apply(uu, zj);
       A.apply(zj, w.p);
     } else if (!left) {
       M.apply(uu, z.p);
@@ -590,15 +609,18 @@ static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, cons
       hipLaunchKernelGGL(k_arn_dots, dim3(nbk), dim3(kArnBlock), sizeof(double) * 4 * (2 * j + 2), stream(), n, j,
                          (const double*)V.p, uu, wp, part.p, nbk);
       DECO_CHECK_LAUNCH();
+      count_bytes(8.0 * n * (j + (wp ? 2 : 1)) + 8.0 * nv * nbk);
       red_seq = ++ctx().tag_seq;
       red_nv = nv;
       hipLaunchKernelGGL(k_arn_reduce, dim3(nv), dim3(256), 0, stream(), nbk, (const double*)part.p, redd.p,
                          ctx().pinned_dev, ctx().tags_dev, red_seq);
       DECO_CHECK_LAUNCH();
+      count_bytes(8.0 * nv * (nbk + 2));
       if (wp) {   // queued before the host decision; unused if the cycle ends here
         hipLaunchKernelGGL(k_arn_update, dim3(nbk), dim3(kArnBlock), 0, stream(), n, j, (const double*)V.p, uu, wp,
                            (const double*)redd.p, V.p + (size_t)j * n, un);
         DECO_CHECK_LAUNCH();
+        count_bytes(8.0 * n * (j + 4) + 8.0 * (2 * j + 2));
       }
     };
     // finalise column j-1 with u_j = Q s + rho v_j and rotate it (rgmres.tpp:80-93); true when
@@ -704,13 +726,17 @@ static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, cons
         }
       }
       copy_h2d(gdev.p, g.data(), k);
-      if (flex)
+      if (flex32)
         hipLaunchKernelGGL(k_combine<float>, dim3(S.g256()), dim3(256), 0, stream(), n, k, (const float*)Z,
+                           (int64_t)n, (const double*)gdev.p, w.p);
+      else if (flex64)
+        hipLaunchKernelGGL(k_combine<double>, dim3(S.g256()), dim3(256), 0, stream(), n, k, (const double*)Z64,
                            (int64_t)n, (const double*)gdev.p, w.p);
       else
         hipLaunchKernelGGL(k_combine<double>, dim3(S.g256()), dim3(256), 0, stream(), n, k, (const double*)V.p,
                            (int64_t)n, (const double*)gdev.p, w.p);
       DECO_CHECK_LAUNCH();
+      count_bytes((double)n * (k * (flex32 ? 4 : 8) + 8) + 8.0 * k);
       if (flex || left) S.add(w.p, x);
       else { M.apply(w.p, z.p); ++napp; S.add(z.p, x); }
     }
@@ -741,7 +767,7 @@ static void bicgstab(bool left, const deco_sol_params& sp, const AOp& A, const M
   float it = 0.0f;
   int flag = 1, napp = 0, nspmv = 0;
   double norm_b, tol, res_norm;
-  if (!sp.use_x0) DECO_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, stream()));
+  if (!sp.use_x0) zero_async(x, sizeof(double) * n);
   if (!left) {
     norm_b = S.nrm2(b);
     tol = norm_b * reltol;
@@ -767,6 +793,7 @@ static void bicgstab(bool left, const deco_sol_params& sp, const AOp& A, const M
       hipLaunchKernelGGL(k_bicg_p, dim3(S.g256()), dim3(256), 0, stream(), n, beta, -beta * omega,
                          (const double*)r.p, (const double*)v.p, p.p);
       DECO_CHECK_LAUNCH();
+      count_bytes(32.0 * n);
     }
     if (!left) { M.apply(p.p, y.p); A.apply(y.p, v.p); }
     else { A.apply(p.p, y.p); M.apply(y.p, v.p); }
@@ -780,6 +807,7 @@ static void bicgstab(bool left, const deco_sol_params& sp, const AOp& A, const M
                          (const double*)v.p, r.p, 0.0, (const double*)nullptr, (double*)nullptr,
                          (const double*)nullptr, S.rs.part.p);
     DECO_CHECK_LAUNCH();
+    count_bytes((left ? 16.0 : 48.0) * n + 8.0 * nb);
     S.host_sums(1, sums);
     res_norm = std::sqrt(sums[0]);
     if (res_norm <= tol) {
@@ -804,6 +832,7 @@ static void bicgstab(bool left, const deco_sol_params& sp, const AOp& A, const M
                          (const double*)t.p, r.p, alpha * omega, (const double*)y.p, x, (const double*)rt.p,
                          S.rs.part.p);
     DECO_CHECK_LAUNCH();
+    count_bytes(56.0 * n + 16.0 * nb);
     S.host_sums(2, sums);
     res_norm = std::sqrt(sums[0]);
     next_rho = sums[1];
@@ -827,7 +856,7 @@ static void pcg(const deco_sol_params& sp, const AOp& A, const MOp& M, const dou
   WsRef r = ws(0, n), p = ws(2, n), z = ws(1, n), g = ws(7, n);
   const double norm_b = S.nrm2(b), tol = norm_b * sp.reltol;
   int it = 0, napp = 0, nspmv = 0;
-  if (!sp.use_x0) { DECO_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, stream())); copy_d2d(r.p, b, n); }
+  if (!sp.use_x0) { zero_async(x, sizeof(double) * n); copy_d2d(r.p, b, n); }
   else { A.residual(x, b, r.p); ++nspmv; }
   M.apply(r.p, p.p); ++napp;
   double rho1 = S.dot(r.p, p.p);
@@ -989,6 +1018,7 @@ void Jacobi::apply(const double* r, double* x) {
     hipLaunchKernelGGL((k_jacobi_apply<double, double, false>), dim3(grid_for(n, 256)), dim3(256), 0, stream(), n,
                        bs, (const double*)J.p, r, x);
   DECO_CHECK_LAUNCH();
+  count_bytes((double)n * (bs * (precision == DECO_PREC_FP32 ? 4 : 8) + 16));
 }
 void Jacobi::apply_t(const double* r, double* x) {
   if (precision == DECO_PREC_FP32)
@@ -998,16 +1028,19 @@ void Jacobi::apply_t(const double* r, double* x) {
     hipLaunchKernelGGL((k_jacobi_apply<double, double, true>), dim3(grid_for(n, 256)), dim3(256), 0, stream(), n,
                        bs, (const double*)J.p, r, x);
   DECO_CHECK_LAUNCH();
+  count_bytes((double)n * (bs * (precision == DECO_PREC_FP32 ? 4 : 8) + 16));
 }
 void Jacobi::apply32(const float* r, float* x) {
   hipLaunchKernelGGL((k_jacobi_apply<float, float, false>), dim3(grid_for(n, 256)), dim3(256), 0, stream(), n, bs,
                      (const float*)Jf.p, r, x);
   DECO_CHECK_LAUNCH();
+  count_bytes((double)n * (bs * 4 + 8));
 }
 void Jacobi::apply32_t(const float* r, float* x) {
   hipLaunchKernelGGL((k_jacobi_apply<float, float, true>), dim3(grid_for(n, 256)), dim3(256), 0, stream(), n, bs,
                      (const float*)Jf.p, r, x);
   DECO_CHECK_LAUNCH();
+  count_bytes((double)n * (bs * 4 + 8));
 }
 
 // ------------------------------------------------------------------ adjoint helpers ----
@@ -1129,6 +1162,7 @@ int deco_grad_A_pattern(const deco_csr* A, const double* lam, const double* x, d
     if (A->n == 0) return;
     hipLaunchKernelGGL(k_grad_A, dim3(grid_for(A->n, 256)), dim3(256), 0, stream(), A->n, A->r.p, A->c.p, lam, x, dA);
     DECO_CHECK_LAUNCH();
+    count_bytes((double)A->nnz * 12 + 4.0 * (A->n + 1) + 16.0 * A->n);
     sync();
   });
 }

@@ -128,6 +128,11 @@ typedef struct deco_sol_params {
   int    ortho;             /* GMRES Gram-Schmidt: DECO_ORTHO_DCGS2 (default) or DECO_ORTHO_MGS, the
                                reference's modified Gram-Schmidt order (rgmres.tpp:70-78); no
                                counterpart key in the reference's "sol" tree */
+  int    flexible;          /* right GMRES with the default Gram-Schmidt: 1 keeps the directions
+                               M^{-1} u_j (flexible GMRES) also with an FP64 preconditioner, so no
+                               final M^{-1}(V y) application; 0 (default) keeps the form of
+                               rgmres.tpp:106-118 there. An FP32 preconditioner always runs
+                               flexible. No counterpart key in the reference's "sol" tree */
 } deco_sol_params;
 
 #define DECO_ORTHO_DCGS2 0   /* one reduction per step, delayed reorthogonalisation */
@@ -327,6 +332,15 @@ int deco_profile_start(const char* name);
 int deco_profile_stop(void);
 int deco_profile_time(const char* name, double* seconds);
 int deco_profile_report(char* buf, size_t len, size_t* needed);
+/* Algorithmic byte count of the device work queued from now on (on != 0; 0 stops counting; both
+ * reset the count): every product adds the bytes it must move at least once in the storage it
+ * reads -- stored values and column indices (16-bit where the product reads 16-bit offsets), row
+ * pointers or slice descriptors, one pass over x and over every vector its epilogue reads or
+ * writes -- and every vector kernel, copy and fill the vectors it reads and writes once. The
+ * strided products of deco_spmv (bs > 1) are not counted. Divided by a solve's time it gives the
+ * solve's achieved algorithmic bandwidth. No reference counterpart. */
+int deco_count_bytes(int on);
+int deco_counted_bytes(double* bytes, long long* launches);
 /* The solvers keep their work vectors (Krylov basis, flexible directions, partial sums) between
  * calls and grow them on demand; this frees them. No reference counterpart (the reference
  * allocates per solve, solver/rgmres.tpp:18-29). */
