@@ -109,6 +109,7 @@ SIGNATURES = {
     "deco_spgemm": (i, [vp, vp, pp]),
     "deco_spgemm_tril_pattern": (i, [vp, pp]),
     "deco_dot": (i, [i, vp, vp, dp]),
+    "deco_dot_noalloc": (i, [i, vp, vp, dp]),
     "deco_nrm2": (i, [i, vp, dp]),
     "deco_axpby": (i, [i, d, vp, d, vp]),
     "deco_cpr_create": (i, [C.POINTER(CprParams), vp, vp, pp]),

@@ -1,6 +1,7 @@
 // BLAS-1 entry points (alg::dot core/alg.hpp:278-307, alg::l2norm :334-352, alg::axpby :185-190).
 #include "blas.hpp"
 #include <cmath>
+#include <mutex>
 
 namespace deco {
 
@@ -40,6 +41,40 @@ double nrm2_host(int n, const double* x, RedScratch& rs) {
   return h;
 }
 
+// ------------------------------------------------------------------ allocation-free dot --
+// deco_dot_noalloc keeps the block partials of its reduction and the reduced value in this
+// file-scope device array instead of library buffers: red_grid() caps any length at kRedMaxGrid
+// partials, so kNoAllocDoubles (1056 doubles, 8448 bytes) holds every call. Each GPU the library
+// runs on holds its own copy until the process ends. In host code the name `scratch` is not a
+// device address: each device's address comes from hipGetSymbolAddress (scratch_dev) and only
+// that address goes to kernels. Nothing in it needs clearing: every partial a call reads is
+// written by the same call. Calls that overlap on one GPU share the array and race.
+constexpr int kNoAllocDoubles = kRedMaxGrid + 32;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+constexpr int kNoAllocDevices = 64;
+static double* g_scratch_dev[kNoAllocDevices] = {};
+static std::mutex g_scratch_mu;
+
+// the library device's address of `scratch`, looked up once per device with that device current
+static double* scratch_dev() {
+  const int dev = ctx().device;
+  if (dev < 0 || dev >= kNoAllocDevices)
+    fail(DECO_ERR_DEVICE, "device %d is beyond the %d the allocation-free dot tracks", dev, kNoAllocDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    int cur = -1;
+    DECO_HIP(hipGetDevice(&cur));
+    if (cur != dev) DECO_HIP(hipSetDevice(dev));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !p)
+      fail(DECO_ERR_DEVICE, "hipGetSymbolAddress of the dot scratch failed: %s", hipGetErrorString(e));
+    g_scratch_dev[dev] = static_cast<double*>(p);
+  }
+  return g_scratch_dev[dev];
+}
+
 }  // namespace deco
 
 using namespace deco;
@@ -53,6 +88,29 @@ int deco_dot(int n, const double* x, const double* y, double* result) {
     need(y, n, "y");
     RedScratch rs;
     *result = dot_host(n, x, y, rs);
+  });
+}
+
+int deco_dot_noalloc(int n, const double* x, const double* y, double* result) {
+  return guard([&] {
+    if (n < 0 || !result) fail(DECO_ERR_ARG, "bad argument");
+    need(x, n, "x");
+    need(y, n, "y");
+    if (!ctx().stream) fail(DECO_ERR_STATE, "deco_init has not run (the allocation-free dot creates nothing)");
+    const int nb = red_grid(n);
+    if (nb > kNoAllocDoubles - 1)
+      fail(DECO_ERR_ARG, "n = %d needs %d partials, more than the %d the scratch holds", n, nb, kNoAllocDoubles - 1);
+    double* part = scratch_dev();
+    double* value = part + (kNoAllocDoubles - 1);
+    hipLaunchKernelGGL(k_dot_part<double>, dim3(nb), dim3(kRedBlock), 0, stream(), n, x, y, part);
+    DECO_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, stream(), (const double*)part, nb, value, 0);
+    DECO_CHECK_LAUNCH();
+    // the init-time pinned host buffer's last slot receives the value (no staging allocation)
+    double* host = ctx().pinned + 4095;
+    DECO_HIP(hipMemcpyAsync(host, value, sizeof(double), hipMemcpyDeviceToHost, stream()));
+    DECO_HIP(hipStreamSynchronize(stream()));
+    *result = *host;
   });
 }
 

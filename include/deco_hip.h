@@ -249,6 +249,14 @@ int deco_spgemm_tril_pattern(const deco_csr* A, deco_csr** F);
 /* Deterministic fixed-grid reductions (alg::dot core/alg.hpp:278-307, alg::l2norm :334-373). */
 int deco_dot(int n, const double* x_dev, const double* y_dev, double* result);
 int deco_nrm2(int n, const double* x_dev, double* result);
+/* deco_dot without allocating: no reference counterpart; for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). The same kernels, grid and result as deco_dot, with the block partials and the
+ * value in a file-scope device array of 8448 bytes per GPU (any n: the reduction grid is capped
+ * at 1024 partials), looked up once per device; the value reaches the host through the pinned
+ * buffer deco_init allocates. deco_init must have run (DECO_ERR_STATE otherwise). Calls that
+ * overlap on one GPU share the array and race. */
+int deco_dot_noalloc(int n, const double* x_dev, const double* y_dev, double* result);
 /* y = b*y + a*x (alg::axpby core/alg.hpp:185-190). */
 int deco_axpby(int n, double a, const double* x_dev, double b, double* y_dev);
 
