@@ -231,6 +231,8 @@ class CSR : public std::enable_shared_from_this<CSR<R, C, V>> {
   }
   ~CSR() {
     if (owns_data) { std::free(r); std::free(c); std::free(v); }
+    std::free(displs_nr);
+    std::free(counts_nr);
     if (dev) deco_csr_destroy(dev);
   }
 
@@ -444,6 +446,13 @@ struct PROFILE {
     return t;
   }
   void reset() { deco_facade::check(deco_profile_reset(), "profile"); }
+  // PROFILE::post_process (core/profiler.hpp:159-175) subtracts the matrix-format conversions of
+  // hypre ("Hypre_CSR2IJMatrix") and Chronos from the setup phases; this library records no such
+  // phase (its AMG reads the device CSR directly), so there is nothing to subtract
+  void post_process() const {
+    const double conv = getTime("Hypre_CSR2IJMatrix") + getTime("Chronos_build_dsmat");
+    if (conv != 0.0) throw std::logic_error("profile: unexpected matrix-format conversion phase");
+  }
   void report() const {
     size_t need = 0;
     deco_facade::check(deco_profile_report(nullptr, 0, &need), "profile");

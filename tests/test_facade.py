@@ -61,8 +61,59 @@ int main() {
 
 
 def test_facade_compiles_against_header():
-    subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
-                    os.path.join(CPP, "cpr_driver.cpp")], check=True)
+    subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), "-I",
+                    os.path.join(CPP, "compat"), os.path.join(CPP, "cpr_driver.cpp")], check=True)
+
+
+ONE_RANK_PROBE = r"""
+#include "core/pch.h"
+#include <cassert>
+int main(int argc, char* argv[]) {
+  int mid = -1, np = -1;
+  init_MPI(&mid, &np, &argc, &argv);                       // cpr.cpp:12
+  assert(mid == 0 && np == 1);
+  assert(MPI_Comm_set_errhandler(deco_comm, MPI_ERRORS_RETURN) == MPI_SUCCESS);
+  boost::property_tree::ptree param = IO::read_ptree(argc, argv);
+  assert(param.get<int>("prec.block_size") == 2);
+  auto A = std::make_shared<CSR<int, int, double>>(4, 4, 4);
+  for (int i = 0; i <= 4; ++i) A->r[i] = i;
+  for (int i = 0; i < 4; ++i) { A->c[i] = i; A->v[i] = 1.0 + i; }
+  double* b = memres<double>(4);
+  for (int i = 0; i < 4; ++i) b[i] = 10.0 * i;
+  auto Aloc = IO::distribute_csr_matrix(A, mid, np, 2);   // cpr.cpp:47
+  assert(Aloc.get() == A.get() && Aloc->counts_nr[0] == 4 && Aloc->displs_nr[0] == 0);
+  double* b_loc = memres<double>(Aloc->n);
+  assert(MPI_Scatterv(b, Aloc->counts_nr, Aloc->displs_nr, get_mpi_type<double>(), b_loc, Aloc->n,
+                      get_mpi_type<double>(), 0, deco_comm) == MPI_SUCCESS);
+  for (int i = 0; i < 4; ++i) assert(b_loc[i] == b[i]);
+  assert(MPI_Barrier(deco_comm) == MPI_SUCCESS);
+  bool threw = false;
+  try { IO::distribute_csr_matrix(A, 0, 2, 2); } catch (const std::invalid_argument&) { threw = true; }
+  assert(threw);
+  profiler.post_process();
+  std::free(b);
+  std::free(b_loc);
+  assert(MPI_Finalize() == MPI_SUCCESS);
+  std::puts("one-rank calls ok");
+  return 0;
+}
+"""
+
+
+def test_one_rank_calls_of_the_reference_driver(tmp_path):
+    """The set-up calls the reference's driver makes around the solve (cpr.cpp:12-13,17,47-53,84,88:
+    init_MPI, MPI_Comm_set_errhandler, IO::read_ptree, IO::distribute_csr_matrix, memres,
+    MPI_Scatterv with get_mpi_type, MPI_Barrier, profiler.post_process, MPI_Finalize) with the
+    reference's arguments, through compat/core/pch.h, in their one-rank meaning; more than one rank
+    is an error. Host only (the library is linked, no device call is made)."""
+    src = tmp_path / "probe.cpp"
+    src.write_text(ONE_RANK_PROBE)
+    exe = tmp_path / "probe"
+    lib = os.path.join(ROOT, "deco_amd")
+    subprocess.run(["g++", "-std=c++17", "-O0", "-I", os.path.join(ROOT, "include"), "-I", os.path.join(CPP, "compat"),
+                    str(src), "-o", str(exe), "-L", lib, "-ldeco_hip", f"-Wl,-rpath,{lib}"], check=True)
+    res = subprocess.run([str(exe), PARAM], capture_output=True, text=True, timeout=60)
+    assert res.returncode == 0 and "one-rank calls ok" in res.stdout, res.stdout + res.stderr
 
 
 @pytest.mark.gpu
@@ -162,3 +213,49 @@ int main(int argc, char** argv) {
                     "-L", lib_dir, "-ldeco_hip", "-Wl,-rpath," + lib_dir], check=True)
     res = subprocess.run([str(exe), str(tmp_path)], capture_output=True, text=True, timeout=60)
     assert res.returncode == 0 and "round-trip-ok" in res.stdout, res.stdout + res.stderr
+
+
+@pytest.mark.gpu
+def test_profiler_records_the_reference_phases_of_a_device_solve(gpu):
+    """The phase tree of a CPR setup and solve on the device carries the reference's labels
+    (precond/CPR.hpp:47-170, precond/FSAI/FSAI.hpp:42-45, precond/hypre_AMG/hypre_AMG.hpp:27,
+    CPR.hpp:240-291, and the driver's "setup & solve" / "Prec" / "Solve", cpr.cpp:57-75) nested as
+    the reference nests them, with times consistent with the solve's device-timed duration."""
+    import deco_amd
+    from conftest import gen_system, small_params
+    g = gen_system(1)
+    n = g["n"]
+    A = deco_amd.CSR(n, n, g["r"], g["c"], g["v"])
+    b = deco_amd.DeviceArray.from_numpy(g["b"])
+    sol = {"type": "rgmres", "reltol": "1e-8", "restart": "30", "itmax": "300", "verbosity": "0"}
+    deco_amd.profile_reset()
+    deco_amd.profile_enable(True)
+    try:
+        deco_amd.profile_start("setup & solve")
+        deco_amd.profile_start("Prec")
+        M = deco_amd.CPR(small_params(2), A, b)
+        deco_amd.profile_stop()
+        deco_amd.profile_start("Solve")
+        S = deco_amd.SOL(sol, A, M, b)
+        deco_amd.profile_stop()
+        deco_amd.profile_stop()
+    finally:
+        deco_amd.profile_enable(False)
+    assert S.flag == 0
+    rep = deco_amd.profile_report()
+    names = [l.split("└── ", 1)[1].rsplit("%", 1)[0].rsplit(None, 1)[0] for l in rep.splitlines()
+             if "└" in l]
+    depth = {nm: (l.index("└") // 4) for nm, l in zip(names, [l for l in rep.splitlines() if "└" in l])}
+    for label in ("setup & solve", "Prec", "decoupling", "quasiimpes weights", "CPD decoup", "Global precond",
+                  "Local  precond", "Solve", "FSAI apply", "AMG apply", "CPR_rest"):
+        assert label in names, (label, rep)
+    assert depth["Prec"] == depth["Solve"] == depth["setup & solve"] + 1
+    assert depth["Global precond"] == depth["Local  precond"] == depth["Prec"] + 1
+    assert depth["FSAI apply"] > depth["Solve"] and depth["AMG apply"] > depth["Solve"]
+    t_solve = deco_amd.profile_time("Solve")
+    parts = deco_amd.profile_time("FSAI apply") + deco_amd.profile_time("AMG apply")
+    # the host clock around a solve that waits for the device at every phase boundary
+    assert t_solve * 1e3 >= 0.9 * S.info.time_ms
+    assert 0.0 < parts <= t_solve
+    assert deco_amd.profile_time("setup & solve") >= deco_amd.profile_time("Prec") + t_solve * 0.999
+    deco_amd.profile_reset()
