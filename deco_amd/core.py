@@ -600,9 +600,8 @@ def set_sell_offsets16(enable):
 
 
 def set_amg_dense_tail(max_rows):
-    """AMG hierarchies set up from now on replace the sub-cycle below their last level with one
-    dense operator when that level has at most max_rows rows (default 2048, at most 4096; 0
-    disables)."""
+    """AMG hierarchies set up from now on replace the whole sub-cycle of their highest level of at
+    most max_rows rows with one dense operator (default 4608, at most 8192; 0 disables)."""
     check(lib().deco_set_amg_dense_tail(int(max_rows)))
 
 

@@ -311,8 +311,86 @@ __global__ __launch_bounds__(256) void k_dense(int n, const MT* __restrict__ C, 
   }
 }
 
-// largest last level (rows) whose sub-cycle is formed as one dense operator at setup
-static int g_dense_tail_rows = 2048;
+// C (M x N) = A (M x K) B (K x N), all row-major FP64 (the dense tail's setup products): 64 x 64
+// output tiles, 16-deep slices of A and B staged in LDS, 4 x 4 outputs per thread
+constexpr int kGemmTile = 64, kGemmDepth = 16;
+__global__ __launch_bounds__(256) void k_gemm64(int M, int N, int K, const double* __restrict__ A,
+                                                const double* __restrict__ B, double* __restrict__ C) {
+  __shared__ double As[kGemmDepth][kGemmTile + 1];
+  __shared__ double Bs[kGemmDepth][kGemmTile + 1];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int row0 = blockIdx.y * kGemmTile, col0 = blockIdx.x * kGemmTile;
+  double acc[4][4] = {};
+  for (int k0 = 0; k0 < K; k0 += kGemmDepth) {
+    for (int e = threadIdx.x; e < kGemmTile * kGemmDepth; e += 256) {
+      const int ar = e / kGemmDepth, ak = e % kGemmDepth;          // A tile: 64 rows x 16
+      const int gr = row0 + ar, gk = k0 + ak;
+      As[ak][ar] = (gr < M && gk < K) ? A[(size_t)gr * K + gk] : 0.0;
+      const int bk = e / kGemmTile, bc = e % kGemmTile;           // B tile: 16 x 64 columns
+      const int hk = k0 + bk, hc = col0 + bc;
+      Bs[bk][bc] = (hk < K && hc < N) ? B[(size_t)hk * N + hc] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int k = 0; k < kGemmDepth; ++k) {
+      double a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = As[k][ty * 4 + i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = Bs[k][tx * 4 + j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int r = row0 + ty * 4 + i;
+    if (r >= M) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int c = col0 + tx * 4 + j;
+      if (c < N) C[(size_t)r * N + c] = acc[i][j];
+    }
+  }
+}
+
+// Z[i, c] += v for the entries of a CSR matrix (n rows, Z n x n row-major)
+__global__ void k_add_csr_dense(int n, const int* __restrict__ r, const int* __restrict__ c,
+                                const double* __restrict__ v, double* __restrict__ Z) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int e = r[i]; e < r[i + 1]; ++e) Z[(size_t)i * n + c[e]] += v[e];
+}
+
+// Zt = Z^T for an n x n row-major matrix, through 32 x 32 LDS tiles
+__global__ __launch_bounds__(256) void k_transpose_dense(int n, const double* __restrict__ Z, double* __restrict__ Zt) {
+  __shared__ double t[32][33];
+  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8 threads
+  for (int k = ty; k < 32; k += 8) {
+    const int r = by + k, c = bx + tx;
+    t[k][tx] = (r < n && c < n) ? Z[(size_t)r * n + c] : 0.0;
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const int r = bx + k, c = by + tx;
+    if (r < n && c < n) Zt[(size_t)r * n + c] = t[tx][k];
+  }
+}
+
+static void gemm64(int M, int N, int K, const double* A, const double* B, double* C) {
+  if (M <= 0 || N <= 0) return;
+  dim3 grid((N + kGemmTile - 1) / kGemmTile, (M + kGemmTile - 1) / kGemmTile);
+  hipLaunchKernelGGL(k_gemm64, grid, dim3(256), 0, stream(), M, N, K, A, B, C);
+  DECO_CHECK_LAUNCH();
+}
+
+// largest level (rows) whose whole sub-cycle, down to the coarsest solve, is formed as one dense
+// operator at setup
+static int g_dense_tail_rows = 4608;
 
 struct AMG : deco_prec {
   struct Level {
@@ -344,6 +422,7 @@ struct AMG : deco_prec {
   // as one dense operator Z, row-major, and its transpose for the transposed cycle (Jacobi-type
   // smoothers, last level of at most g_dense_tail_rows rows); tail = its rows, 0 when absent
   int tail = 0;
+  int tail_lev = -1;          // the level whose sub-cycle the dense operator replaces (-1: none)
   DBuf<double> Z64, Z64t;
   DBuf<float> Z32, Z32t;
   int relax = 18;              // 18 l1-Jacobi, 0 Jacobi (weighted), 17 F-C-F, 11 two-stage Gauss-Seidel
@@ -493,7 +572,7 @@ struct AMG : deco_prec {
     CT.upload(invT.data(), invT.size());
     xc.alloc(nc); bc.alloc(nc); xcf.alloc(nc); bcf.alloc(nc);
     ph_ci.end();
-    if (relax != 17 && !L.empty() && L.back().A->n <= g_dense_tail_rows) form_tail(inv);
+    if (relax != 17 && !L.empty() && L.back().A->n <= g_dense_tail_rows) form_tail();
     sync();
   }
 
@@ -550,31 +629,87 @@ struct AMG : deco_prec {
     return X;
   }
 
-  // Z = Xpre + Xpost - Xpost A Xpre + (P - Xpost A P) C (R - R A Xpre) of the last level, formed on
-  // the host in FP64 and stored in FP64 and FP32 with its transpose (the transposed sub-cycle is
-  // Z^T: R = P^T, and the transposed cycle's smoothers are the transposes of the other side's)
-  void form_tail(const std::vector<double>& inv) {
+  // The sub-cycle of level k (pre-smoother from zero, residual, restriction, the cycle below,
+  // prolongation, post-smoother) is the fixed map Z_k = E + W C G with E = Xpre + Xpost - Xpost A Xpre,
+  // W = P - Xpost A P, G = R - R A Xpre and C the operator of the cycle below (the coarsest inverse,
+  // or Z_{k+1}). Starting from the last level with interpolation, Z is formed for every level of at
+  // most g_dense_tail_rows rows: E, W and G on the host (sparse products), C G and W (C G) on the
+  // device in FP64. The highest such level's Z is kept in FP64 and FP32 with its transpose (R = P^T
+  // and the transposed cycle's smoothers are the transposes of the other side's, so the transposed
+  // sub-cycle is Z^T).
+  void form_tail() {
     ProfPhase ph("dense tail");
-    Level& lv = L.back();
-    const int nl = lv.A->n;
-    amg::HostCSR A, P, R;
-    download(lv.A, A);
-    download(lv.P, P);
-    download(lv.R, R);
-    std::vector<double> d(nl), Z, Zt((size_t)nl * nl);
-    lv.d.download(d.data(), nl);
-    const amg::HostCSR Xpre = smoother_matrix(A, d, dir_down);
-    const amg::HostCSR Xpost = smoother_matrix(A, d, dir_up);
-    amg::dense_tail(A, Xpre, Xpost, P, R, inv, Z);
-    for (int i = 0; i < nl; ++i)
-      for (int k = 0; k < nl; ++k) Zt[(size_t)k * nl + i] = Z[(size_t)i * nl + k];
-    std::vector<float> z32(Z.begin(), Z.end()), zt32(Zt.begin(), Zt.end());
-    Z64.upload(Z.data(), Z.size());
-    Z64t.upload(Zt.data(), Zt.size());
-    Z32.upload(z32.data(), z32.size());
-    Z32t.upload(zt32.data(), zt32.size());
-    sync();   // the host arrays go out of scope here
+    int k = (int)L.size() - 1;
+    while (k > 0 && L[k - 1].A->n <= g_dense_tail_rows) --k;
+    DBuf<double> Cprev, Z, H, W, G;
+    int nC = nc;
+    const double* C = Cinv.p;
+    for (int lev = (int)L.size() - 1; lev >= k; --lev) {
+      Level& lv = L[lev];
+      const int nl = lv.A->n;
+      amg::HostCSR A, P, R;
+      download(lv.A, A);
+      download(lv.P, P);
+      download(lv.R, R);
+      std::vector<double> d(nl);
+      lv.d.download(d.data(), nl);
+      const amg::HostCSR Xpre = smoother_matrix(A, d, dir_down);
+      const amg::HostCSR Xpost = smoother_matrix(A, d, dir_up);
+      const amg::HostCSR AX = amg::host_spgemm(A, Xpre);
+      const amg::HostCSR RAX = amg::host_spgemm(R, AX);
+      const amg::HostCSR XAP = amg::host_spgemm(Xpost, amg::host_spgemm(A, P));
+      const amg::HostCSR XAX = amg::host_spgemm(Xpost, AX);
+      std::vector<double> Wh((size_t)nl * nC, 0.0), Gh((size_t)nC * nl, 0.0);
+      for (int i = 0; i < nl; ++i) {
+        for (int e = P.r[i]; e < P.r[i + 1]; ++e) Wh[(size_t)i * nC + P.c[e]] += P.v[e];
+        for (int e = XAP.r[i]; e < XAP.r[i + 1]; ++e) Wh[(size_t)i * nC + XAP.c[e]] -= XAP.v[e];
+      }
+      for (int i = 0; i < nC; ++i) {
+        for (int e = R.r[i]; e < R.r[i + 1]; ++e) Gh[(size_t)i * nl + R.c[e]] += R.v[e];
+        for (int e = RAX.r[i]; e < RAX.r[i + 1]; ++e) Gh[(size_t)i * nl + RAX.c[e]] -= RAX.v[e];
+      }
+      // E = Xpre + Xpost - Xpost A Xpre as one CSR (duplicate columns within a row are summed by
+      // the scatter-add)
+      std::vector<int> er(nl + 1, 0), ec;
+      std::vector<double> ev;
+      for (int i = 0; i < nl; ++i) {
+        for (int e = Xpre.r[i]; e < Xpre.r[i + 1]; ++e) { ec.push_back(Xpre.c[e]); ev.push_back(Xpre.v[e]); }
+        for (int e = Xpost.r[i]; e < Xpost.r[i + 1]; ++e) { ec.push_back(Xpost.c[e]); ev.push_back(Xpost.v[e]); }
+        for (int e = XAX.r[i]; e < XAX.r[i + 1]; ++e) { ec.push_back(XAX.c[e]); ev.push_back(-XAX.v[e]); }
+        er[i + 1] = (int)ec.size();
+      }
+      W.upload(Wh.data(), Wh.size());
+      G.upload(Gh.data(), Gh.size());
+      DBuf<int> erd, ecd;
+      DBuf<double> evd;
+      erd.upload(er.data(), er.size());
+      ecd.upload(ec.data(), ec.size());
+      evd.upload(ev.data(), ev.size());
+      H.alloc((size_t)nC * nl);
+      gemm64(nC, nl, nC, C, G.p, H.p);                 // C G
+      DBuf<double> Zl((size_t)nl * nl);
+      gemm64(nl, nl, nC, W.p, H.p, Zl.p);              // W (C G)
+      hipLaunchKernelGGL(k_add_csr_dense, dim3(grid_for(nl, 256)), dim3(256), 0, stream(), nl, (const int*)erd.p,
+                         (const int*)ecd.p, (const double*)evd.p, Zl.p);
+      DECO_CHECK_LAUNCH();
+      sync();   // the host arrays go out of scope here
+      Cprev = std::move(Zl);
+      C = Cprev.p;
+      nC = nl;
+    }
+    const int nl = nC;
+    Z64 = std::move(Cprev);
+    Z64t.alloc((size_t)nl * nl);
+    hipLaunchKernelGGL(k_transpose_dense, dim3((nl + 31) / 32, (nl + 31) / 32), dim3(256), 0, stream(), nl,
+                       (const double*)Z64.p, Z64t.p);
+    DECO_CHECK_LAUNCH();
+    Z32.alloc((size_t)nl * nl);
+    Z32t.alloc((size_t)nl * nl);
+    convert((const double*)Z64.p, Z32.p, (int64_t)nl * nl);
+    convert((const double*)Z64t.p, Z32t.p, (int64_t)nl * nl);
+    sync();
     tail = nl;
+    tail_lev = k;
   }
 
   template <class MT>
@@ -633,7 +768,7 @@ struct AMG : deco_prec {
       dense(nc, (const double*)(tr ? CT.p : Cinv.p), b, x, (T*)nullptr, 0);
       return;
     }
-    if (tail && l + 1 == L.size()) {   // the whole sub-cycle of the last level as one product
+    if (tail && (int)l == tail_lev) {   // the whole sub-cycle below as one product
       dense(tail, tail_op(tr, b), b, x, add_to, stride);
       return;
     }
@@ -686,7 +821,7 @@ struct AMG : deco_prec {
     const bool last = (l + 1 == L.size());
     T* bn = last ? BC<T>() : B<T>(L[l + 1]);
     T* xn = last ? XC<T>() : X<T>(L[l + 1]);
-    const bool fuse = !last && relax != 17 && !(tail && l + 2 == L.size());
+    const bool fuse = !last && relax != 17 && !(tail && (int)l + 1 == tail_lev);
     if (fuse)
       spmv_auto<T>(lv.R, r, EpiRestrictJacobi0<T>{bn, D<T>(L[l + 1]), xn});
     else
@@ -882,7 +1017,7 @@ int deco_amg_level(const deco_prec* M, int level, const deco_csr** A, const deco
 
 int deco_set_amg_dense_tail(int max_rows) {
   return guard([&] {
-    if (max_rows < 0 || max_rows > 4096) fail(DECO_ERR_ARG, "max_rows %d outside [0, 4096]", max_rows);
+    if (max_rows < 0 || max_rows > 8192) fail(DECO_ERR_ARG, "max_rows %d outside [0, 8192]", max_rows);
     g_dense_tail_rows = max_rows;
   });
 }

@@ -411,51 +411,5 @@ HostCSR host_spgemm(const HostCSR& A, const HostCSR& B) {
   return C;
 }
 
-void dense_tail(const HostCSR& A, const HostCSR& Xpre, const HostCSR& Xpost, const HostCSR& P, const HostCSR& R,
-                const std::vector<double>& C, std::vector<double>& Z) {
-  const int n = A.n, nc = R.n;
-  const HostCSR AX = host_spgemm(A, Xpre);        // A Xpre
-  const HostCSR RAX = host_spgemm(R, AX);         // R A Xpre
-  const HostCSR XAX = host_spgemm(Xpost, AX);     // Xpost A Xpre
-  const HostCSR XAP = host_spgemm(Xpost, host_spgemm(A, P));   // Xpost A P
-  // G = R - R A Xpre (nc x n): the restriction of the residual the pre-smoother leaves
-  std::vector<double> G((size_t)nc * n, 0.0);
-  for (int i = 0; i < nc; ++i) {
-    double* g = &G[(size_t)i * n];
-    for (int e = R.r[i]; e < R.r[i + 1]; ++e) g[R.c[e]] += R.v[e];
-    for (int e = RAX.r[i]; e < RAX.r[i + 1]; ++e) g[RAX.c[e]] -= RAX.v[e];
-  }
-  // H = C G: the coarse correction
-  std::vector<double> H((size_t)nc * n, 0.0);
-#pragma omp parallel for schedule(static)
-  for (int i = 0; i < nc; ++i) {
-    double* h = &H[(size_t)i * n];
-    for (int k = 0; k < nc; ++k) {
-      const double c = C[(size_t)i * nc + k];
-      const double* g = &G[(size_t)k * n];
-      for (int j = 0; j < n; ++j) h[j] += c * g[j];
-    }
-  }
-  // Z = (P - Xpost A P) H + Xpre + Xpost - Xpost A Xpre
-  Z.assign((size_t)n * n, 0.0);
-#pragma omp parallel for schedule(dynamic, 16)
-  for (int i = 0; i < n; ++i) {
-    double* z = &Z[(size_t)i * n];
-    for (int e = P.r[i]; e < P.r[i + 1]; ++e) {
-      const double s = P.v[e];
-      const double* h = &H[(size_t)P.c[e] * n];
-      for (int j = 0; j < n; ++j) z[j] += s * h[j];
-    }
-    for (int e = XAP.r[i]; e < XAP.r[i + 1]; ++e) {
-      const double s = XAP.v[e];
-      const double* h = &H[(size_t)XAP.c[e] * n];
-      for (int j = 0; j < n; ++j) z[j] -= s * h[j];
-    }
-    for (int e = Xpre.r[i]; e < Xpre.r[i + 1]; ++e) z[Xpre.c[e]] += Xpre.v[e];
-    for (int e = Xpost.r[i]; e < Xpost.r[i + 1]; ++e) z[Xpost.c[e]] += Xpost.v[e];
-    for (int e = XAX.r[i]; e < XAX.r[i + 1]; ++e) z[XAX.c[e]] -= XAX.v[e];
-  }
-}
-
 }  // namespace amg
 }  // namespace deco

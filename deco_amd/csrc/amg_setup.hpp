@@ -54,16 +54,6 @@ HostCSR interp_extpi(const HostCSR& A, const std::vector<char>& strong, const st
 // Dense inverse of a small matrix by LU with partial pivoting; returns false if singular.
 bool dense_inverse(const HostCSR& A, std::vector<double>& inv_rowmajor);
 
-// The sub-cycle below the last level with interpolation, whose next level is the coarsest, as one
-// dense n x n operator (row-major). With the smoothers as stationary steps -- the pre-smoother from
-// a zero guess x1 = Xpre b, the post-smoother x <- x + Xpost (b - A x) -- and C the coarsest inverse
-// (row-major, R.n x R.n):
-//   Z = Xpre + Xpost - Xpost A Xpre + (P - Xpost A P) C (R - R A Xpre).
-// Jacobi-type smoothers have Xpre = Xpost = D^{-1}; two-stage Gauss-Seidel has the truncated
-// Neumann series D^{-1} - D^{-1} T D^{-1} (+ D^{-1} T D^{-1} T D^{-1}) of (D + T)^{-1}.
-void dense_tail(const HostCSR& A, const HostCSR& Xpre, const HostCSR& Xpost, const HostCSR& P, const HostCSR& R,
-                const std::vector<double>& C, std::vector<double>& Z);
-
 // C = A B on host CSR arrays (columns sorted); small operands (dense accumulator per row)
 HostCSR host_spgemm(const HostCSR& A, const HostCSR& B);
 

@@ -305,14 +305,13 @@ int deco_amg_coarse_inverse(const deco_prec* M, int* nc, const double** inv_dev)
 /* Rows of the AMG's dense tail operator (0 when the hierarchy has none): the last level's
  * sub-cycle formed as one dense matrix at setup. No reference counterpart. */
 int deco_amg_dense_tail(const deco_prec* M, int* rows);
-/* AMG hierarchies set up from now on form the sub-cycle below their last level with
- * interpolation (zero-guess Jacobi-type sweep, residual, restriction, coarsest solve, up-sweep)
- * as one dense operator Z = 2D^-1 - D^-1 A D^-1 + (P - D^-1 A P) C P^T (I - A D^-1), applied by
- * one product per cycle, when that level has at most max_rows rows (default 2048, at most 4096;
- * 0 keeps the sparse sub-cycle; the F-C-F smoother 17 always does). Forming Z costs
- * O(n_c^2 n) host flops and 24 n^2 bytes of device memory (FP64 and FP32, forward and
- * transposed). Equal to the sparse sub-cycle up to rounding. No reference counterpart (hypre's
- * cycle runs level by level). */
+/* AMG hierarchies set up from now on form the whole sub-cycle of their highest level of at most
+ * max_rows rows (its smoothing, residual, restriction, the levels below it with the coarsest solve,
+ * prolongation and smoothing) as one dense operator, applied by one product per cycle (default
+ * 4608 rows, at most 8192; 0 keeps the level-by-level cycle; the F-C-F smoother 17 always does).
+ * Forming it costs O(n^2 n_c) device flops and 24 n^2 bytes of device memory (FP64 and FP32,
+ * forward and transposed). Equal to the level-by-level sub-cycle up to rounding. No reference
+ * counterpart (hypre's cycle runs level by level). */
 int deco_set_amg_dense_tail(int max_rows);
 /* Host-only: one level of the AMG setup (strength, C/F splitting, extended+i interpolation and
  * truncation) on host CSR arrays, as deco_amg_create runs it per level. cf[i] = 1 (C) / -1 (F);

@@ -126,7 +126,11 @@ def test_cpr_sequences(gpu, seq, relax):
     g = gen_system(4)
     tree = small_params(2, seq=str(seq), **{"prec_loc.amg_relax_type": relax})
     A = dev_csr(g)
-    M = deco_amd.CPR(tree, A, None)
+    deco_amd.set_amg_dense_tail(0)            # the level-by-level cycle (2048 pressure rows)
+    try:
+        M = deco_amd.CPR(tree, A, None)
+    finally:
+        deco_amd.set_amg_dense_tail(4608)
     Mo_A = to_mat(g)
     Mo = oracle.CPR(deco_amd.cpr_params(tree), Mo_A, None)
     Go, _, _ = oracle_amg(M.amg(), Mo.pressure_matrix(), tree)
@@ -344,11 +348,12 @@ def test_amg_wide_chunk_span_falls_back(gpu):
     assert rel(G.apply(x).numpy(), Go.apply(x)) <= 1e-12
 
 
-@pytest.mark.parametrize("relax", [18, 7, -1, 12, 14])
-def test_amg_dense_tail_matches_sparse_subcycle(gpu, relax):
-    """The last level's sub-cycle formed as one dense operator at setup, against the same
-    hierarchy with the sparse sub-cycle (dense tail disabled) and the oracle's level-by-level
-    V-cycle, forward and transposed, in both precisions."""
+@pytest.mark.parametrize("relax,cover", [(18, 1), (7, 2), (-1, 1), (-1, 2), (12, 2), (14, 1)])
+def test_amg_dense_tail_matches_sparse_subcycle(gpu, relax, cover):
+    """The sub-cycle of the last `cover` levels with interpolation, formed as one dense operator at
+    setup (on the device, level by level from the coarsest inverse), against the same hierarchy
+    with the level-by-level cycle (dense tail disabled) and the oracle's level-by-level V-cycle,
+    forward and transposed, in both precisions."""
     g = gen_system(1)
     A = dev_csr(g)
     Ap = A.local_matrix(2, 0)
@@ -358,11 +363,16 @@ def test_amg_dense_tail_matches_sparse_subcycle(gpu, relax):
     try:
         G0 = deco_amd.AMG(tree, Ap)
     finally:
-        deco_amd.set_amg_dense_tail(2048)
-    G1 = deco_amd.AMG(tree, Ap)
-    Go, levels, _ = oracle_amg(G1, oracle.local_matrix(to_mat(g), 2, 0), tree)
-    assert len(levels) >= 2
-    assert G0.dense_tail() == 0 and G1.dense_tail() == levels[-1]["n"] > 0
+        deco_amd.set_amg_dense_tail(4608)
+    Go, levels, _ = oracle_amg(G0, oracle.local_matrix(to_mat(g), 2, 0), tree)
+    assert len(levels) >= 3
+    rows = levels[-cover]["n"]
+    deco_amd.set_amg_dense_tail(rows)
+    try:
+        G1 = deco_amd.AMG(tree, Ap)
+    finally:
+        deco_amd.set_amg_dense_tail(4608)
+    assert G0.dense_tail() == 0 and G1.dense_tail() == rows > 0
     x = np.random.default_rng(7).uniform(-1, 1, Ap.n)
     y0, y1 = G0.apply(x).numpy(), G1.apply(x).numpy()
     assert rel(y1, y0) <= 1e-13 and not np.array_equal(y1, y0)   # the dense product did run

@@ -82,9 +82,9 @@ def test_global_layout_switches_validate_without_a_device():
     restored), out-of-range ones rejected with DECO_ERR_ARG, with no device present."""
     import deco_amd
     deco_amd.set_amg_dense_tail(0)
-    deco_amd.set_amg_dense_tail(4096)
-    deco_amd.set_amg_dense_tail(2048)
-    for bad in (-1, 4097):
+    deco_amd.set_amg_dense_tail(8192)
+    deco_amd.set_amg_dense_tail(4608)
+    for bad in (-1, 8193):
         with pytest.raises(deco_amd._lib.DecoError) as e:
             deco_amd.set_amg_dense_tail(bad)
         assert deco_amd._lib.ERRORS[e.value.code] == "DECO_ERR_ARG"
