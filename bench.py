@@ -121,6 +121,28 @@ def cpu_solve_baseline(g, M, prec_tree, sol, gpu_iters, args, log):
                       f"per-iteration {dt / max(info.iter, 1) * 1e3:.1f} ms; one run on 4 threads {dt4:.2f}s"}
 
 
+def solve_only(deco_amd, g, A, key, log):
+    """CPR setup with the bench's parameters, then in each precision one warm-up and one timed
+    flexible CPR-GMRES(50) solve to rtol 1e-8, and five 512 MiB copies (the reference reading of the
+    FETCH_SIZE / WRITE_SIZE counters): the whole-solve workload of the counter passes."""
+    n = g["n"]
+    b = deco_amd.DeviceArray.from_numpy(g["b"])
+    M = deco_amd.CPR(deco_amd.PTree(dict(PREC_TREE, block_size=str(g["bs"]))), A, b)
+    deco_amd.lib().deco_sync()
+    log("CPR setup done")
+    sol = {"type": "rgmres", "reltol": "1e-8", "restart": "50", "itmax": "500", "verbosity": "0", "flexible": "1"}
+    out = {"workload": key, "n": n, "nnz": g["nnz"]}
+    for prec, name in ((deco_amd.PREC_FP64, "fp64"), (deco_amd.PREC_FP32, "fp32_precond")):
+        M.set_precision(prec)
+        deco_amd.SOL(sol, A, M, b)
+        S = deco_amd.SOL(sol, A, M, b)
+        out[name] = {"iter": S.iter, "ms": S.info.time_ms, "n_prec": S.info.n_prec}
+        log(f"{name} solve: {out[name]}")
+    out["copy_bytes"] = copy_calibration(deco_amd)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def adjoint_small(deco_amd, args, log):
     """Forward CPR-GMRES(30) solve of b = A x* and the adjoint of L = w.x (w ~ N(0,1)): the
     transposed solve, lambda = W^T mu and dL/dA on A's pattern, to rtol 1e-8 on the 32x32x16
@@ -268,6 +290,9 @@ def main():
     ap.add_argument("--relax", type=int, default=None,
                     help="amg_relax_type of the headline solves (default: the reference parameter set's -1)")
     ap.add_argument("--relax-weight", type=float, default=0.0, help="amg_relax_weight (relax 7)")
+    ap.add_argument("--solve-only", action="store_true",
+                    help="CPR setup, one warm-up and one timed solve per precision and a 512 MiB copy (for "
+                         "counter passes over whole solves)")
     ap.add_argument("--no-smoother-compare", action="store_true",
                     help="skip the second CPR setup with weighted Jacobi (relax 7, weight 2/3)")
     args = ap.parse_args()
@@ -293,6 +318,8 @@ def main():
     log(f"CSR upload {upload_s * 1e3:.1f} ms ({(12 * nnz + 4 * (n + 1)) / upload_s / 1e9:.1f} GB/s)")
 
     # ---------------------------------------------------------------- SpMV roofline
+    if args.solve_only:
+        return solve_only(deco_amd, g, A, key, log)
     spmv = spmv_section(deco_amd, A, n, nnz, args.spmv_iters, g["bs"])
     log(f"spmv fp64 {spmv['fp64_us']:.1f} us {spmv['fp64_GBps']:.0f} GB/s; "
         f"fp32 {spmv['fp32_us']:.1f} us {spmv['fp32_GBps']:.0f} GB/s")

@@ -284,9 +284,11 @@ __global__ void k_axpby2(int n, double a, const double* __restrict__ x, double b
 // Work vectors of the Krylov solvers, kept between solves and grown on demand, so that a solve
 // does not allocate and free its basis (1.6 GB at 4M unknowns and restart 50, half as much again
 // for the FP32 flexible directions) every time. One solve at a time uses them (the library runs on one
-// stream); deco_workspace_release() frees them.
+// stream); deco_workspace_release() frees them. The array lives until the process ends and is
+// never destroyed: a destructor at exit would free device memory after the HIP runtime (and any
+// profiling tool attached to it) has shut down; the driver reclaims the memory with the process.
 constexpr int kWsSlots = 12;
-static DBuf<double> g_ws[kWsSlots];
+static DBuf<double>* const g_ws = new DBuf<double>[kWsSlots];
 struct WsRef {
   double* p;
 };
@@ -884,7 +886,7 @@ static void pcg(const deco_sol_params& sp, const AOp& A, const MOp& M, const dou
 }
 
 static void release_workspace() {
-  for (auto& b : g_ws) b.release();
+  for (int k = 0; k < kWsSlots; ++k) g_ws[k].release();
 }
 
 static void run_solver(const deco_sol_params& sp, const AOp& A, const MOp& M, const double* b, double* x,
