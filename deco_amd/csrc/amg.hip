@@ -289,20 +289,40 @@ static deco_csr* up_operator(const deco_csr* P, const deco_csr* Q, const double*
   return S;
 }
 
-// x = C b with C row-major n x n in MT (the coarsest inverse for the forward cycle, its
-// transpose for the transposed one; or the last level's dense sub-cycle operator): one wavefront
-// per row, lanes stride the columns, fixed-order butterfly sum; input/output in T, FP64
-// arithmetic; with add_to set, add_to[i*stride] += x_i instead of storing x
-template <class T, class MT>
-__global__ __launch_bounds__(256) void k_dense(int n, const MT* __restrict__ C, const T* __restrict__ b,
+typedef float dense_f32x4 __attribute__((ext_vector_type(4)));
+typedef double dense_f64x2 __attribute__((ext_vector_type(2)));
+
+// x = C b with C row-major n x n in MT, row stride ld (the coarsest inverse for the forward cycle,
+// its transpose for the transposed one; or a dense sub-cycle operator): one wavefront per row,
+// lanes stride the columns -- with VEC, 16-byte loads of W = 16/sizeof(MT) entries, which needs ld
+// a multiple of W and C 16-byte aligned (the padding columns [n, ld) are read and multiply
+// nothing) -- fixed-order butterfly sum; input/output in T, FP64 arithmetic; with add_to set,
+// add_to[i*stride] += x_i instead of storing x
+template <class T, class MT, bool VEC>
+__global__ __launch_bounds__(256) void k_dense(int n, int ld, const MT* __restrict__ C, const T* __restrict__ b,
                                                T* __restrict__ x, T* __restrict__ add_to, int stride) {
   const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (row >= n) return;
-  const MT* cr = C + (size_t)row * n;
+  const MT* cr = C + (size_t)row * ld;
   double s = 0.0;
+  if constexpr (!VEC) {
 #pragma unroll 4
-  for (int k = lane; k < n; k += 64) s += (double)cr[k] * (double)b[k];
+    for (int k = lane; k < n; k += 64) s += (double)cr[k] * (double)b[k];
+  } else {
+    using VT = typename std::conditional<std::is_same<MT, float>::value, dense_f32x4, dense_f64x2>::type;
+    constexpr int W = 16 / sizeof(MT);
+    const VT* cv = reinterpret_cast<const VT*>(cr);
+    const int nv = (n + W - 1) / W;
+#pragma unroll 4
+    for (int q = lane; q < nv; q += 64) {
+      const VT c = cv[q];
+      const int k = q * W;
+#pragma unroll
+      for (int j = 0; j < W; ++j)
+        if (k + j < n) s += (double)c[j] * (double)b[k + j];
+    }
+  }
   s = wave_sum(s);
   if (lane == 0) {
     const T y = (T)s;
@@ -365,6 +385,15 @@ __global__ void k_add_csr_dense(int n, const int* __restrict__ r, const int* __r
   for (int e = r[i]; e < r[i + 1]; ++e) Z[(size_t)i * n + c[e]] += v[e];
 }
 
+// dst (n x ld, row-major) = src (n x n) with zero columns [n, ld)
+template <class T>
+__global__ void k_pad_rows(int n, int ld, const T* __restrict__ src, T* __restrict__ dst) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)n * ld) return;
+  const int i = (int)(e / ld), j = (int)(e % ld);
+  dst[e] = j < n ? src[(size_t)i * n + j] : (T)0;
+}
+
 // Zt = Z^T for an n x n row-major matrix, through 32 x 32 LDS tiles
 __global__ __launch_bounds__(256) void k_transpose_dense(int n, const double* __restrict__ Z, double* __restrict__ Zt) {
   __shared__ double t[32][33];
@@ -423,8 +452,9 @@ struct AMG : deco_prec {
   // smoothers, last level of at most g_dense_tail_rows rows); tail = its rows, 0 when absent
   int tail = 0;
   int tail_lev = -1;          // the level whose sub-cycle the dense operator replaces (-1: none)
-  DBuf<double> Z64, Z64t;
+  DBuf<double> Z64, Z64t;     // n x tail_ld, row-major, zero padding columns
   DBuf<float> Z32, Z32t;
+  int tail_ld = 0;
   int relax = 18;              // 18 l1-Jacobi, 0 Jacobi (weighted), 17 F-C-F, 11 two-stage Gauss-Seidel
   double weight = 1.0;
   int dir_down = 1, dir_up = -1, inner = 1;   // two-stage: +1 forward, -1 backward; inner steps
@@ -697,32 +727,42 @@ struct AMG : deco_prec {
       C = Cprev.p;
       nC = nl;
     }
+    // stored with rows padded to a multiple of 4 entries, so the product reads 16-byte vectors
     const int nl = nC;
-    Z64 = std::move(Cprev);
-    Z64t.alloc((size_t)nl * nl);
+    const int ld = (nl + 3) / 4 * 4;
+    DBuf<double> Zt((size_t)nl * nl);
     hipLaunchKernelGGL(k_transpose_dense, dim3((nl + 31) / 32, (nl + 31) / 32), dim3(256), 0, stream(), nl,
-                       (const double*)Z64.p, Z64t.p);
+                       (const double*)Cprev.p, Zt.p);
     DECO_CHECK_LAUNCH();
-    Z32.alloc((size_t)nl * nl);
-    Z32t.alloc((size_t)nl * nl);
-    convert((const double*)Z64.p, Z32.p, (int64_t)nl * nl);
-    convert((const double*)Z64t.p, Z32t.p, (int64_t)nl * nl);
+    Z64.alloc((size_t)nl * ld);
+    Z64t.alloc((size_t)nl * ld);
+    const unsigned gp = grid_for((int64_t)nl * ld, 256);
+    hipLaunchKernelGGL(k_pad_rows<double>, dim3(gp), dim3(256), 0, stream(), nl, ld, (const double*)Cprev.p, Z64.p);
+    DECO_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_pad_rows<double>, dim3(gp), dim3(256), 0, stream(), nl, ld, (const double*)Zt.p, Z64t.p);
+    DECO_CHECK_LAUNCH();
+    Z32.alloc((size_t)nl * ld);
+    Z32t.alloc((size_t)nl * ld);
+    convert((const double*)Z64.p, Z32.p, (int64_t)nl * ld);
+    convert((const double*)Z64t.p, Z32t.p, (int64_t)nl * ld);
     sync();
+    tail_ld = ld;
     tail = nl;
     tail_lev = k;
   }
 
-  template <class MT>
-  void dense(int m, const MT* C, const float* b, float* x, float* add_to, int stride) {
-    hipLaunchKernelGGL((k_dense<float, MT>), dim3((m + 3) / 4), dim3(256), 0, stream(), m, C, b, x, add_to, stride);
+  // x = C b (ld = m, scalar loads: the coarsest inverse) or, with vec, 16-byte loads over rows
+  // of stride ld (a multiple of 4: the dense tail operators)
+  template <class T, class MT>
+  void dense(int m, int ld, bool vec, const MT* C, const T* b, T* x, T* add_to, int stride) {
+    if (vec)
+      hipLaunchKernelGGL((k_dense<T, MT, true>), dim3((m + 3) / 4), dim3(256), 0, stream(), m, ld, C, b, x, add_to,
+                         stride);
+    else
+      hipLaunchKernelGGL((k_dense<T, MT, false>), dim3((m + 3) / 4), dim3(256), 0, stream(), m, ld, C, b, x, add_to,
+                         stride);
     DECO_CHECK_LAUNCH();
-    count_bytes((double)m * m * sizeof(MT) + (double)m * sizeof(float) * (add_to ? 3 : 2));
-  }
-  template <class MT>
-  void dense(int m, const MT* C, const double* b, double* x, double* add_to, int stride) {
-    hipLaunchKernelGGL((k_dense<double, MT>), dim3((m + 3) / 4), dim3(256), 0, stream(), m, C, b, x, add_to, stride);
-    DECO_CHECK_LAUNCH();
-    count_bytes((double)m * m * sizeof(MT) + (double)m * sizeof(double) * (add_to ? 3 : 2));
+    count_bytes((double)m * m * sizeof(MT) + (double)m * sizeof(T) * (add_to ? 3 : 2));
   }
   // the dense tail operator in the cycle's precision
   const double* tail_op(bool tr, const double*) const { return tr ? Z64t.p : Z64.p; }
@@ -765,11 +805,11 @@ struct AMG : deco_prec {
   void cycle(size_t l, const T* b, T* x, bool tr, bool x_ready = false, T* add_to = nullptr, int stride = 0) {
     if (l == L.size()) {
       if (nc == 0) return;
-      dense(nc, (const double*)(tr ? CT.p : Cinv.p), b, x, (T*)nullptr, 0);
+      dense(nc, nc, false, (const double*)(tr ? CT.p : Cinv.p), b, x, (T*)nullptr, 0);
       return;
     }
     if (tail && (int)l == tail_lev) {   // the whole sub-cycle below as one product
-      dense(tail, tail_op(tr, b), b, x, add_to, stride);
+      dense(tail, tail_ld, true, tail_op(tr, b), b, x, add_to, stride);
       return;
     }
     if (relax == 11) {
@@ -896,7 +936,7 @@ struct AMG : deco_prec {
   void run(const T* r, T* x, bool tr) {
     if (tr) ensure_t();
     if (L.empty()) {
-      dense(nc, (const double*)(tr ? CT.p : Cinv.p), r, x, (T*)nullptr, 0);
+      dense(nc, nc, false, (const double*)(tr ? CT.p : Cinv.p), r, x, (T*)nullptr, 0);
       return;
     }
     cycle<T>(0, r, x, tr);

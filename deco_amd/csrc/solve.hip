@@ -8,6 +8,7 @@
 #include "prec.hpp"
 #include "blas.hpp"
 #include <cmath>
+#include <cstring>
 #include <vector>
 #include <chrono>
 
@@ -134,14 +135,22 @@ __global__ __launch_bounds__(kArnBlock) void k_arn_dots(int n, int j, const doub
     part[(size_t)v * nb + blockIdx.x] = ((red[v * 4] + red[v * 4 + 1]) + red[v * 4 + 2]) + red[v * 4 + 3];
 }
 
+// The tag that validates a reduced value published to the host: the value's bits XOR a bijective
+// mix of the reduction's sequence number. The host accepts value v of reduction seq only when
+// tag[v] ^ bits[v] == tag_mix(seq), so neither a value that reaches host memory after its tag
+// nor a tag left from an earlier reduction passes (both have been observed: a value store to
+// coherent host memory can become visible after a later, fenced tag store).
+__host__ __device__ inline unsigned long long tag_mix(long long seq) {
+  return (unsigned long long)seq * 0x9E3779B97F4A7C15ull;
+}
+
 // out[v] = sum over the nb block partials of value v: one workgroup per value, every lane a
-// strided subset (loads issued together), then a fixed-order workgroup sum. The value also goes
-// to host (coherent pinned host memory), followed by a system-scope fence and tag[v] = seq, so
-// the host reads it as soon as the tag shows seq (no event in the stream between this kernel
-// and the update sweep behind it).
+// strided subset (loads issued together), then a fixed-order workgroup sum. The value's bits also
+// go to host (coherent pinned host memory) with their tag, so the host reads them as soon as they
+// validate (no event in the stream between this kernel and the update sweep behind it).
 __global__ __launch_bounds__(256) void k_arn_reduce(int nb, const double* __restrict__ part, double* __restrict__ out,
-                                                   double* __restrict__ host, long long* __restrict__ tag,
-                                                   long long seq) {
+                                                   unsigned long long* __restrict__ host,
+                                                   unsigned long long* __restrict__ tag, long long seq) {
   __shared__ double sm[4];
   const double* p = part + (size_t)blockIdx.x * nb;
   double s = 0.0;
@@ -150,20 +159,29 @@ __global__ __launch_bounds__(256) void k_arn_reduce(int nb, const double* __rest
   s = block_sum<256>(s, sm);
   if (threadIdx.x == 0) {
     out[blockIdx.x] = s;
-    host[blockIdx.x] = s;
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(s);
+    host[blockIdx.x] = bits;
     __threadfence_system();
-    tag[blockIdx.x] = seq;
+    tag[blockIdx.x] = bits ^ tag_mix(seq);
   }
 }
 
 // Waits until the reduction with sequence number seq has published values [0, nv) to the pinned
-// host buffer. While spinning it polls the stream now and then: a stream error is raised, and a
-// drained stream without the tags is an error rather than an endless wait.
-static void wait_tags(int nv, long long seq) {
-  const long long* t = ctx().tags;
+// host buffer and copies each validated value into vals (host memory the solver then reads).
+// While spinning it polls the stream now and then: a stream error is raised, and a drained
+// stream without the values is an error rather than an endless wait.
+static void wait_tags(int nv, long long seq, double* vals) {
+  const unsigned long long* t = reinterpret_cast<const unsigned long long*>(ctx().tags);
+  const unsigned long long* h = reinterpret_cast<const unsigned long long*>(ctx().pinned);
+  const unsigned long long mix = tag_mix(seq);
+  int done = 0;   // values [0, done) validated
   auto arrived = [&] {
-    for (int i = 0; i < nv; ++i)
-      if (__atomic_load_n(&t[i], __ATOMIC_ACQUIRE) != seq) return false;
+    for (; done < nv; ++done) {
+      const unsigned long long tv = __atomic_load_n(&t[done], __ATOMIC_ACQUIRE);
+      const unsigned long long bv = __atomic_load_n(&h[done], __ATOMIC_ACQUIRE);
+      if ((tv ^ bv) != mix) return false;
+      std::memcpy(&vals[done], &bv, sizeof(double));
+    }
     return true;
   };
   for (unsigned spin = 1;; ++spin) {
@@ -561,7 +579,8 @@ static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, cons
   double* Z64 = flex64 ? ws(11, (size_t)n * m).p : nullptr;
   const size_t ld = (size_t)m + 1;
   std::vector<double> Hr(ld * m), Rm(ld * m), T(flex ? ld * m : 0), g(m + 1), cs(m), sn(m), hs(m + 1);
-  double* red = ctx().pinned;
+  std::vector<double> red_h(2 * (size_t)m + 2);   // the reduced values of the current step
+  double* red = red_h.data();
   int flag = 1, iter = 0, napp = 0, nspmv = 0;
   double bnorm, absTol, rnorm = 0.0;
   if (!left) {
@@ -615,7 +634,8 @@ static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, cons
       red_seq = ++ctx().tag_seq;
       red_nv = nv;
       hipLaunchKernelGGL(k_arn_reduce, dim3(nv), dim3(256), 0, stream(), nbk, (const double*)part.p, redd.p,
-                         ctx().pinned_dev, ctx().tags_dev, red_seq);
+                         reinterpret_cast<unsigned long long*>(ctx().pinned_dev),
+                         reinterpret_cast<unsigned long long*>(ctx().tags_dev), red_seq);
       DECO_CHECK_LAUNCH();
       count_bytes(8.0 * nv * (nbk + 2));
       if (wp) {   // queued before the host decision; unused if the cycle ends here
@@ -667,7 +687,7 @@ static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, cons
       bool finalized = false;
       if (!last && !have_w && j > 0 && rn1 > 0.0 && rn2 > 0.0 && rn1 * (rn1 / rn2) < absTol) {
         reduce_step(u, nullptr, un);
-        wait_tags(red_nv, red_seq);
+        wait_tags(red_nv, red_seq, red);
         rho = rho_of(false, gam);
         if (finalize(rho, false)) break;
         finalized = true;
@@ -686,7 +706,7 @@ static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, cons
       const bool likely_stop = rn1 > 0.0 && rn2 > 0.0 && rn1 * (rn1 / rn2) < absTol;
       have_w = wp && !next_last && !likely_stop;
       if (have_w) apply_op(j + 1, un);
-      wait_tags(red_nv, red_seq);
+      wait_tags(red_nv, red_seq, red);
       rho = rho_of(wp != nullptr, gam);
       if (j == 0) {
         g[0] = rho;

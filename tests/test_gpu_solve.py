@@ -230,3 +230,23 @@ def test_bicgstab_fp32_preconditioner(gpu, cfg):
     assert S64.flag == 0 and S32.flag == 0
     assert S32.res_rel < 1e-8 and abs(S32.iter - S64.iter) <= 2
     assert rel(S32.x.numpy(), S64.x.numpy()) <= 1e-6
+
+
+def test_repeated_long_solves_are_bitwise_identical(gpu):
+    """The Gram-Schmidt reductions reach the host through pinned memory; each value is accepted only
+    with its sequence-tagged check word. Block-Jacobi GMRES(60) needs about 425 iterations over seven
+    restarts (some 850 host reads), so a stale read would change its trajectory; 12 solves in each
+    precision must agree bit for bit."""
+    g = gen_system(1)
+    A = dev_csr(g)
+    b = deco_amd.DeviceArray.from_numpy(g["b"])
+    M = deco_amd.Jacobi({"block_size": "2"}, A)
+    sol = {"type": "rgmres", "reltol": "1e-9", "restart": "60", "itmax": "2000", "verbosity": "0"}
+    for prec in (deco_amd.PREC_FP64, deco_amd.PREC_FP32):
+        M.set_precision(prec)
+        first = deco_amd.SOL(sol, A, M, b)
+        x0 = first.x.numpy()
+        for _ in range(11):
+            S = deco_amd.SOL(sol, A, M, b)
+            assert S.iter == first.iter and np.array_equal(S.x.numpy(), x0)
+    M.set_precision(deco_amd.PREC_FP64)
