@@ -289,40 +289,68 @@ static deco_csr* up_operator(const deco_csr* P, const deco_csr* Q, const double*
   return S;
 }
 
-typedef float dense_f32x4 __attribute__((ext_vector_type(4)));
-typedef double dense_f64x2 __attribute__((ext_vector_type(2)));
+// K = [R, -R A] with interleaved columns (column 2j: R on b_j, column 2j+1: -(R A) on t_j), so
+// that b_c = R (b - A t) = K [b; t] is one product: the residual and the restriction of a
+// two-stage level's down-sweep in one launch. Rows merge the two sorted patterns.
+__global__ void k_rcat_count(int n, const int* __restrict__ rr, const int* __restrict__ qr, int* __restrict__ cnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  cnt[i] = (rr[i + 1] - rr[i]) + (qr[i + 1] - qr[i]);
+}
+__global__ void k_rcat_fill(int n, const int* __restrict__ rr, const int* __restrict__ rc, const double* __restrict__ rv,
+                            const int* __restrict__ qr, const int* __restrict__ qc, const double* __restrict__ qv,
+                            const int* __restrict__ kr, int* __restrict__ kc, double* __restrict__ kv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int a = rr[i], b = qr[i], o = kr[i];
+  const int ae = rr[i + 1], be = qr[i + 1];
+  while (a < ae || b < be) {   // column j of R (2j) precedes column j of R A (2j + 1)
+    if (b >= be || (a < ae && rc[a] <= qc[b])) { kc[o] = 2 * rc[a]; kv[o] = rv[a]; ++a; }
+    else { kc[o] = 2 * qc[b] + 1; kv[o] = -qv[b]; ++b; }
+    ++o;
+  }
+}
 
-// x = C b with C row-major n x n in MT, row stride ld (the coarsest inverse for the forward cycle,
-// its transpose for the transposed one; or a dense sub-cycle operator): one wavefront per row,
-// lanes stride the columns -- with VEC, 16-byte loads of W = 16/sizeof(MT) entries, which needs ld
-// a multiple of W and C 16-byte aligned (the padding columns [n, ld) are read and multiply
-// nothing) -- fixed-order butterfly sum; input/output in T, FP64 arithmetic; with add_to set,
+static deco_csr* restrict_operator(const deco_csr* R, const deco_csr* RA) {
+  deco_csr* K = csr_new(R->n, 2 * R->m);
+  DBuf<int> cnt(R->n + 1);
+  if (R->n) {
+    hipLaunchKernelGGL(k_rcat_count, dim3(grid_for(R->n, 256)), dim3(256), 0, stream(), R->n, R->r.p, RA->r.p, cnt.p);
+    DECO_CHECK_LAUNCH();
+  }
+  build_row_ptr(K, cnt.p, R->n);
+  if (R->n) {
+    hipLaunchKernelGGL(k_rcat_fill, dim3(grid_for(R->n, 256)), dim3(256), 0, stream(), R->n, R->r.p, R->c.p, R->v.p,
+                       RA->r.p, RA->c.p, RA->v.p, (const int*)K->r.p, K->c.p, K->v.p);
+    DECO_CHECK_LAUNCH();
+  }
+  return K;
+}
+
+// x access of K's product: entry 2j is b_j, entry 2j+1 is t_j
+template <class T>
+struct CatGather {
+  using type = T;
+  const T* __restrict__ b;
+  const T* __restrict__ t;
+  __device__ __forceinline__ T operator()(int j) const { return (j & 1) ? t[j >> 1] : b[j >> 1]; }
+};
+
+// x = C b with C row-major n x n in MT (the coarsest inverse for the forward cycle, its transpose
+// for the transposed one; or a dense sub-cycle operator): one wavefront per row, lanes stride the
+// columns (scalar loads: 16-byte vector loads with the b entries they pair with measured slower),
+// fixed-order butterfly sum; input/output in T, FP64 arithmetic; with add_to set,
 // add_to[i*stride] += x_i instead of storing x
-template <class T, class MT, bool VEC>
-__global__ __launch_bounds__(256) void k_dense(int n, int ld, const MT* __restrict__ C, const T* __restrict__ b,
+template <class T, class MT>
+__global__ __launch_bounds__(256) void k_dense(int n, const MT* __restrict__ C, const T* __restrict__ b,
                                                T* __restrict__ x, T* __restrict__ add_to, int stride) {
   const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (row >= n) return;
-  const MT* cr = C + (size_t)row * ld;
+  const MT* cr = C + (size_t)row * n;
   double s = 0.0;
-  if constexpr (!VEC) {
 #pragma unroll 4
-    for (int k = lane; k < n; k += 64) s += (double)cr[k] * (double)b[k];
-  } else {
-    using VT = typename std::conditional<std::is_same<MT, float>::value, dense_f32x4, dense_f64x2>::type;
-    constexpr int W = 16 / sizeof(MT);
-    const VT* cv = reinterpret_cast<const VT*>(cr);
-    const int nv = (n + W - 1) / W;
-#pragma unroll 4
-    for (int q = lane; q < nv; q += 64) {
-      const VT c = cv[q];
-      const int k = q * W;
-#pragma unroll
-      for (int j = 0; j < W; ++j)
-        if (k + j < n) s += (double)c[j] * (double)b[k + j];
-    }
-  }
+  for (int k = lane; k < n; k += 64) s += (double)cr[k] * (double)b[k];
   s = wave_sum(s);
   if (lane == 0) {
     const T y = (T)s;
@@ -385,15 +413,6 @@ __global__ void k_add_csr_dense(int n, const int* __restrict__ r, const int* __r
   for (int e = r[i]; e < r[i + 1]; ++e) Z[(size_t)i * n + c[e]] += v[e];
 }
 
-// dst (n x ld, row-major) = src (n x n) with zero columns [n, ld)
-template <class T>
-__global__ void k_pad_rows(int n, int ld, const T* __restrict__ src, T* __restrict__ dst) {
-  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (size_t)n * ld) return;
-  const int i = (int)(e / ld), j = (int)(e % ld);
-  dst[e] = j < n ? src[(size_t)i * n + j] : (T)0;
-}
-
 // Zt = Z^T for an n x n row-major matrix, through 32 x 32 LDS tiles
 __global__ __launch_bounds__(256) void k_transpose_dense(int n, const double* __restrict__ Z, double* __restrict__ Zt) {
   __shared__ double t[32][33];
@@ -434,6 +453,8 @@ struct AMG : deco_prec {
     deco_csr* Up = nullptr;
     deco_csr* Lot = nullptr;  // the same of A^T
     deco_csr* Upt = nullptr;
+    deco_csr* K = nullptr;    // [R, -R A] interleaved: residual and restriction of the down-sweep
+    deco_csr* Kt = nullptr;   // [R, -R A^T] for the transposed cycle
     DBuf<double> d;
     DBuf<float> df;
     DBuf<int> cf;
@@ -452,9 +473,8 @@ struct AMG : deco_prec {
   // smoothers, last level of at most g_dense_tail_rows rows); tail = its rows, 0 when absent
   int tail = 0;
   int tail_lev = -1;          // the level whose sub-cycle the dense operator replaces (-1: none)
-  DBuf<double> Z64, Z64t;     // n x tail_ld, row-major, zero padding columns
+  DBuf<double> Z64, Z64t;     // n x n, row-major
   DBuf<float> Z32, Z32t;
-  int tail_ld = 0;
   int relax = 18;              // 18 l1-Jacobi, 0 Jacobi (weighted), 17 F-C-F, 11 two-stage Gauss-Seidel
   double weight = 1.0;
   int dir_down = 1, dir_up = -1, inner = 1;   // two-stage: +1 forward, -1 backward; inner steps
@@ -570,6 +590,7 @@ struct AMG : deco_prec {
         lv.Up = strict_part(lv.A, false);
         lv.u.alloc(nl);
         lv.uf.alloc(nl);
+        if (&lv != &L[0]) lv.K = fused_restriction(lv, lv.A);
       }
       // the up-sweep's prolongation and Jacobi-type sweep as one operator
       if (jacobi_type()) lv.S = up_operator(lv.P, lv.AP, lv.d.p);
@@ -727,40 +748,25 @@ struct AMG : deco_prec {
       C = Cprev.p;
       nC = nl;
     }
-    // stored with rows padded to a multiple of 4 entries, so the product reads 16-byte vectors
     const int nl = nC;
-    const int ld = (nl + 3) / 4 * 4;
-    DBuf<double> Zt((size_t)nl * nl);
+    Z64 = std::move(Cprev);
+    Z64t.alloc((size_t)nl * nl);
     hipLaunchKernelGGL(k_transpose_dense, dim3((nl + 31) / 32, (nl + 31) / 32), dim3(256), 0, stream(), nl,
-                       (const double*)Cprev.p, Zt.p);
+                       (const double*)Z64.p, Z64t.p);
     DECO_CHECK_LAUNCH();
-    Z64.alloc((size_t)nl * ld);
-    Z64t.alloc((size_t)nl * ld);
-    const unsigned gp = grid_for((int64_t)nl * ld, 256);
-    hipLaunchKernelGGL(k_pad_rows<double>, dim3(gp), dim3(256), 0, stream(), nl, ld, (const double*)Cprev.p, Z64.p);
-    DECO_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_pad_rows<double>, dim3(gp), dim3(256), 0, stream(), nl, ld, (const double*)Zt.p, Z64t.p);
-    DECO_CHECK_LAUNCH();
-    Z32.alloc((size_t)nl * ld);
-    Z32t.alloc((size_t)nl * ld);
-    convert((const double*)Z64.p, Z32.p, (int64_t)nl * ld);
-    convert((const double*)Z64t.p, Z32t.p, (int64_t)nl * ld);
+    Z32.alloc((size_t)nl * nl);
+    Z32t.alloc((size_t)nl * nl);
+    convert((const double*)Z64.p, Z32.p, (int64_t)nl * nl);
+    convert((const double*)Z64t.p, Z32t.p, (int64_t)nl * nl);
     sync();
-    tail_ld = ld;
     tail = nl;
     tail_lev = k;
   }
 
-  // x = C b (ld = m, scalar loads: the coarsest inverse) or, with vec, 16-byte loads over rows
-  // of stride ld (a multiple of 4: the dense tail operators)
+  // x = C b (or add_to[stride*i] += (C b)_i) with the m x m row-major C
   template <class T, class MT>
-  void dense(int m, int ld, bool vec, const MT* C, const T* b, T* x, T* add_to, int stride) {
-    if (vec)
-      hipLaunchKernelGGL((k_dense<T, MT, true>), dim3((m + 3) / 4), dim3(256), 0, stream(), m, ld, C, b, x, add_to,
-                         stride);
-    else
-      hipLaunchKernelGGL((k_dense<T, MT, false>), dim3((m + 3) / 4), dim3(256), 0, stream(), m, ld, C, b, x, add_to,
-                         stride);
+  void dense(int m, const MT* C, const T* b, T* x, T* add_to, int stride) {
+    hipLaunchKernelGGL((k_dense<T, MT>), dim3((m + 3) / 4), dim3(256), 0, stream(), m, C, b, x, add_to, stride);
     DECO_CHECK_LAUNCH();
     count_bytes((double)m * m * sizeof(MT) + (double)m * sizeof(T) * (add_to ? 3 : 2));
   }
@@ -771,9 +777,9 @@ struct AMG : deco_prec {
   void release() {
     for (auto& lv : L) {
       delete lv.A; delete lv.At; delete lv.P; delete lv.R; delete lv.AP; delete lv.S; delete lv.St;
-      delete lv.Lo; delete lv.Up; delete lv.Lot; delete lv.Upt;
+      delete lv.Lo; delete lv.Up; delete lv.Lot; delete lv.Upt; delete lv.K; delete lv.Kt;
       lv.A = lv.At = lv.P = lv.R = lv.AP = lv.S = lv.St = nullptr;
-      lv.Lo = lv.Up = lv.Lot = lv.Upt = nullptr;
+      lv.Lo = lv.Up = lv.Lot = lv.Upt = lv.K = lv.Kt = nullptr;
     }
     L.clear();
     delete Ac;
@@ -805,11 +811,11 @@ struct AMG : deco_prec {
   void cycle(size_t l, const T* b, T* x, bool tr, bool x_ready = false, T* add_to = nullptr, int stride = 0) {
     if (l == L.size()) {
       if (nc == 0) return;
-      dense(nc, nc, false, (const double*)(tr ? CT.p : Cinv.p), b, x, (T*)nullptr, 0);
+      dense(nc, (const double*)(tr ? CT.p : Cinv.p), b, x, (T*)nullptr, 0);
       return;
     }
     if (tail && (int)l == tail_lev) {   // the whole sub-cycle below as one product
-      dense(tail, tail_ld, true, tail_op(tr, b), b, x, add_to, stride);
+      dense(tail, tail_op(tr, b), b, x, add_to, stride);
       return;
     }
     if (relax == 11) {
@@ -852,22 +858,39 @@ struct AMG : deco_prec {
     }
   }
 
-  // restriction of the residual r of level l, the cycle below, and the coarse solution's
-  // location; the restriction also writes the next level's zero-guess sweep (g_0 = b/d) unless
-  // that level is the coarsest, the dense tail or an F-C-F level
-  template <class T>
-  T* coarse_correction(size_t l, const T* r, bool tr) {
-    Level& lv = L[l];
+  // restriction into level l+1 (restr(epi) forms the restricted residual with epilogue epi),
+  // the cycle below, and the coarse solution's location; the restriction also writes the next
+  // level's zero-guess sweep (g_0 = b/d) unless that level is the coarsest, the dense tail or an
+  // F-C-F level
+  template <class T, class F>
+  T* coarse_correction_with(size_t l, bool tr, F restr) {
     const bool last = (l + 1 == L.size());
     T* bn = last ? BC<T>() : B<T>(L[l + 1]);
     T* xn = last ? XC<T>() : X<T>(L[l + 1]);
     const bool fuse = !last && relax != 17 && !(tail && (int)l + 1 == tail_lev);
-    if (fuse)
-      spmv_auto<T>(lv.R, r, EpiRestrictJacobi0<T>{bn, D<T>(L[l + 1]), xn});
-    else
-      spmv_auto<T>(lv.R, r, EpiStd<EPI_SET, T, T>{bn, nullptr});
+    if (fuse) restr(EpiRestrictJacobi0<T>{bn, D<T>(L[l + 1]), xn});
+    else restr(EpiStd<EPI_SET, T, T>{bn, nullptr});
     cycle<T>(l + 1, bn, xn, tr, fuse);
     return xn;
+  }
+  // the same with the residual r of level l restricted by R
+  template <class T>
+  T* coarse_correction(size_t l, const T* r, bool tr) {
+    return coarse_correction_with<T>(l, tr, [&](auto epi) { spmv_auto<T>(L[l].R, r, epi); });
+  }
+
+  // K = [R, -R A] of level lv (A: its matrix or the transpose) for the fused down-sweep restriction
+  static deco_csr* fused_restriction(const Level& lv, const deco_csr* A) {
+    deco_csr* RA = spgemm(lv.R, A);
+    deco_csr* K = nullptr;
+    try {
+      K = restrict_operator(lv.R, RA);
+    } catch (...) {
+      delete RA;
+      throw;
+    }
+    delete RA;
+    return K;
   }
 
   // strict part of the (transposed) level matrix for a two-stage sweep of direction dir
@@ -876,7 +899,8 @@ struct AMG : deco_prec {
   }
 
   // Level l with two-stage Gauss-Seidel sweeps. Down from a zero guess (rho = b): g_0 = b/d is in
-  // x (x_ready, or written here), the inner steps leave the sweep's result in t. Up: x = t + P e,
+  // x (x_ready, or written here), the inner steps leave the sweep's result in t; below level 0 the
+  // residual b - A t and its restriction are one product, K [b; t] with K = [R, -R A]. Up: x = t + P e,
   // rho = b - A x and g_0 = rho/d (into r and u), the inner steps add g_s to x (or, with add_to
   // set, x + g_s to add_to[stride*i]). The transposed cycle runs on A^T with the other side's
   // directions reversed.
@@ -902,8 +926,13 @@ struct AMG : deco_prec {
       spmv_auto<T>(Td, (const T*)x, EpiTwoStage<T>{nullptr, b, d, u, nullptr, 0});
       spmv_auto<T>(Td, (const T*)u, EpiTwoStage<T>{nullptr, b, d, t, nullptr, 0});
     }
-    spmv_auto<T>(A, (const T*)t, EpiStd<EPI_RES, T, T>{r, b});
-    const T* xn = coarse_correction<T>(l, r, tr);
+    const T* xn;
+    if (deco_csr* K = tr ? lv.Kt : lv.K) {   // b_c = R (b - A t) as K [b; t]
+      xn = coarse_correction_with<T>(l, tr, [&](auto epi) { spmv_stream_c16(K, vals<T>(K), CatGather<T>{b, t}, epi); });
+    } else {
+      spmv_auto<T>(A, (const T*)t, EpiStd<EPI_RES, T, T>{r, b});
+      xn = coarse_correction<T>(l, r, tr);
+    }
     spmv_auto<T>(lv.P, xn, EpiStd<EPI_ADDZ, T, T>{x, t});
     spmv_auto<T>(A, (const T*)x, EpiResG<T>{b, d, r, u});
     const deco_csr* Tu = strict(lv, tr, up);
@@ -922,6 +951,7 @@ struct AMG : deco_prec {
       if (relax == 11) {
         lv.Lot = strict_part(lv.At, true);
         lv.Upt = strict_part(lv.At, false);
+        if (&lv != &L[0]) lv.Kt = fused_restriction(lv, lv.At);
       }
       if (jacobi_type()) {
         deco_csr* Q = spgemm(lv.At, lv.P);
@@ -936,7 +966,7 @@ struct AMG : deco_prec {
   void run(const T* r, T* x, bool tr) {
     if (tr) ensure_t();
     if (L.empty()) {
-      dense(nc, nc, false, (const double*)(tr ? CT.p : Cinv.p), r, x, (T*)nullptr, 0);
+      dense(nc, (const double*)(tr ? CT.p : Cinv.p), r, x, (T*)nullptr, 0);
       return;
     }
     cycle<T>(0, r, x, tr);

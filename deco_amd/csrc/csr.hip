@@ -725,6 +725,14 @@ __global__ void k_sell_fill(int nb, int R, int C, const int* __restrict__ r, con
   }
 }
 
+// *out = max(*out, longest row)
+__global__ __launch_bounds__(256) void k_max_row(int n, const int* __restrict__ ptr, int* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int len = i < n ? ptr[i + 1] - ptr[i] : 0;
+  for (int off = 32; off > 0; off >>= 1) len = max(len, __shfl_xor(len, off));
+  if ((threadIdx.x & 63) == 0 && len > 0) atomicMax(out, len);
+}
+
 // rb[k] = (i, ptr[i]) for the first row i in [0, n] with ptr[i] >= k*cap; rb[nchunks] = (n, ptr[n])
 __global__ void k_chunks_cap(int n, const int* __restrict__ ptr, int nchunks, int cap, int2* __restrict__ rb) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -892,10 +900,23 @@ const deco_bsr* deco_csr::block64(int R, int C) const {
 
 const int2* deco_csr::row_chunks() const {
   if (!rb_valid) {
-    nchunks = (int)std::max<int64_t>(1, ((int64_t)nnz + deco::kStreamCap - 1) / deco::kStreamCap);
+    // chunks of at most kStreamTile entries: the cap drops below kStreamCap when the longest row
+    // would let a chunk overflow the LDS tile (rows up to the tile's size)
+    int cap = deco::kStreamCap;
+    if (n > 0 && (int64_t)nnz > deco::kStreamCap) {
+      deco::DBuf<int> mx(1);
+      mx.zero();
+      hipLaunchKernelGGL(deco::k_max_row, dim3(deco::grid_for(n, 256)), dim3(256), 0, deco::stream(), n, r.p, mx.p);
+      DECO_CHECK_LAUNCH();
+      int longest = 0;
+      deco::copy_d2h(&longest, mx.p, 1);
+      if (longest > deco::kStreamTile - deco::kStreamCap + 1 && longest <= deco::kStreamTile / 2)
+        cap = deco::kStreamTile - longest + 1;
+    }
+    nchunks = (int)std::max<int64_t>(1, ((int64_t)nnz + cap - 1) / cap);
     rb.ensure(nchunks + 1);
     hipLaunchKernelGGL(deco::k_chunks_cap, dim3(deco::grid_for(nchunks + 1, 256)), dim3(256), 0, deco::stream(), n,
-                       r.p, nchunks, deco::kStreamCap, rb.p);
+                       r.p, nchunks, cap, rb.p);
     DECO_CHECK_LAUNCH();
     rb_valid = true;
   }

@@ -152,9 +152,12 @@ void launch_spmv(int G, int nout, int rs, const int* r, const int* c, const VT* 
 // then the products. The tile has no per-entry branches (indices past the range are clamped
 // and their products dropped), so the gathers are in flight together instead of one at a
 // time. Rows shorter than 8 entries on average are summed in column order (the same order as
-// alg::spmv, core/alg.hpp:59-61); longer rows by a lane group and a fixed xor tree. Ranges that
-// exceed the LDS tile (rows over 129 entries) fall back to a block reduction per row.
+// alg::spmv, core/alg.hpp:59-61); longer rows by a lane group and a fixed xor tree. A chunk holds
+// at most cap + longest row - 1 entries, so a matrix whose longest row exceeds kStreamTile -
+// kStreamCap + 1 takes a smaller cap (deco_csr::row_chunks); a range that still exceeds the LDS
+// tile (a row longer than the tile) falls back to a block reduction per row.
 constexpr int kStreamCap = 3072;                 // target entries per chunk
+constexpr int kStreamTile = kStreamCap + 128;    // LDS entries of a chunk's products
 template <class VT>
 constexpr int stream_block() {
   return sizeof(VT) == 8 ? 256 : 512;
@@ -251,7 +254,7 @@ __global__ __launch_bounds__(stream_block<VT>()) void k_spmv_stream(const int2* 
   using XT = typename XG::type;
   using AT = acc_t<VT, XT>;
   constexpr int BLK = stream_block<VT>();
-  constexpr int LDSN = kStreamCap + 128;   // a chunk holds <= cap + longest row - 1
+  constexpr int LDSN = kStreamTile;   // a chunk holds <= cap + longest row - 1
   constexpr int U = kStreamCap / BLK;
   __shared__ AT prod[LDSN];
   const int ch = xcd_block(blockIdx.x, gridDim.x);
