@@ -333,7 +333,10 @@ struct CatGather {
   using type = T;
   const T* __restrict__ b;
   const T* __restrict__ t;
-  __device__ __forceinline__ T operator()(int j) const { return (j & 1) ? t[j >> 1] : b[j >> 1]; }
+  __device__ __forceinline__ T operator()(int j) const {
+    const T* p = (j & 1) ? t : b;   // one load per entry with a per-lane address
+    return p[j >> 1];
+  }
 };
 
 // x = C b with C row-major n x n in MT (the coarsest inverse for the forward cycle, its transpose

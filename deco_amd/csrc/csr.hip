@@ -901,18 +901,23 @@ const deco_bsr* deco_csr::block64(int R, int C) const {
 const int2* deco_csr::row_chunks() const {
   if (!rb_valid) {
     // chunks of at most kStreamTile entries: the cap drops below kStreamCap when the longest row
-    // would let a chunk overflow the LDS tile (rows up to the tile's size)
-    int cap = deco::kStreamCap;
-    if (n > 0 && (int64_t)nnz > deco::kStreamCap) {
+    // would let a chunk overflow the tile (rows up to half the tile); the window a chunk's range
+    // can span is cap + longest - 1
+    int cap = deco::kStreamCap, win = deco::kStreamTile;
+    if (n > 0 && nnz > 0) {
       deco::DBuf<int> mx(1);
       mx.zero();
       hipLaunchKernelGGL(deco::k_max_row, dim3(deco::grid_for(n, 256)), dim3(256), 0, deco::stream(), n, r.p, mx.p);
       DECO_CHECK_LAUNCH();
       int longest = 0;
       deco::copy_d2h(&longest, mx.p, 1);
-      if (longest > deco::kStreamTile - deco::kStreamCap + 1 && longest <= deco::kStreamTile / 2)
-        cap = deco::kStreamTile - longest + 1;
+      if (longest <= deco::kStreamTile / 2) {
+        if (longest > deco::kStreamTile - deco::kStreamCap + 1) cap = deco::kStreamTile - longest + 1;
+        win = cap + std::max(longest, 1) - 1;
+      }
     }
+    chunk_cap = cap;
+    chunk_win = win;
     nchunks = (int)std::max<int64_t>(1, ((int64_t)nnz + cap - 1) / cap);
     rb.ensure(nchunks + 1);
     hipLaunchKernelGGL(deco::k_chunks_cap, dim3(deco::grid_for(nchunks + 1, 256)), dim3(256), 0, deco::stream(), n,

@@ -40,10 +40,12 @@ struct deco_csr {
   deco::DBuf<float> vf;      // f32 copy of v, valid when vf_valid
   bool vf_valid = false;
   deco_csr* At = nullptr;    // cached transpose (adjoint SpMV)
-  // SpMV row chunks: chunk k holds the rows whose first entry lies in [k*CAP, (k+1)*CAP)
+  // SpMV row chunks: chunk k holds the rows whose first entry lies in [k*cap, (k+1)*cap)
   // as (first row, its first entry) pairs, nchunks + 1 of them
   mutable deco::DBuf<int2> rb;
   mutable int nchunks = 0;
+  mutable int chunk_cap = 0;
+  mutable int chunk_win = 0;   // entries a chunk's range can span: cap + longest row - 1 (at most kStreamTile)
   mutable bool rb_valid = false;
   ~deco_csr();
   const float* values_f32();  // creates / refreshes the shadow
@@ -146,25 +148,27 @@ void launch_spmv(int G, int nout, int rs, const int* r, const int* c, const VT* 
 }
 
 // ------------------------------------------------------------------ CSR-stream SpMV ------
-// Chunk k holds the rows whose first entry lies in [k*kStreamCap, (k+1)*kStreamCap). A block
-// streams the contiguous entry range of its chunk into LDS: every lane issues 12 (FP64, 256
-// lanes) or 6 (FP32, 512 lanes) independent (column, value) loads, then all of its x gathers,
-// then the products. The tile has no per-entry branches (indices past the range are clamped
-// and their products dropped), so the gathers are in flight together instead of one at a
-// time. Rows shorter than 8 entries on average are summed in column order (the same order as
-// alg::spmv, core/alg.hpp:59-61); longer rows by a lane group and a fixed xor tree. A chunk holds
-// at most cap + longest row - 1 entries, so a matrix whose longest row exceeds kStreamTile -
-// kStreamCap + 1 takes a smaller cap (deco_csr::row_chunks); a range that still exceeds the LDS
-// tile (a row longer than the tile) falls back to a block reduction per row.
+// Chunk k holds the rows whose first entry lies in [k*cap, (k+1)*cap), so its entries lie in the
+// window [k*cap, k*cap + win), win = cap + longest row - 1 (deco_csr::row_chunks lowers the cap
+// below kStreamCap when win would exceed kStreamTile). A workgroup loads the whole window -- every
+// lane 13 (FP64, 256 lanes) or 7 (FP32, 512 lanes) independent (column, value) loads, indices
+// past the window clamped to its last entry, then all of its x gathers -- without waiting for the
+// chunk's row range, which loads in parallel and only drives the sums: one dependent memory round
+// fewer than loading the range first. Entries of the window outside the chunk's rows (the
+// previous chunk's last row, the next chunk's first rows) are gathered at a clamped column and
+// their products dropped. Rows shorter than 8 entries on average are summed in column order (the
+// same order as alg::spmv, core/alg.hpp:59-61); longer rows by a lane group and a fixed xor tree.
+// A chunk whose range exceeds the window (a row longer than half the tile) falls back to a block
+// reduction per row.
 constexpr int kStreamCap = 3072;                 // target entries per chunk
-constexpr int kStreamTile = kStreamCap + 128;    // LDS entries of a chunk's products
+constexpr int kStreamTile = kStreamCap + 128;    // window entries a chunk's range must fit
 template <class VT>
 constexpr int stream_block() {
   return sizeof(VT) == 8 ? 256 : 512;
 }
 
 // Epilogue functors. pre(o) loads the operands row o needs besides its sum; the stream kernels
-// issue it together with the tile's column/value loads, so its latency is hidden behind them.
+// issue it before the tile's gathers, so its latency is hidden behind them.
 // operator()(o, acc, pre) writes row o from its sum (accumulator type AT).
 template <int EPI, class YT, class ZT>
 struct EpiStd {
@@ -250,22 +254,35 @@ __global__ __launch_bounds__(stream_block<VT>()) void k_spmv_stream(const int2* 
                                                                    const int* __restrict__ r,
                                                                    const CT* __restrict__ c,
                                                                    const VT* __restrict__ v, XG x, E epi,
-                                                                   const int* __restrict__ cbase = nullptr) {
+                                                                   const int* __restrict__ cbase, int cap, int win,
+                                                                   int nnz, int m) {
   using XT = typename XG::type;
   using AT = acc_t<VT, XT>;
   constexpr int BLK = stream_block<VT>();
-  constexpr int LDSN = kStreamTile;   // a chunk holds <= cap + longest row - 1
-  constexpr int U = kStreamCap / BLK;
-  __shared__ AT prod[LDSN];
+  constexpr int U = (kStreamTile + BLK - 1) / BLK;
+  constexpr int WIN = U * BLK;
+  __shared__ AT prod[WIN];
   const int ch = xcd_block(blockIdx.x, gridDim.x);
-  const int2 d0 = desc[ch], d1 = desc[ch + 1];   // (first row, its first entry) of chunks ch, ch+1
-  const int r0 = d0.x, r1 = d1.x;
-  if (r0 >= r1) return;
+  const int w0 = ch * cap;   // window start
+  const int tid = threadIdx.x;
   int cb = 0;
   if constexpr (!std::is_same<CT, int>::value) cb = cbase[ch];
+  int cc[U];
+  VT vv[U];
+  const int jl = min(w0 + win, nnz) - 1;   // last entry of the window
+  if (nnz > 0) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int j = min(w0 + tid + u * BLK, jl);
+      cc[u] = min(cb + (int)__builtin_nontemporal_load(c + j), m - 1);
+      vv[u] = __builtin_nontemporal_load(v + j);
+    }
+  }
+  const int2 d0 = desc[ch], d1 = desc[ch + 1];   // (first row, its first entry) of chunks ch, ch+1
+  const int r0 = d0.x, r1 = d1.x;
   const int j0 = d0.y, j1 = d1.y;
   const int cnt = j1 - j0;
-  const int tid = threadIdx.x;
+  const bool fits = j1 - w0 <= win;
   // Lanes per row, 2^lg: 1 (the row summed in column order) unless the rows average 8 or more
   // entries, then the widest group (at most a wavefront) that gives every row of the chunk its
   // own group, so a long row's sum is not one lane's chain of LDS reads.
@@ -274,46 +291,32 @@ __global__ __launch_bounds__(stream_block<VT>()) void k_spmv_stream(const int2* 
   if (cnt >= 8 * nr)
     while (lg < 6 && (nr << (lg + 1)) <= BLK) ++lg;
   const int sub = tid & ((1 << lg) - 1);
-  // this lane's first row: bounds and epilogue operands load together with the tile
+  // this lane's first row: bounds and epilogue operands load behind the window's gathers
   const int row = r0 + (tid >> lg);
   int rbeg = 0, rend = 0;
   typename E::Pre pre{};
-  if (row < r1) {
+  if (row < r1 && fits) {
     rbeg = r[row];
     rend = r[row + 1];
     pre = epi.pre(row);
   }
-  if (cnt <= LDSN) {
-    if (cnt > 0) {
-      const int jl = j1 - 1;
-      int cc[U];
-      VT vv[U];
-      XT xx[U];
+  if (nnz > 0) {
+    XT xx[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int j = min(j0 + tid + u * BLK, jl);
-        cc[u] = cb + (int)__builtin_nontemporal_load(c + j);
-        vv[u] = __builtin_nontemporal_load(v + j);
-      }
+    for (int u = 0; u < U; ++u) xx[u] = x(cc[u]);
 #pragma unroll
-      for (int u = 0; u < U; ++u) xx[u] = x(cc[u]);
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int t = tid + u * BLK;
-        if (t < cnt) prod[t] = (AT)vv[u] * (AT)xx[u];
-      }
-      for (int t = tid + kStreamCap; t < cnt; t += BLK)
-        prod[t] = (AT)__builtin_nontemporal_load(v + j0 + t) * (AT)x(cb + (int)__builtin_nontemporal_load(c + j0 + t));
-    }
-    __syncthreads();
+    for (int u = 0; u < U; ++u) prod[tid + u * BLK] = (AT)vv[u] * (AT)xx[u];
+  }
+  __syncthreads();
+  if (fits) {
     if (row < r1) {   // a lane group is active or inactive as a whole
       const int G = 1 << lg;
       AT acc = AT(0);
-      for (int k = rbeg - j0 + sub; k < rend - j0; k += G) acc += prod[k];
+      for (int k = rbeg - w0 + sub; k < rend - w0; k += G) acc += prod[k];
       for (int off = G >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
       if (sub == 0) epi(row, acc, pre);
       for (int rw = row + (BLK >> lg); rw < r1; rw += BLK >> lg) {
-        const int b = r[rw] - j0, e = r[rw + 1] - j0;
+        const int b = r[rw] - w0, e = r[rw + 1] - w0;
         AT a2 = AT(0);
         for (int k = b + sub; k < e; k += G) a2 += prod[k];
         for (int off = G >> 1; off > 0; off >>= 1) a2 += __shfl_xor(a2, off);
@@ -337,7 +340,7 @@ void spmv_stream_g(const deco_csr* A, const VT* v, XG x, E epi) {
   if (A->n <= 0) return;
   const int2* desc = A->row_chunks();
   hipLaunchKernelGGL((k_spmv_stream<VT, XG, E>), dim3(A->nchunks), dim3(stream_block<VT>()), 0, stream(), desc,
-                     A->r.p, A->c.p, v, x, epi, (const int*)nullptr);
+                     A->r.p, A->c.p, v, x, epi, (const int*)nullptr, A->chunk_cap, A->chunk_win, A->nnz, A->m);
   DECO_CHECK_LAUNCH();
   count_bytes((double)A->nnz * (sizeof(VT) + 4) + 4.0 * (A->n + 1) + 8.0 * (A->nchunks + 1) +
               (double)sizeof(typename XG::type) * A->m + (double)A->n * epi.row_bytes());
@@ -353,7 +356,8 @@ void spmv_stream_c16(const deco_csr* A, const VT* v, XG x, E epi) {
   }
   const int2* desc = A->row_chunks();
   hipLaunchKernelGGL((k_spmv_stream<VT, XG, E, unsigned short>), dim3(A->nchunks), dim3(stream_block<VT>()), 0,
-                     stream(), desc, A->r.p, (const unsigned short*)A->c16.p, v, x, epi, (const int*)A->cbase.p);
+                     stream(), desc, A->r.p, (const unsigned short*)A->c16.p, v, x, epi, (const int*)A->cbase.p,
+                     A->chunk_cap, A->chunk_win, A->nnz, A->m);
   DECO_CHECK_LAUNCH();
   count_bytes((double)A->nnz * (sizeof(VT) + 2) + 4.0 * (A->n + 1) + 12.0 * A->nchunks + 8.0 +
               (double)sizeof(typename XG::type) * A->m + (double)A->n * epi.row_bytes());
