@@ -584,7 +584,16 @@ __device__ __forceinline__ int sell_col(const CT* __restrict__ bc, size_t e, int
   else return __builtin_nontemporal_load(bc + e);
 }
 
-template <class VT, int R, int C, class XT, class E, class CT = int, int KU = 4>
+// Each lane sums its block row's entries in column order, KU at a time: a batch's loads are all
+// in flight before its gathers. Scalar FP32 entries take batches of 8 whose last one is clamped
+// to the row's last entry with zero values, so a slice of width w takes ceil(w/8) dependent
+// rounds; FP64 and block entries take batches of 4 and a remainder loop of single entries
+// (measured: the clamped batches cost these more in registers than they gain).
+template <class VT, int R, int C>
+constexpr int sell_batch() {
+  return R * C * sizeof(VT) <= 4 ? 8 : 4;
+}
+template <class VT, int R, int C, class XT, class E, class CT = int, int KU = sell_batch<VT, R, C>()>
 __global__ __launch_bounds__(256) void k_sell_stream(int nb, int nslices, const int* __restrict__ soff,
                                                     const int* __restrict__ swid, const CT* __restrict__ bc,
                                                     const VT* __restrict__ bv, size_t T, const XT* __restrict__ x,
@@ -605,16 +614,21 @@ __global__ __launch_bounds__(256) void k_sell_stream(int nb, int nslices, const 
   AT acc[R];
 #pragma unroll
   for (int rr = 0; rr < R; ++rr) acc[rr] = AT(0);
+  constexpr bool clamped = (KU == 8);
   int k = 0;
-  for (; k + KU <= w; k += KU) {
+  for (; clamped ? k < w : k + KU <= w; k += KU) {
     int cc[KU];
     BsrVals<VT, R * C> vv[KU];
     XT xv[KU][C];
 #pragma unroll
     for (int u = 0; u < KU; ++u) {
-      const size_t e = (size_t)o + (size_t)(k + u) * kSellRows + lane;
+      const size_t e = (size_t)o + (size_t)min(k + u, w - 1) * kSellRows + lane;
       cc[u] = sell_col(bc, e, base);
       vv[u] = sell_load<VT, R, C>(bv, e, T);
+      if (clamped && k + u >= w) {
+#pragma unroll
+        for (int q = 0; q < R * C; ++q) vv[u].a[q] = VT(0);
+      }
     }
 #pragma unroll
     for (int u = 0; u < KU; ++u) bsr_gather<C>(x, cc[u], xv[u]);
@@ -628,7 +642,7 @@ __global__ __launch_bounds__(256) void k_sell_stream(int nb, int nslices, const 
         acc[rr] += p;
       }
   }
-  for (; k < w; ++k) {
+  for (; k < w; ++k) {   // remainder of the unclamped batches
     const size_t e = (size_t)o + (size_t)k * kSellRows + lane;
     const BsrVals<VT, R * C> vv = sell_load<VT, R, C>(bv, e, T);
     XT xv[C];
