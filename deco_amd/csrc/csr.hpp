@@ -585,10 +585,10 @@ __device__ __forceinline__ int sell_col(const CT* __restrict__ bc, size_t e, int
 }
 
 // Each lane sums its block row's entries in column order, KU at a time: a batch's loads are all
-// in flight before its gathers. Scalar FP32 entries take batches of 8 whose last one is clamped
-// to the row's last entry with zero values, so a slice of width w takes ceil(w/8) dependent
-// rounds; FP64 and block entries take batches of 4 and a remainder loop of single entries
-// (measured: the clamped batches cost these more in registers than they gain).
+// in flight before its gathers. FP32 values take batches (8 scalar entries, 4 blocks) whose last
+// one is clamped to the row's last entry with zero values, so a slice of width w takes
+// ceil(w/KU) dependent rounds; FP64 values take batches of 4 and a remainder loop of single
+// entries (measured: clamped batches cost these more in registers than they gain).
 template <class VT, int R, int C>
 constexpr int sell_batch() {
   return R * C * sizeof(VT) <= 4 ? 8 : 4;
@@ -614,7 +614,7 @@ __global__ __launch_bounds__(256) void k_sell_stream(int nb, int nslices, const 
   AT acc[R];
 #pragma unroll
   for (int rr = 0; rr < R; ++rr) acc[rr] = AT(0);
-  constexpr bool clamped = (KU == 8);
+  constexpr bool clamped = (sizeof(VT) == 4);
   int k = 0;
   for (; clamped ? k < w : k + KU <= w; k += KU) {
     int cc[KU];
