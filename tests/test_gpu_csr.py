@@ -280,3 +280,33 @@ def test_invalid_csr_structures_are_rejected(gpu):
     from conftest import small_params
     with pytest.raises(deco_amd._lib.DecoError):
         deco_amd.CPR(small_params(2), deco_amd.CSR.from_scipy(S), None)
+
+
+def test_stream_product_row_lengths(gpu):
+    """The CSR-stream product over rows of every length class: short rows (summed in column
+    order), rows of 130-1600 entries (the chunk cap drops so a chunk fits the LDS window), rows
+    longer than half the window (block reduction per row), and empty rows; a matrix without
+    entries gives zeros."""
+    rng = np.random.default_rng(11)
+    m = 60_000
+    lens = np.concatenate([rng.integers(0, 21, 2500), [0, 0, 130, 257, 394, 800, 1600, 1601, 2500, 4000],
+                           rng.integers(100, 400, 40), rng.integers(0, 8, 700)])
+    rng.shuffle(lens)
+    n = len(lens)
+    r = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    c = np.concatenate([np.sort(rng.choice(m, k, replace=False)) for k in lens]).astype(np.int32)
+    v = rng.uniform(-1, 1, len(c))
+    S = sp.csr_matrix((v, c, r), shape=(n, m))
+    A = deco_amd.CSR(n, m, r, c, v)
+    x = rng.uniform(-1, 1, m)
+    z = rng.uniform(-1, 1, n)
+    y = A.apply(x).numpy()
+    ref = S @ x
+    absAx = abs(S) @ np.abs(x)
+    assert np.all(np.abs(y - ref) <= 2 * max(lens.max(), 1) * 1.2e-16 * absAx)
+    assert np.array_equal(y[lens == 0], np.zeros(np.count_nonzero(lens == 0)))
+    assert rel(A.residual(x, z).numpy(), z - ref) <= 1e-13
+    y32 = A.apply_f32(x.astype(np.float32)).numpy().astype(np.float64)
+    assert np.all(np.abs(y32 - ref) <= 1e-5 * (absAx + 1e-30) + 1e-6)
+    E = deco_amd.CSR(5000, 100, np.zeros(5001, np.int32), np.zeros(0, np.int32), np.zeros(0))
+    assert np.array_equal(E.apply(np.ones(100)).numpy(), np.zeros(5000))
