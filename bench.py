@@ -425,11 +425,17 @@ def main():
         deco_amd.SOL(sp_, Ap, Mp, bp)
         nbytes, nlaunch = deco_amd.counted_bytes()
         deco_amd.count_bytes(False)
+        # one more solve with events around each product with A: the roofline kernel's in-solve time
+        deco_amd.time_outer_products(True)
+        deco_amd.SOL(sp_, Ap, Mp, bp)
+        otot, ocnt = deco_amd.outer_product_times()
+        deco_amd.time_outer_products(False)
         ms = float(np.median(times))
         x = last.x.numpy()
         r = dict(ms=ms, wall_ms=wall * 1e3, iter=last.iter, res_rel=last.res_rel, flag=last.flag,
                  n_prec=last.info.n_prec, alg_bytes=nbytes, launches=nlaunch,
-                 alg_GBps=nbytes / (ms * 1e-3) / 1e9, frac_of_peak=nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS)
+                 alg_GBps=nbytes / (ms * 1e-3) / 1e9, frac_of_peak=nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                 outer_products=ocnt, outer_product_us=(otot / ocnt * 1e3) if ocnt else None)
         if xref is not None:
             r["err_vs_fp64_same_rtol"] = err(x, xref)
         if xtight is not None:
@@ -483,6 +489,19 @@ def main():
                                           "Gram-Schmidt sweeps; vector kernels, copies and fills) / the solve's "
                                           "device time")},
     })
+
+    # the roofline kernel inside the solves (events around each product with A, one solve each):
+    # FP64 directions in the flexible FP64 solve, f32 directions (x read as 4 bytes) in the FP32 one
+    rl = result["roofline"]
+    if r64f["outer_product_us"]:
+        rl["in_solve_us"] = r64f["outer_product_us"]
+        rl["in_solve_GBps"] = rl["bytes_per_launch"] / (r64f["outer_product_us"] * 1e-6) / 1e9
+        rl["in_solve_frac"] = rl["in_solve_GBps"] / HBM_PEAK_GBS
+        rl["back_to_back_us"] = rl["bytes_per_launch"] / (rl["achieved"] * 1e9) * 1e6
+    if r32["outer_product_us"]:
+        b32 = rl["bytes_per_launch"] - 4.0 * n
+        rl["in_solve_fp32_directions"] = {"us": r32["outer_product_us"], "bytes_per_launch": b32,
+                                          "GBps": b32 / (r32["outer_product_us"] * 1e-6) / 1e9}
 
     # the same workload with the explicit weighted-Jacobi smoother (relax 7, weight 2/3) against
     # the reference parameter set's default (-1: two-stage Gauss-Seidel here)

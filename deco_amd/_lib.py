@@ -144,6 +144,8 @@ SIGNATURES = {
     "deco_profile_report": (i, [C.c_char_p, sz, C.POINTER(C.c_size_t)]),
     "deco_count_bytes": (i, [i]),
     "deco_counted_bytes": (i, [C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
+    "deco_time_outer_products": (i, [i]),
+    "deco_outer_product_times": (i, [C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
     "deco_solve": (i, [C.POINTER(SolParams), vp, vp, vp, vp, C.POINTER(SolInfo)]),
     "deco_solve_transpose": (i, [C.POINTER(SolParams), vp, vp, vp, vp, C.POINTER(SolInfo)]),
     "deco_grad_A_pattern": (i, [vp, vp, vp, vp]),

@@ -643,6 +643,18 @@ def counted_bytes():
     return b.value, n.value
 
 
+def time_outer_products(on=True):
+    """Start (or stop) event timing of the solvers' products with the system matrix; both reset."""
+    check(lib().deco_time_outer_products(1 if on else 0))
+
+
+def outer_product_times():
+    """(total ms, count) of the products with the system matrix timed since time_outer_products()."""
+    t, n = C.c_double(), C.c_longlong()
+    check(lib().deco_outer_product_times(C.byref(t), C.byref(n)))
+    return t.value, n.value
+
+
 def profile_report():
     need = C.c_size_t()
     check(lib().deco_profile_report(None, 0, C.byref(need)))

@@ -379,6 +379,28 @@ static void comp_givens(double x, double y, double& c, double& s) {      // rgmr
   else { const double nu = y / x; c = 1.0 / std::sqrt(1.0 + nu * nu); s = nu * c; }
 }
 
+// Event timing of the solvers' products with A (deco_time_outer_products): a pair of events
+// around each product on the solver stream, created on first use and reused.
+struct OuterTiming {
+  bool on = false;
+  std::vector<hipEvent_t> ev;
+  size_t used = 0;   // events recorded since timing was switched on
+};
+static OuterTiming& outer_timing() {
+  static OuterTiming* t = new OuterTiming();   // never destroyed: its events outlive the runtime's teardown
+  return *t;
+}
+static void outer_mark() {
+  OuterTiming& t = outer_timing();
+  if (!t.on) return;
+  if (t.used == t.ev.size()) {
+    hipEvent_t e;
+    DECO_HIP(hipEventCreate(&e));
+    t.ev.push_back(e);
+  }
+  DECO_HIP(hipEventRecord(t.ev[t.used++], stream()));
+}
+
 // The solvers' operator: products with A (or A^T) on its FP64 cell-block shadow when the CPR
 // setup recorded a cell size, else on the CSR arrays.
 struct AOp {
@@ -394,9 +416,17 @@ struct AOp {
       }
     spmv_stream_f(M, M->v.p, x, e);
   }
-  void apply(const double* x, double* y) const { product(x, EpiOuter{y}); }
+  void apply(const double* x, double* y) const {
+    outer_mark();
+    product(x, EpiOuter{y});
+    outer_mark();
+  }
   // x held in FP32 (flexible GMRES directions), FP64 values and sums
-  void apply(const float* x, double* y) const { product(x, EpiOuter{y}); }
+  void apply(const float* x, double* y) const {
+    outer_mark();
+    product(x, EpiOuter{y});
+    outer_mark();
+  }
   void residual(const double* x, const double* b, double* r) const {  // r = b - A x
     product(x, EpiStd<EPI_RES, double, double>{r, b});
   }
@@ -1141,6 +1171,31 @@ int deco_prec_set_layout(deco_prec* M, int layout) {
     if (layout != DECO_LAYOUT_BLOCK && layout != DECO_LAYOUT_CSR) fail(DECO_ERR_ARG, "bad layout %d", layout);
     M->set_layout(layout);
     sync();
+  });
+}
+
+int deco_time_outer_products(int on) {
+  return guard([&] {
+    OuterTiming& t = outer_timing();
+    t.on = on != 0;
+    t.used = 0;
+  });
+}
+
+int deco_outer_product_times(double* total_ms, long long* count) {
+  return guard([&] {
+    OuterTiming& t = outer_timing();
+    double tot = 0.0;
+    if (t.used >= 2) {
+      DECO_HIP(hipEventSynchronize(t.ev[t.used - 1]));
+      for (size_t k = 0; k + 1 < t.used; k += 2) {
+        float ms = 0.f;
+        DECO_HIP(hipEventElapsedTime(&ms, t.ev[k], t.ev[k + 1]));
+        tot += ms;
+      }
+    }
+    if (total_ms) *total_ms = tot;
+    if (count) *count = (long long)(t.used / 2);
   });
 }
 
