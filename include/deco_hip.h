@@ -348,10 +348,11 @@ int deco_profile_report(char* buf, size_t len, size_t* needed);
  * solve's achieved algorithmic bandwidth. No reference counterpart. */
 int deco_count_bytes(int on);
 int deco_counted_bytes(double* bytes, long long* launches);
-/* Event timing of the solvers' products with the system matrix (on != 0 starts, 0 stops; both
- * reset): a pair of events on the library stream around each product, so the average in-solve
- * duration of the roofline kernel can be read beside its back-to-back time. total_ms sums the
- * recorded products (waiting for the last one), count is their number. No reference
+/* Event timing of the solvers' operator products with the system matrix, the products that form
+ * the Krylov directions (on != 0 starts, 0 stops; both reset; the residual products b - A x are
+ * not timed): a pair of events on the library stream around each product, so the average
+ * in-solve duration of the roofline kernel can be read beside its back-to-back time. total_ms
+ * sums the recorded products (waiting for the last one), count is their number. No reference
  * counterpart. */
 int deco_time_outer_products(int on);
 int deco_outer_product_times(double* total_ms, long long* count);

@@ -250,3 +250,22 @@ def test_repeated_long_solves_are_bitwise_identical(gpu):
             S = deco_amd.SOL(sol, A, M, b)
             assert S.iter == first.iter and np.array_equal(S.x.numpy(), x0)
     M.set_precision(deco_amd.PREC_FP64)
+
+
+def test_outer_product_timing_counts_every_product_with_A(gpu):
+    """Event timing of the solvers' products with the system matrix: one interval per operator
+    product of the iteration (at least one per iteration; info.n_spmv also counts the residual
+    products b - A x, which are not timed); stopping resets and records nothing."""
+    g = gen_system(1)
+    A = dev_csr(g)
+    b = deco_amd.DeviceArray.from_numpy(g["b"])
+    M = deco_amd.Jacobi({"block_size": "2"}, A)
+    sol = {"type": "rgmres", "reltol": "1e-8", "restart": "30", "itmax": "40", "verbosity": "0"}
+    deco_amd.time_outer_products(True)
+    S = deco_amd.SOL(sol, A, M, b)
+    total, count = deco_amd.outer_product_times()
+    deco_amd.time_outer_products(False)
+    assert S.iter <= count < S.info.n_spmv
+    assert 0.0 < total < S.info.time_ms
+    deco_amd.SOL(sol, A, M, b)
+    assert deco_amd.outer_product_times() == (0.0, 0)
