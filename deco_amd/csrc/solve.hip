@@ -650,7 +650,7 @@ static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, cons
     double* u = r.p;
     int j = 0, k = 0;
     bool have_w = false;   // step j's operator application is already queued
-    double rn1 = -1.0, rn2 = -1.0;   // the last two residual estimates of this cycle
+    double rn1 = -1.0, rn2 = -1.0, rn3 = -1.0;   // the last three residual estimates of this cycle
     // one reduction of step j: s = Q^T u, |u|^2 (and z = Q^T w, u.w with w); the update sweep
     // is queued behind it when w is there. The reduction writes the values to red (pinned host
     // memory) itself, each followed by its sequence tag, so the host reads them while the update
@@ -690,6 +690,7 @@ static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, cons
       ++iter;
       k = j;
       rnorm = std::fabs(g[j]);
+      rn3 = rn2;
       rn2 = rn1;
       rn1 = rnorm;
       if (rnorm < absTol) { flag = 0; return true; }
@@ -729,11 +730,18 @@ static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, cons
       }
       reduce_step(u, wp, un);
       // Queue step j+1's operator application on u_{j+1} before the host reads step j, so the
-      // device works while the host rotates the Hessenberg column; if the solve stops at step j
-      // this application is not used (stream order keeps every later use of w, z after it).
-      // Skipped where the last two estimates extrapolate below the tolerance at this step.
+      // device works while the host rotates the Hessenberg column; if the solve stops at step j,
+      // or at step j+1 (which finalises column j from Q^T u_{j+1} and |u_{j+1}|^2 alone), this
+      // application is not used (stream order keeps every later use of w, z after it). Skipped
+      // where the estimates extrapolate below the tolerance at column j, two columns past the
+      // last finalised one, at the larger of the last two convergence factors (a wrong skip only
+      // serialises step j+1's application behind the host; the iterates are unchanged).
       const bool next_last = (j + 1 == m) || (it0 + j + 1 >= itmax);
-      const bool likely_stop = rn1 > 0.0 && rn2 > 0.0 && rn1 * (rn1 / rn2) < absTol;
+      bool likely_stop = false;
+      if (rn1 > 0.0 && rn2 > 0.0) {
+        const double q = rn3 > 0.0 ? std::max(rn1 / rn2, rn2 / rn3) : rn1 / rn2;
+        likely_stop = rn1 * q * q < absTol;
+      }
       have_w = wp && !next_last && !likely_stop;
       if (have_w) apply_op(j + 1, un);
       wait_tags(red_nv, red_seq, red);
