@@ -167,15 +167,18 @@ def adjoint_small(deco_amd, args, log):
             ts.append(time.perf_counter() - t1)
         return float(np.median(ts)) * 1e3, r
 
+    # outputs allocated once, outside the timed calls (x, mu, lambda, dL/dA)
+    xf, xv, mu = (deco_amd.DeviceArray(n) for _ in range(3))
+    lam, dA = deco_amd.DeviceArray(n), deco_amd.DeviceArray(g["nnz"])
     for prec, key in ((deco_amd.PREC_FP64, "fp64"), (deco_amd.PREC_FP32, "fp32_precond")):
         M.set_precision(prec)
-        fw_ms, Sf = med(lambda: deco_amd.SOL(sol, A, M, b))
-        fwv_ms, Sv = med(lambda: deco_amd.SOL(sol, A, M, w))
+        fw_ms, Sf = med(lambda: deco_amd.SOL(sol, A, M, b, xf))
+        fwv_ms, Sv = med(lambda: deco_amd.SOL(sol, A, M, w, xv))
 
         def adjoint():
-            St = deco_amd.SOL(sol, A, M, w, transpose=True)
-            lam = M.transform_transpose(St.x)
-            deco_amd.grad_A_pattern(A, lam, Sf.x)
+            St = deco_amd.SOL(sol, A, M, w, mu, transpose=True)
+            M.transform_transpose(St.x, out=lam)
+            deco_amd.grad_A_pattern(A, lam, Sf.x, out=dA)
             return St
         adj_ms, St = med(adjoint)
         out[key] = {"forward_ms": fw_ms, "forward_iter": Sf.iter, "adjoint_ms": adj_ms, "adjoint_iter": St.iter,
@@ -411,27 +414,28 @@ def main():
         the library's byte count on, the solve's algorithmic bytes"""
         Mp.set_precision(precision)
         sp_ = dict(sol, flexible=str(flexible))
+        xbuf = deco_amd.DeviceArray(n)   # the solution's storage, allocated outside the timed solves
         for _ in range(args.warmup):
-            deco_amd.SOL(sp_, Ap, Mp, bp)
+            deco_amd.SOL(sp_, Ap, Mp, bp, xbuf)
         deco_amd.lib().deco_sync()
         times, last = [], None
         t1 = time.perf_counter()
         for _ in range(args.steps):
-            last = deco_amd.SOL(sp_, Ap, Mp, bp)
+            last = deco_amd.SOL(sp_, Ap, Mp, bp, xbuf)
             times.append(last.info.time_ms)
         deco_amd.lib().deco_sync()
         wall = (time.perf_counter() - t1) / args.steps
+        x = last.x.numpy()
         deco_amd.count_bytes(True)
-        deco_amd.SOL(sp_, Ap, Mp, bp)
+        deco_amd.SOL(sp_, Ap, Mp, bp, xbuf)
         nbytes, nlaunch = deco_amd.counted_bytes()
         deco_amd.count_bytes(False)
         # one more solve with events around each product with A: the roofline kernel's in-solve time
         deco_amd.time_outer_products(True)
-        deco_amd.SOL(sp_, Ap, Mp, bp)
+        deco_amd.SOL(sp_, Ap, Mp, bp, xbuf)
         otot, ocnt = deco_amd.outer_product_times()
         deco_amd.time_outer_products(False)
         ms = float(np.median(times))
-        x = last.x.numpy()
         r = dict(ms=ms, wall_ms=wall * 1e3, iter=last.iter, res_rel=last.res_rel, flag=last.flag,
                  n_prec=last.info.n_prec, alg_bytes=nbytes, launches=nlaunch,
                  alg_GBps=nbytes / (ms * 1e-3) / 1e9, frac_of_peak=nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
@@ -533,22 +537,24 @@ def main():
         w = deco_amd.DeviceArray.from_numpy(np.random.default_rng(1004).standard_normal(n))
         fwd = {deco_amd.PREC_FP64: r64, deco_amd.PREC_FP32: r32}
         result["adjoint"] = {}
+        # outputs allocated once, outside the timed calls (mu, lambda, dL/dA on A's pattern, x)
+        mu, lam, xw = (deco_amd.DeviceArray(n) for _ in range(3))
+        dA = deco_amd.DeviceArray(g["nnz"])
         for prec, key in ((deco_amd.PREC_FP64, "fp64"), (deco_amd.PREC_FP32, "fp32_precond")):
             M.set_precision(prec)
-            St = deco_amd.SOL(sol, A, M, w, transpose=True)      # warm-up (builds A^T and the transposes)
+            St = deco_amd.SOL(sol, A, M, w, mu, transpose=True)  # warm-up (builds A^T and the transposes)
             deco_amd.lib().deco_sync()
             ts, tf = [], []
             for _ in range(args.steps):
                 t1 = time.perf_counter()
-                St = deco_amd.SOL(sol, A, M, w, transpose=True)
-                lam = M.transform_transpose(St.x)
-                dA = deco_amd.grad_A_pattern(A, lam, S64.x)
+                St = deco_amd.SOL(sol, A, M, w, mu, transpose=True)
+                M.transform_transpose(St.x, out=lam)
+                deco_amd.grad_A_pattern(A, lam, S64.x, out=dA)
                 deco_amd.lib().deco_sync()
                 ts.append(time.perf_counter() - t1)
-            del dA
             for _ in range(args.steps):
                 t1 = time.perf_counter()
-                Sf = deco_amd.SOL(sol, A, M, w)
+                Sf = deco_amd.SOL(sol, A, M, w, xw)
                 deco_amd.lib().deco_sync()
                 tf.append(time.perf_counter() - t1)
             adj_ms, fw_ms = float(np.median(ts)) * 1e3, float(np.median(tf)) * 1e3

@@ -554,9 +554,10 @@ class CPR(Prec):
         check(lib().deco_cpr_left_transform(self.h, _vp(W)))
         return W
 
-    def transform_transpose(self, mu):
+    def transform_transpose(self, mu, out=None):
+        """lambda = W^T mu (W the decoupling's left transform), into out (a DeviceArray) or a new one."""
         md = as_dev(mu)
-        lam = DeviceArray(md.n)
+        lam = out if out is not None else DeviceArray(md.n)
         check(lib().deco_cpr_transform_transpose(self.h, md.ptr, lam.ptr))
         return lam
 
@@ -663,10 +664,10 @@ def profile_report():
     return buf.value.decode()
 
 
-def grad_A_pattern(A, lam, x):
-    """dL/dA_ij = -lambda_i x_j on A's pattern (device)."""
+def grad_A_pattern(A, lam, x, out=None):
+    """dL/dA_ij = -lambda_i x_j on A's pattern (device), into out (a DeviceArray) or a new one."""
     ld, xd = as_dev(lam), as_dev(x)
-    dA = DeviceArray(A.nz)
+    dA = out if out is not None else DeviceArray(A.nz)
     check(lib().deco_grad_A_pattern(A.h, ld.ptr, xd.ptr, dA.ptr))
     return dA
 
