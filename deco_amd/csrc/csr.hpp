@@ -587,13 +587,15 @@ __device__ __forceinline__ int sell_col(const CT* __restrict__ bc, size_t e, int
 // Each lane sums its block row's entries in column order, KU at a time: a batch's loads are all
 // in flight before its gathers. FP32 values take batches (8 scalar entries, 4 blocks) whose last
 // one is clamped to the row's last entry with zero values, so a slice of width w takes
-// ceil(w/KU) dependent rounds; FP64 values take batches of 4 and a remainder loop of single
-// entries (measured: clamped batches cost these more in registers than they gain).
-template <class VT, int R, int C>
+// ceil(w/KU) dependent rounds. The solvers' FP64 product with 2 x 2 blocks (EpiOuter) takes
+// clamped batches of 7, one round for the 7 blocks of a 7-point cell row; other FP64 products
+// take batches of 4 and a remainder loop of single entries (measured: clamped or larger batches
+// cost those more in registers than they gain).
+template <class VT, int R, int C, class E>
 constexpr int sell_batch() {
-  return R * C * sizeof(VT) <= 4 ? 8 : 4;
+  return R * C * sizeof(VT) <= 4 ? 8 : (sizeof(VT) == 8 && R * C == 4 && std::is_same<E, EpiOuter>::value) ? 7 : 4;
 }
-template <class VT, int R, int C, class XT, class E, class CT = int, int KU = sell_batch<VT, R, C>()>
+template <class VT, int R, int C, class XT, class E, class CT = int, int KU = sell_batch<VT, R, C, E>()>
 __global__ __launch_bounds__(256) void k_sell_stream(int nb, int nslices, const int* __restrict__ soff,
                                                     const int* __restrict__ swid, const CT* __restrict__ bc,
                                                     const VT* __restrict__ bv, size_t T, const XT* __restrict__ x,
@@ -614,7 +616,7 @@ __global__ __launch_bounds__(256) void k_sell_stream(int nb, int nslices, const 
   AT acc[R];
 #pragma unroll
   for (int rr = 0; rr < R; ++rr) acc[rr] = AT(0);
-  constexpr bool clamped = (sizeof(VT) == 4);
+  constexpr bool clamped = (sizeof(VT) == 4) || KU == 7;
   int k = 0;
   for (; clamped ? k < w : k + KU <= w; k += KU) {
     int cc[KU];
