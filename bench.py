@@ -355,7 +355,7 @@ def main():
             "bound": "hbm", "achieved": spmv["block_fp64_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": spmv["block_fp64_GBps"] / HBM_PEAK_GBS, "traffic": traffic_from_profiles("spmv_block_fp64", key),
             "kernel": ((f"k_sell_stream<double,{bs},{bs},double,EpiOuter,"
-                        f"{'short' if '16-bit' in spmv['block_layout'] else 'int'}>")
+                        f"{'short' if '16-bit' in spmv['block_layout'] else 'int'},{7 if bs == 2 else 4}>")
                        if spmv["block_layout"].startswith("sliced")
                        else f"k_bsr_stream<double,{bs},{bs},double,EpiOuter>")
                       + f" (FP64 {bs}x{bs} cell-block product with A, {spmv['block_layout']})",
