@@ -587,13 +587,20 @@ __device__ __forceinline__ int sell_col(const CT* __restrict__ bc, size_t e, int
 // Each lane sums its block row's entries in column order, KU at a time: a batch's loads are all
 // in flight before its gathers. FP32 values take batches (8 scalar entries, 4 blocks) whose last
 // one is clamped to the row's last entry with zero values, so a slice of width w takes
-// ceil(w/KU) dependent rounds. The solvers' FP64 product with 2 x 2 blocks (EpiOuter) takes
-// clamped batches of 7, one round for the 7 blocks of a 7-point cell row; other FP64 products
-// take batches of 4 and a remainder loop of single entries (measured: clamped or larger batches
-// cost those more in registers than they gain).
+// ceil(w/KU) dependent rounds. The products of full 7-entry block rows -- the solvers' product
+// with A (EpiOuter) and the residuals with A and with its pressure rows, blocks of up to 4
+// values -- take clamped batches of 7, one round for the 7 blocks of a 7-point cell row; other
+// FP64 products (the FSAI factors' 4-block rows among them) take batches of 4 and a remainder
+// loop of single entries (measured: clamped or larger batches cost those more in registers than
+// they gain).
+template <class E> struct is_res_epi : std::false_type {};
+template <class YT, class ZT> struct is_res_epi<EpiStd<EPI_RES, YT, ZT>> : std::true_type {};
+template <class YT, class ZT> struct is_res_epi<EpiStrideRes<YT, ZT>> : std::true_type {};
+template <class YT, class ZT> struct is_res_epi<EpiStrideResJ0<YT, ZT>> : std::true_type {};
 template <class VT, int R, int C, class E>
 constexpr int sell_batch() {
-  return R * C * sizeof(VT) <= 4 ? 8 : (sizeof(VT) == 8 && R * C == 4 && std::is_same<E, EpiOuter>::value) ? 7 : 4;
+  return R * C * sizeof(VT) <= 4 ? 8
+         : ((R * C == 4 && std::is_same<E, EpiOuter>::value) || (R * C <= 4 && is_res_epi<E>::value)) ? 7 : 4;
 }
 template <class VT, int R, int C, class XT, class E, class CT = int, int KU = sell_batch<VT, R, C, E>()>
 __global__ __launch_bounds__(256) void k_sell_stream(int nb, int nslices, const int* __restrict__ soff,
