@@ -126,9 +126,6 @@ struct EpiResG {
   size_t row_bytes() const { return 4 * sizeof(T); }
 };
 
-// the up-sweep residual runs over full level rows: clamped batches of 7 in the sliced product
-template <class T> struct is_res_epi<EpiResG<T>> : std::true_type {};
-
 // One inner step of a two-stage Gauss-Seidel sweep, the product being (T g)[o]:
 // y = x[o] + (rho[o] - (T g)[o])/d[o] (x NULL: 0 + ...); stored in out, or with add_to set added to
 // add_to[o*stride] (the finest level's last sweep of the fused CPR stage)

@@ -585,24 +585,15 @@ __device__ __forceinline__ int sell_col(const CT* __restrict__ bc, size_t e, int
 }
 
 // Each lane sums its block row's entries in column order, KU at a time: a batch's loads are all
-// in flight before its gathers. FP32 values take batches (8 scalar entries, 4 blocks) whose last
-// one is clamped to the row's last entry with zero values, so a slice of width w takes
-// ceil(w/KU) dependent rounds. The products of full 7-entry block rows -- the solvers' product
-// with A (EpiOuter) and the residuals with A and with its pressure rows, blocks of up to 4
-// values -- take clamped batches of 7, one round for the 7 blocks of a 7-point cell row; other
-// FP64 products (the FSAI factors' 4-block rows among them) take batches of 4 and a remainder
-// loop of single entries (measured: clamped or larger batches cost those more in registers than
-// they gain).
-template <class E> struct is_res_epi : std::false_type {};
-template <class YT, class ZT> struct is_res_epi<EpiStd<EPI_RES, YT, ZT>> : std::true_type {};
-template <class YT, class ZT> struct is_res_epi<EpiStrideRes<YT, ZT>> : std::true_type {};
-template <class YT, class ZT> struct is_res_epi<EpiStrideResJ0<YT, ZT>> : std::true_type {};
-template <class VT, int R, int C, class E>
-constexpr int sell_batch() {
-  return R * C * sizeof(VT) <= 4 ? 8
-         : ((R * C == 4 && std::is_same<E, EpiOuter>::value) || (R * C <= 4 && is_res_epi<E>::value)) ? 7 : 4;
-}
-template <class VT, int R, int C, class XT, class E, class CT = int, int KU = sell_batch<VT, R, C, E>()>
+// in flight before its gathers. The launcher picks KU from the value type and the layout's mean
+// width: 8 for scalar FP32 entries; 7 for other blocks of up to 4 values when the slices average
+// more than 5 blocks per row (the 7-block cell rows of A, the 13-block rows of an FSAI factor on
+// the A^2 pattern: one or two rounds); 4 otherwise (the FSAI factors' 4-block rows on A's
+// pattern, 3 x 3 blocks). FP32 batches and batches of 7 are clamped: the last one of a row
+// re-reads the row's last entry with zero values, so a slice of width w takes ceil(w/KU)
+// dependent rounds. FP64 batches of 4 keep a remainder loop of single entries (measured: clamped
+// or larger batches cost those products more in registers than they gain).
+template <class VT, int R, int C, class XT, class E, class CT, int KU>
 __global__ __launch_bounds__(256) void k_sell_stream(int nb, int nslices, const int* __restrict__ soff,
                                                     const int* __restrict__ swid, const CT* __restrict__ bc,
                                                     const VT* __restrict__ bv, size_t T, const XT* __restrict__ x,
@@ -675,14 +666,23 @@ void bsr_stream_v(const deco_bsr* B, const VT* bv, const XT* x, E epi) {
   if (B->nb <= 0) return;
   if (B->sell) {
     const unsigned grid = (unsigned)((B->nslices + 3) / 4);
-#define DECO_SELL_LAUNCH(RR, CC)                                                                             \
-  do {                                                                                                       \
-    if (B->d16)                                                                                              \
-      hipLaunchKernelGGL((k_sell_stream<VT, RR, CC, XT, E, short>), dim3(grid), dim3(256), 0, stream(), B->nb, \
-                         B->nslices, B->soff.p, B->swid.p, (const short*)B->bcd.p, bv, B->stot, x, epi);      \
-    else                                                                                                     \
-      hipLaunchKernelGGL((k_sell_stream<VT, RR, CC, XT, E, int>), dim3(grid), dim3(256), 0, stream(), B->nb,   \
-                         B->nslices, B->soff.p, B->swid.p, (const int*)B->bc.p, bv, B->stot, x, epi);         \
+    const bool wide = B->stot > (size_t)5 * kSellRows * B->nslices;   // mean width over 5 blocks
+#define DECO_SELL_LAUNCH_KU(RR, CC, KU)                                                                          \
+  do {                                                                                                           \
+    if (B->d16)                                                                                                  \
+      hipLaunchKernelGGL((k_sell_stream<VT, RR, CC, XT, E, short, KU>), dim3(grid), dim3(256), 0, stream(), B->nb, \
+                         B->nslices, B->soff.p, B->swid.p, (const short*)B->bcd.p, bv, B->stot, x, epi);          \
+    else                                                                                                         \
+      hipLaunchKernelGGL((k_sell_stream<VT, RR, CC, XT, E, int, KU>), dim3(grid), dim3(256), 0, stream(), B->nb,   \
+                         B->nslices, B->soff.p, B->swid.p, (const int*)B->bc.p, bv, B->stot, x, epi);             \
+  } while (0)
+#define DECO_SELL_LAUNCH(RR, CC)                                                        \
+  do {                                                                                  \
+    if constexpr (RR * CC * sizeof(VT) <= 4) DECO_SELL_LAUNCH_KU(RR, CC, 8);            \
+    else if constexpr (RR * CC <= 4) {                                                  \
+      if (wide) DECO_SELL_LAUNCH_KU(RR, CC, 7);                                         \
+      else DECO_SELL_LAUNCH_KU(RR, CC, 4);                                              \
+    } else DECO_SELL_LAUNCH_KU(RR, CC, 4);                                              \
   } while (0)
     if (B->R == 2 && B->C == 2) DECO_SELL_LAUNCH(2, 2);
     else if (B->R == 1 && B->C == 1) DECO_SELL_LAUNCH(1, 1);
@@ -691,6 +691,7 @@ void bsr_stream_v(const deco_bsr* B, const VT* bv, const XT* x, E epi) {
     else if (B->R == 1 && B->C == 3) DECO_SELL_LAUNCH(1, 3);
     else fail(DECO_ERR_UNSUPPORTED, "block shape %d x %d", B->R, B->C);
 #undef DECO_SELL_LAUNCH
+#undef DECO_SELL_LAUNCH_KU
     DECO_CHECK_LAUNCH();
     count_bytes((double)B->nnzb * (B->R * B->C * sizeof(VT) + (B->d16 ? 2 : 4)) + 8.0 * B->nslices +
                 (double)sizeof(XT) * B->mb * B->C + (double)B->nb * B->R * epi.row_bytes());
