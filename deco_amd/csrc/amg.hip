@@ -362,6 +362,83 @@ __global__ __launch_bounds__(256) void k_dense(int n, const MT* __restrict__ C, 
   }
 }
 
+// x = Z b for the dense sub-cycle operator Z, n rows of ld >= n entries (ld a multiple of 4, the
+// columns past n zero, so every row starts on a 16-byte boundary): two rows per wavefront, each lane
+// loads 16-byte vectors of both rows and the b entries they pair with, kDenseU vectors per row in
+// flight (one scalar load per lane and row kept about 4 MB in flight over the chip, a third of what
+// the HBM needs); FP64 arithmetic, fixed-order butterfly sums; with add_to set,
+// add_to[i*stride] += x_i instead of storing x
+constexpr int kDenseU = 6;
+template <class T, class MT>
+__global__ __launch_bounds__(256) void k_dense_tail(int n, int ld, const MT* __restrict__ Z, const T* __restrict__ b,
+                                                    T* __restrict__ x, T* __restrict__ add_to, int stride) {
+  constexpr int VM = 16 / sizeof(MT);   // entries per 16-byte vector
+  typedef MT mv __attribute__((ext_vector_type(VM)));
+  const int lane = threadIdx.x & 63;
+  const int row0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 2;
+  if (row0 >= n) return;
+  const bool two = row0 + 1 < n;
+  const mv* z0 = reinterpret_cast<const mv*>(Z + (size_t)row0 * ld);
+  const mv* z1 = two ? z0 + ld / VM : z0;
+  const bool bvec = ((uintptr_t)b & 15) == 0 && sizeof(T) == sizeof(MT);
+  const int nv = ld / VM;
+  double s0 = 0.0, s1 = 0.0;
+  for (int v0 = 0; v0 < nv; v0 += 64 * kDenseU) {
+    mv a0[kDenseU], a1[kDenseU];
+    T bb[kDenseU][VM];
+#pragma unroll
+    for (int u = 0; u < kDenseU; ++u) {
+      const int v = v0 + lane + 64 * u;
+      if (v < nv) {
+        a0[u] = __builtin_nontemporal_load(z0 + v);
+        a1[u] = __builtin_nontemporal_load(z1 + v);
+      } else {
+        a0[u] = mv(0);
+        a1[u] = mv(0);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kDenseU; ++u) {
+      const int e = (v0 + lane + 64 * u) * VM;
+      if (bvec && e + VM <= n) {
+        const mv t = *reinterpret_cast<const mv*>(b + e);
+#pragma unroll
+        for (int q = 0; q < VM; ++q) bb[u][q] = (T)t[q];
+      } else {
+#pragma unroll
+        for (int q = 0; q < VM; ++q) bb[u][q] = e + q < n ? b[e + q] : T(0);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kDenseU; ++u)
+#pragma unroll
+      for (int q = 0; q < VM; ++q) {
+        s0 += (double)a0[u][q] * (double)bb[u][q];
+        s1 += (double)a1[u][q] * (double)bb[u][q];
+      }
+  }
+  s0 = wave_sum(s0);
+  s1 = wave_sum(s1);
+  if (lane == 0) {
+    if (add_to) {
+      add_to[(size_t)row0 * stride] += (T)s0;
+      if (two) add_to[(size_t)(row0 + 1) * stride] += (T)s1;
+    } else {
+      x[row0] = (T)s0;
+      if (two) x[row0 + 1] = (T)s1;
+    }
+  }
+}
+
+// dst (n rows of ld entries) = src (n x n, row-major) with zero columns n .. ld-1
+template <class S, class D>
+__global__ __launch_bounds__(256) void k_pad_rows(int n, int ld, const S* __restrict__ src, D* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (int64_t)n * ld) return;
+  const int r = (int)(i / ld), c = (int)(i % ld);
+  dst[i] = c < n ? (D)src[(size_t)r * n + c] : D(0);
+}
+
 // C (M x N) = A (M x K) B (K x N), all row-major FP64 (the dense tail's setup products): 64 x 64
 // output tiles, 16-deep slices of A and B staged in LDS, 4 x 4 outputs per thread
 constexpr int kGemmTile = 64, kGemmDepth = 16;
@@ -476,7 +553,8 @@ struct AMG : deco_prec {
   // smoothers, last level of at most g_dense_tail_rows rows); tail = its rows, 0 when absent
   int tail = 0;
   int tail_lev = -1;          // the level whose sub-cycle the dense operator replaces (-1: none)
-  DBuf<double> Z64, Z64t;     // n x n, row-major
+  int tail_ld = 0;            // row length of the stored operators: tail rounded up to a multiple of 4
+  DBuf<double> Z64, Z64t;     // tail rows of tail_ld entries (zero past column tail), row-major
   DBuf<float> Z32, Z32t;
   int relax = 18;              // 18 l1-Jacobi, 0 Jacobi (weighted), 17 F-C-F, 11 two-stage Gauss-Seidel
   double weight = 1.0;
@@ -752,17 +830,28 @@ struct AMG : deco_prec {
       nC = nl;
     }
     const int nl = nC;
-    Z64 = std::move(Cprev);
-    Z64t.alloc((size_t)nl * nl);
+    const int ld = (nl + 3) / 4 * 4;
+    DBuf<double> Zt((size_t)nl * nl);
     hipLaunchKernelGGL(k_transpose_dense, dim3((nl + 31) / 32, (nl + 31) / 32), dim3(256), 0, stream(), nl,
-                       (const double*)Z64.p, Z64t.p);
+                       (const double*)Cprev.p, Zt.p);
     DECO_CHECK_LAUNCH();
-    Z32.alloc((size_t)nl * nl);
-    Z32t.alloc((size_t)nl * nl);
-    convert((const double*)Z64.p, Z32.p, (int64_t)nl * nl);
-    convert((const double*)Z64t.p, Z32t.p, (int64_t)nl * nl);
+    const unsigned g = grid_for((int64_t)nl * ld, 256);
+    Z64.alloc((size_t)nl * ld);
+    Z64t.alloc((size_t)nl * ld);
+    Z32.alloc((size_t)nl * ld);
+    Z32t.alloc((size_t)nl * ld);
+    hipLaunchKernelGGL((k_pad_rows<double, double>), dim3(g), dim3(256), 0, stream(), nl, ld, (const double*)Cprev.p,
+                       Z64.p);
+    hipLaunchKernelGGL((k_pad_rows<double, double>), dim3(g), dim3(256), 0, stream(), nl, ld, (const double*)Zt.p,
+                       Z64t.p);
+    hipLaunchKernelGGL((k_pad_rows<double, float>), dim3(g), dim3(256), 0, stream(), nl, ld, (const double*)Cprev.p,
+                       Z32.p);
+    hipLaunchKernelGGL((k_pad_rows<double, float>), dim3(g), dim3(256), 0, stream(), nl, ld, (const double*)Zt.p,
+                       Z32t.p);
+    DECO_CHECK_LAUNCH();
     sync();
     tail = nl;
+    tail_ld = ld;
     tail_lev = k;
   }
 
@@ -770,6 +859,15 @@ struct AMG : deco_prec {
   template <class T, class MT>
   void dense(int m, const MT* C, const T* b, T* x, T* add_to, int stride) {
     hipLaunchKernelGGL((k_dense<T, MT>), dim3((m + 3) / 4), dim3(256), 0, stream(), m, C, b, x, add_to, stride);
+    DECO_CHECK_LAUNCH();
+    count_bytes((double)m * m * sizeof(MT) + (double)m * sizeof(T) * (add_to ? 3 : 2));
+  }
+  // x = Z b (or add_to[stride*i] += (Z b)_i) with the dense tail operator (tail rows of tail_ld)
+  template <class T, class MT>
+  void dense_tail(const MT* Zop, const T* b, T* x, T* add_to, int stride) {
+    const int m = tail;
+    hipLaunchKernelGGL((k_dense_tail<T, MT>), dim3((m + 7) / 8), dim3(256), 0, stream(), m, tail_ld, Zop, b, x, add_to,
+                       stride);
     DECO_CHECK_LAUNCH();
     count_bytes((double)m * m * sizeof(MT) + (double)m * sizeof(T) * (add_to ? 3 : 2));
   }
@@ -818,7 +916,7 @@ struct AMG : deco_prec {
       return;
     }
     if (tail && (int)l == tail_lev) {   // the whole sub-cycle below as one product
-      dense(tail, tail_op(tr, b), b, x, add_to, stride);
+      dense_tail(tail_op(tr, b), b, x, add_to, stride);
       return;
     }
     if (relax == 11) {

@@ -247,6 +247,27 @@ struct PtrGather {
   __device__ __forceinline__ XT operator()(int j) const { return x[j]; }
 };
 
+// Sum of prod[k], prod[k + G], ... below e: in that order when inorder (rows averaging under 8
+// entries, summed as alg::spmv sums them), else in four interleaved partial sums added pairwise,
+// so four LDS reads are in flight instead of one dependent chain
+template <class AT>
+__device__ __forceinline__ AT stream_row_sum(const AT* prod, int k, int e, int G, bool inorder) {
+  if (inorder) {
+    AT a = AT(0);
+    for (; k < e; k += G) a += prod[k];
+    return a;
+  }
+  AT a0 = AT(0), a1 = AT(0), a2 = AT(0), a3 = AT(0);
+  for (; k + 3 * G < e; k += 4 * G) {
+    a0 += prod[k];
+    a1 += prod[k + G];
+    a2 += prod[k + 2 * G];
+    a3 += prod[k + 3 * G];
+  }
+  for (; k < e; k += G) a0 += prod[k];
+  return (a0 + a1) + (a2 + a3);
+}
+
 // CT int: c holds the columns. CT unsigned short: c holds 16-bit offsets from cbase[chunk]
 // (deco_csr::cols16).
 template <class VT, class XG, class E, class CT = int>
@@ -311,14 +332,13 @@ __global__ __launch_bounds__(stream_block<VT>()) void k_spmv_stream(const int2* 
   if (fits) {
     if (row < r1) {   // a lane group is active or inactive as a whole
       const int G = 1 << lg;
-      AT acc = AT(0);
-      for (int k = rbeg - w0 + sub; k < rend - w0; k += G) acc += prod[k];
+      const bool inorder = cnt < 8 * nr;   // short rows: column order
+      AT acc = stream_row_sum<AT>(prod, rbeg - w0 + sub, rend - w0, G, inorder);
       for (int off = G >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
       if (sub == 0) epi(row, acc, pre);
       for (int rw = row + (BLK >> lg); rw < r1; rw += BLK >> lg) {
         const int b = r[rw] - w0, e = r[rw + 1] - w0;
-        AT a2 = AT(0);
-        for (int k = b + sub; k < e; k += G) a2 += prod[k];
+        AT a2 = stream_row_sum<AT>(prod, b + sub, e, G, inorder);
         for (int off = G >> 1; off > 0; off >>= 1) a2 += __shfl_xor(a2, off);
         if (sub == 0) epi(rw, a2, epi.pre(rw));
       }
