@@ -686,7 +686,8 @@ void bsr_stream_v(const deco_bsr* B, const VT* bv, const XT* x, E epi) {
   if (B->nb <= 0) return;
   if (B->sell) {
     const unsigned grid = (unsigned)((B->nslices + 3) / 4);
-    const bool wide = B->stot > (size_t)5 * kSellRows * B->nslices;   // mean width over 5 blocks
+    const bool wide = B->stot > (size_t)5 * kSellRows * B->nslices;     // mean width over 5 blocks
+    const bool narrow = B->stot <= (size_t)4 * kSellRows * B->nslices;  // at most 4 on average
 #define DECO_SELL_LAUNCH_KU(RR, CC, KU)                                                                          \
   do {                                                                                                           \
     if (B->d16)                                                                                                  \
@@ -698,7 +699,10 @@ void bsr_stream_v(const deco_bsr* B, const VT* bv, const XT* x, E epi) {
   } while (0)
 #define DECO_SELL_LAUNCH(RR, CC)                                                        \
   do {                                                                                  \
-    if constexpr (RR * CC * sizeof(VT) <= 4) DECO_SELL_LAUNCH_KU(RR, CC, 8);            \
+    if constexpr (RR * CC * sizeof(VT) <= 4) {                                          \
+      if (narrow) DECO_SELL_LAUNCH_KU(RR, CC, 4);                                       \
+      else DECO_SELL_LAUNCH_KU(RR, CC, 8);                                              \
+    }                                                                                   \
     else if constexpr (RR * CC <= 4) {                                                  \
       if (wide) DECO_SELL_LAUNCH_KU(RR, CC, 7);                                         \
       else DECO_SELL_LAUNCH_KU(RR, CC, 4);                                              \
