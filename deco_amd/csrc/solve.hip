@@ -65,14 +65,20 @@ __global__ __launch_bounds__(kRedBlock) void k_mgs_norm(int n, const double* __r
   for (int i = blockIdx.x * kRedBlock + threadIdx.x; i < n; i += gridDim.x * kRedBlock) v[i] = inv * w[i];
 }
 
-// w = g0*V0 + sum_{i=1}^{j-1} g_i V_i in the order of rgmres.tpp:106-109
+// w = g0*V0 + sum_{i=1}^{j-1} g_i V_i in the order of rgmres.tpp:106-109; the coefficients come
+// as kernel arguments (gc, j <= kCombineArgs) or from device memory (g)
+constexpr int kCombineArgs = 64;
+struct CombineCoef {
+  double g[kCombineArgs];
+};
 template <class QT>
 __global__ void k_combine(int n, int j, const QT* __restrict__ V, int64_t ldv, const double* __restrict__ g,
-                          double* __restrict__ w) {
+                          CombineCoef gc, double* __restrict__ w) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  double s = g[0] * (double)V[k];
-  for (int i = 1; i < j; ++i) s = 1.0 * s + g[i] * (double)V[(int64_t)i * ldv + k];
+  const double* c = j <= kCombineArgs ? gc.g : g;
+  double s = c[0] * (double)V[k];
+  for (int i = 1; i < j; ++i) s = 1.0 * s + c[i] * (double)V[(int64_t)i * ldv + k];
   w[k] = s;
 }
 
@@ -315,6 +321,19 @@ static WsRef ws(int slot, size_t count) {
   return WsRef{g_ws[slot].p};
 }
 
+// w = sum_i g_i V_i (j columns of V with stride ldv), the j host coefficients passed to the kernel
+// as arguments when they fit, else uploaded to gdev first
+template <class QT>
+static void combine(int n, int j, const QT* V, int64_t ldv, const double* g, double* gdev, double* w) {
+  CombineCoef gc{};
+  if (j <= kCombineArgs) std::memcpy(gc.g, g, sizeof(double) * std::max(j, 0));
+  else copy_h2d(gdev, g, j);
+  hipLaunchKernelGGL(k_combine<QT>, dim3(grid_for(n, 256)), dim3(256), 0, stream(), n, j, V, ldv,
+                     (const double*)gdev, gc, w);
+  DECO_CHECK_LAUNCH();
+  count_bytes((double)n * (j * sizeof(QT) + 8) + 8.0 * j);
+}
+
 // ------------------------------------------------------------------ host helpers -------
 struct Solver {
   int n;
@@ -542,15 +561,8 @@ static void gmres(bool left, const deco_sol_params& sp, const AOp& A, const MOp&
       g[i] /= Hm(i, i);
       for (int k = 0; k < i; ++k) g[k] -= Hm(k, i) * g[i];
     }
-    copy_h2d(gdev.p, g, std::max(j, 1));
-    if (flex && j > 0)
-      hipLaunchKernelGGL(k_combine<float>, dim3(S.g256()), dim3(256), 0, stream(), n, j, (const float*)Z, (int64_t)n,
-                         (const double*)gdev.p, w.p);
-    else if (!flex)
-      hipLaunchKernelGGL(k_combine<double>, dim3(S.g256()), dim3(256), 0, stream(), n, j, (const double*)V.p,
-                         (int64_t)n, (const double*)gdev.p, w.p);
-    DECO_CHECK_LAUNCH();
-    count_bytes((double)n * (j * (flex ? 4 : 8) + 8) + 8.0 * j);
+    if (flex && j > 0) combine(n, j, (const float*)Z, (int64_t)n, g, gdev.p, w.p);
+    else if (!flex) combine(n, std::max(j, 1), (const double*)V.p, (int64_t)n, g, gdev.p, w.p);
     if (flex) {
       if (j > 0) S.add(w.p, x);
       A.residual(x, b, r.p); ++nspmv;
@@ -612,12 +624,17 @@ static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, cons
   std::vector<double> red_h(2 * (size_t)m + 2);   // the reduced values of the current step
   double* red = red_h.data();
   int flag = 1, iter = 0, napp = 0, nspmv = 0;
-  double bnorm, absTol, rnorm = 0.0;
+  double bnorm = -1.0, absTol = 0.0, rnorm = 0.0;
+  // Right variant from x = 0: r_0 = b, so step 0 reads b itself and |b| is the norm its reduction
+  // returns (set at step 0); the first cycle's update writes x instead of adding to it. No host
+  // round trip before the first operator application.
+  const bool from_zero = !left && !sp.use_x0;
   if (!left) {
-    bnorm = S.nrm2(b);
-    absTol = tol * bnorm;
-    if (!sp.use_x0) { zero_async(x, sizeof(double) * n); copy_d2d(r.p, b, n); }
-    else { A.residual(x, b, r.p); ++nspmv; }
+    if (sp.use_x0) {
+      bnorm = S.nrm2(b);
+      absTol = tol * bnorm;
+      A.residual(x, b, r.p); ++nspmv;
+    }
   } else {
     if (!sp.use_x0) { zero_async(x, sizeof(double) * n); M.apply(b, r.p); ++napp; }
     else { A.residual(x, b, z.p); ++nspmv; M.apply(z.p, r.p); ++napp; }
@@ -640,6 +657,11 @@ static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, cons
     } else { A.apply(uu, z.p); M.apply(z.p, w.p); }
     ++napp; ++nspmv;
   };
+  if (from_zero && itmax <= 0) {   // no step: x = 0 and |b| here
+    zero_async(x, sizeof(double) * n);
+    bnorm = S.nrm2(b);
+    absTol = tol * bnorm;
+  }
   long long red_seq = 0;   // sequence number and value count of the last queued reduction
   int red_nv = 0;
   while (iter < itmax) {
@@ -647,7 +669,8 @@ static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, cons
     std::fill(Rm.begin(), Rm.end(), 0.0);
     const int it0 = iter;
     double* bufs[2] = {ua.p, ub.p};
-    double* u = r.p;
+    const bool first_zero = from_zero && it0 == 0;   // x is still 0 (never written)
+    const double* u = first_zero ? b : r.p;
     int j = 0, k = 0;
     bool have_w = false;   // step j's operator application is already queued
     double rn1 = -1.0, rn2 = -1.0, rn3 = -1.0;   // the last three residual estimates of this cycle
@@ -747,6 +770,10 @@ static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, cons
       wait_tags(red_nv, red_seq, red);
       rho = rho_of(wp != nullptr, gam);
       if (j == 0) {
+        if (bnorm < 0.0) {   // |b| from step 0 of a solve from x = 0
+          bnorm = rho;
+          absTol = tol * bnorm;
+        }
         g[0] = rho;
         rnorm = std::fabs(g[0]);
         rn1 = rnorm;
@@ -785,20 +812,18 @@ static void gmres_dcgs2(bool left, const deco_sol_params& sp, const AOp& A, cons
           for (int q = 0; q < i; ++q) g[q] -= T[q + (size_t)i * ld] * g[i];
         }
       }
-      copy_h2d(gdev.p, g.data(), k);
-      if (flex32)
-        hipLaunchKernelGGL(k_combine<float>, dim3(S.g256()), dim3(256), 0, stream(), n, k, (const float*)Z,
-                           (int64_t)n, (const double*)gdev.p, w.p);
-      else if (flex64)
-        hipLaunchKernelGGL(k_combine<double>, dim3(S.g256()), dim3(256), 0, stream(), n, k, (const double*)Z64,
-                           (int64_t)n, (const double*)gdev.p, w.p);
-      else
-        hipLaunchKernelGGL(k_combine<double>, dim3(S.g256()), dim3(256), 0, stream(), n, k, (const double*)V.p,
-                           (int64_t)n, (const double*)gdev.p, w.p);
-      DECO_CHECK_LAUNCH();
-      count_bytes((double)n * (k * (flex32 ? 4 : 8) + 8) + 8.0 * k);
-      if (flex || left) S.add(w.p, x);
-      else { M.apply(w.p, z.p); ++napp; S.add(z.p, x); }
+      // x += (combination), or x = (combination) on the first cycle from x = 0
+      double* wout = (first_zero && flex) ? x : w.p;
+      if (flex32) combine(n, k, (const float*)Z, (int64_t)n, g.data(), gdev.p, wout);
+      else if (flex64) combine(n, k, (const double*)Z64, (int64_t)n, g.data(), gdev.p, wout);
+      else combine(n, k, (const double*)V.p, (int64_t)n, g.data(), gdev.p, w.p);
+      if (left) S.add(w.p, x);
+      else if (!flex) {
+        if (first_zero) { M.apply(w.p, x); ++napp; }
+        else { M.apply(w.p, z.p); ++napp; S.add(z.p, x); }
+      } else if (!first_zero) S.add(w.p, x);
+    } else if (first_zero) {
+      zero_async(x, sizeof(double) * n);   // no direction: x stays 0
     }
     if (!left) { A.residual(x, b, r.p); ++nspmv; }
     else { A.residual(x, b, z.p); ++nspmv; M.apply(z.p, r.p); ++napp; }
