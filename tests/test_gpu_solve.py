@@ -127,6 +127,44 @@ def test_gmres_breakdown_and_itmax_flags(gpu):
     assert Sg.flag == 1 and Sg.iter == 3
 
 
+def test_solution_buffer_written_from_zero(gpu):
+    """A solve into a device buffer starts from x = 0 whatever the buffer holds: the first cycle
+    writes x (flexible form: the combination of the stored directions; reference form: the
+    preconditioner applied to it) instead of adding to it. Buffers filled with NaN give the same
+    solution bit for bit as fresh ones, for both GMRES forms, the FP32 preconditioner and MGS."""
+    g, A, b, M, Ao, bo, Mo = setup_pair(1, small_params(2))
+    n = g["n"]
+    base = {"type": "rgmres", "reltol": "1e-9", "restart": "30", "itmax": "300", "verbosity": "0"}
+    cases = [(dict(base), deco_amd.PREC_FP64), (dict(base, flexible="1"), deco_amd.PREC_FP64),
+             (dict(base), deco_amd.PREC_FP32), (dict(base, ortho="mgs"), deco_amd.PREC_FP64),
+             (dict(base, restart="5"), deco_amd.PREC_FP64)]
+    for sol, prec in cases:
+        M.set_precision(prec)
+        S0 = deco_amd.SOL(sol, A, M, b)
+        xn = deco_amd.DeviceArray.from_numpy(np.full(n, np.nan))
+        S1 = deco_amd.SOL(sol, A, M, b, xn)
+        assert S0.flag == 0 and S1.flag == 0, sol
+        assert S1.iter == S0.iter and np.array_equal(S1.x.numpy(), S0.x.numpy()), (sol, prec)
+        assert rel(S1.x.numpy(), g["xstar"]) <= 1e-5
+    M.set_precision(deco_amd.PREC_FP64)
+
+
+def test_zero_rhs_and_zero_itmax_leave_x_zero(gpu):
+    """From x = 0: a zero right-hand side (no Krylov direction) and itmax 0 (no step) both return
+    x = 0 exactly in a buffer that held NaN."""
+    g, A, b, M, Ao, bo, Mo = setup_pair(1, small_params(2))
+    n = g["n"]
+    sol = {"type": "rgmres", "reltol": "1e-8", "restart": "30", "itmax": "300", "verbosity": "0"}
+    xn = deco_amd.DeviceArray.from_numpy(np.full(n, np.nan))
+    S = deco_amd.SOL(sol, A, M, np.zeros(n), xn)
+    assert np.array_equal(S.x.numpy(), np.zeros(n)) and S.iter == 0
+    xn = deco_amd.DeviceArray.from_numpy(np.full(n, np.nan))
+    S = deco_amd.SOL(dict(sol, itmax="0"), A, M, b, xn)
+    assert np.array_equal(S.x.numpy(), np.zeros(n)) and S.iter == 0 and S.flag == 1
+    bdec = np.linalg.norm(b.numpy())   # the CPR setup decoupled b in place
+    assert abs(S.info.bnorm - bdec) <= 1e-12 * bdec
+
+
 def test_adjoint_transposed_solve_and_gradient(gpu):
     cfg = 4
     tree = small_params(2)
