@@ -201,11 +201,11 @@ def cpu_model():
 
 def bsr_bytes(nb, mb, nnzb, R, C, sv, sx, sy, layout=0):
     # algorithmic bytes of the block product: block values + block columns (4 bytes, 2 for the
-    # 16-bit offsets of layout 2) + x + y, and the row structure the layout reads: block row
-    # pointers (chunked, layout 0) or per-slice offsets and widths of 64 block rows (sliced,
-    # layouts 1 and 2; their padding blocks are not counted)
+    # 16-bit offsets of layout 2, none for the uniform offsets of layout 3) + x + y, and the row
+    # structure the layout reads: block row pointers (chunked, layout 0) or per-slice offsets and
+    # widths of 64 block rows (sliced, layouts 1-3; their padding blocks are not counted)
     structure = 8 * ((nb + 63) // 64) if layout else 4 * (nb + 1)
-    col = 2 if layout == 2 else 4
+    col = {2: 2, 3: 0}.get(layout, 4)
     return nnzb * (R * C * sv + col) + structure + sx * mb * C + sy * nb * R
 
 
@@ -227,7 +227,9 @@ def spmv_section(deco_amd, A, n, nnz, iters, bs):
         bb64 = bsr_bytes(n // bs, n // bs, nnzb, bs, bs, 8, 8, 8, sliced)
         bb32 = bsr_bytes(n // bs, n // bs, nnzb, bs, bs, 4, 4, 4, sliced)
         out.update({"block_layout": ("chunked block CSR", "sliced (SELL, 64 block rows per slice)",
-                                     "sliced (SELL, 64 block rows per slice, 16-bit block-column offsets)")[sliced],
+                                     "sliced (SELL, 64 block rows per slice, 16-bit block-column offsets)",
+                                     "sliced (SELL, 64 block rows per slice, uniform block-column offsets: no "
+                                     "stored columns)")[sliced],
                     "block_padding": stored / max(nnzb, 1) - 1.0,
                     "blocks": nnzb, "block_fp64_us": msb64 * 1e3, "block_fp64_GBps": bb64 / (msb64 * 1e-3) / 1e9,
                     "block_fp32_us": msb32 * 1e3, "block_fp32_GBps": bb32 / (msb32 * 1e-3) / 1e9,
@@ -355,7 +357,8 @@ def main():
             "bound": "hbm", "achieved": spmv["block_fp64_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": spmv["block_fp64_GBps"] / HBM_PEAK_GBS, "traffic": traffic_from_profiles("spmv_block_fp64", key),
             "kernel": ((f"k_sell_stream<double,{bs},{bs},double,EpiOuter,"
-                        f"{'short' if '16-bit' in spmv['block_layout'] else 'int'},{7 if bs == 2 else 4}>")
+                        f"{'DiaCols' if 'uniform' in spmv['block_layout'] else 'short' if '16-bit' in spmv['block_layout'] else 'int'},"
+                        f"{7 if bs == 2 else 4}>")
                        if spmv["block_layout"].startswith("sliced")
                        else f"k_bsr_stream<double,{bs},{bs},double,EpiOuter>")
                       + f" (FP64 {bs}x{bs} cell-block product with A, {spmv['block_layout']})",

@@ -187,7 +187,8 @@ class CSR:
 
     def block_info(self, R, C_):
         """(layout, blocks, stored blocks) of the R x C_ block shadow (created if needed); layout
-        0 chunked, 1 sliced, 2 sliced with 16-bit block-column offsets."""
+        0 chunked, 1 sliced, 2 sliced with 16-bit block-column offsets, 3 sliced with uniform
+        offsets (no stored columns)."""
         sl, nb, ent = C.c_int(), C.c_int(), C.c_longlong()
         check(lib().deco_block_info(self.h, int(R), int(C_), C.byref(sl), C.byref(nb), C.byref(ent)))
         return sl.value, nb.value, ent.value
@@ -598,6 +599,13 @@ def set_sell_offsets16(enable):
     """Sliced block shadows made from now on store their block columns as 16-bit offsets from
     the block row when every offset fits (default), else (or when disabled) 32-bit columns."""
     check(lib().deco_set_sell_offsets16(1 if enable else 0))
+
+
+def set_sell_dia(enable):
+    """Block shadows made from now on take the uniform-offset sliced layout (no stored columns)
+    where the matrix has at most 16 distinct block-column offsets within the padding threshold
+    (default), else the plain sliced or chunked one."""
+    check(lib().deco_set_sell_dia(1 if enable else 0))
 
 
 def set_amg_dense_tail(max_rows):

@@ -2,6 +2,7 @@
 // create/copy/download, filters, pressure extraction, transpose, SpMV.
 // Reference: core/CSR.hpp (container and methods), core/alg.hpp:51-64 (spmv).
 #include "csr.hpp"
+#include <algorithm>
 #include <climits>
 #include <cstring>
 #include <mutex>
@@ -670,6 +671,7 @@ __global__ void k_bsr_fill(int nb, int R, int C, const int* __restrict__ r, cons
 // negative: always the chunked one.
 static double g_sell_padding = 0.10;
 static bool g_sell_offsets16 = true;
+static bool g_sell_dia = true;   // the uniform-offset form where it applies
 
 // 16-bit offsets of the sliced layout's block columns from their block rows (bcd zeroed
 // beforehand: padding entries and the lanes past the last block row keep 0); ovf is set when an
@@ -725,6 +727,64 @@ __global__ void k_sell_fill(int nb, int R, int C, const int* __restrict__ r, con
   }
 }
 
+// Distinct block-column offsets (block column - block row) of the R x C blocks of a CSR matrix,
+// one row per thread: each workgroup collects its offsets in an LDS set of kDiaMax slots, then
+// merges the set into the global table (kDiaMax slots, INT_MIN = free); ovf is set when either
+// overflows.
+__device__ __forceinline__ bool dia_insert(int* set, int d) {
+  for (int q = 0; q < kDiaMax; ++q) {
+    const int v = __atomic_load_n(&set[q], __ATOMIC_RELAXED);
+    if (v == d) return true;
+    if (v == INT_MIN) {
+      const int old = atomicCAS(&set[q], INT_MIN, d);
+      if (old == INT_MIN || old == d) return true;
+    }
+  }
+  return false;
+}
+__global__ __launch_bounds__(256) void k_dia_offsets(int n, int R, int C, const int* __restrict__ r,
+                                                     const int* __restrict__ c, int* __restrict__ table,
+                                                     int* __restrict__ ovf) {
+  __shared__ int set[kDiaMax];
+  __shared__ int bad;
+  if (threadIdx.x < kDiaMax) set[threadIdx.x] = INT_MIN;
+  if (threadIdx.x == 0) bad = 0;
+  __syncthreads();
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) {
+    const int I = i / R;
+    for (int j = r[i]; j < r[i + 1]; ++j)
+      if (!dia_insert(set, c[j] / C - I)) {
+        bad = 1;
+        break;
+      }
+  }
+  __syncthreads();
+  if (threadIdx.x < kDiaMax) {
+    const int d = set[threadIdx.x];
+    if (d != INT_MIN && !dia_insert(table, d)) atomicOr(ovf, 1);
+  }
+  if (threadIdx.x == 0 && bad) atomicOr(ovf, 1);
+}
+
+// uniform-offset sliced layout: the values of block row I's block at block column I + doff[k] at
+// entry soff[I/64] + 64*k + I%64 (bv zeroed beforehand)
+template <class VT>
+__global__ void k_dia_fill(int nb, int R, int C, int noff, const int* __restrict__ doff, const int* __restrict__ r,
+                           const int* __restrict__ c, const double* __restrict__ v, const int* __restrict__ soff,
+                           size_t T, VT* __restrict__ bv) {
+  const int I = blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= nb) return;
+  const size_t e0 = (size_t)soff[I / kSellRows] + (I % kSellRows);
+  for (int q = 0; q < R; ++q)
+    for (int j = r[I * R + q]; j < r[I * R + q + 1]; ++j) {
+      const int J = c[j] / C, d = J - I;
+      int k = 0;
+      while (k < noff - 1 && doff[k] != d) ++k;
+      bv[sell_vidx(C, T, q, c[j] - J * C, e0 + (size_t)k * kSellRows)] = (VT)v[j];
+    }
+}
+
 // *out = max(*out, longest row)
 __global__ __launch_bounds__(256) void k_max_row(int n, const int* __restrict__ ptr, int* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -748,6 +808,34 @@ __global__ void k_chunks_cap(int n, const int* __restrict__ ptr, int nchunks, in
 }
 }  // namespace deco
 
+namespace deco {
+// The sorted block-column offsets of the R x C blocks of A when there are at most kDiaMax of them
+// and the uniform-offset layout (nb block rows in slices of 64, noff entries each) stores at most
+// (1 + padding) x nnzb blocks; else empty.
+static std::vector<int> dia_offsets(const deco_csr* A, int R, int C, int nb, int64_t nnzb) {
+  std::vector<int> off;
+  if (!g_sell_dia || g_sell_padding < 0.0 || nb <= 0 || nnzb <= 0) return off;
+  DBuf<int> table(kDiaMax), ovf(1);
+  std::vector<int> h(kDiaMax, INT_MIN);
+  table.upload(h.data(), kDiaMax);
+  ovf.zero();
+  hipLaunchKernelGGL(k_dia_offsets, dim3(grid_for(A->n, 256)), dim3(256), 0, stream(), A->n, R, C, A->r.p, A->c.p,
+                     table.p, ovf.p);
+  DECO_CHECK_LAUNCH();
+  int o = 0;
+  copy_d2h(&o, ovf.p, 1);
+  if (o) return off;
+  copy_d2h(h.data(), table.p, kDiaMax);
+  for (int d : h)
+    if (d != INT_MIN) off.push_back(d);
+  std::sort(off.begin(), off.end());
+  const int64_t ns = (nb + kSellRows - 1) / kSellRows;
+  const int64_t T = ns * kSellRows * (int64_t)off.size();
+  if (off.empty() || T > INT_MAX || (double)T > (double)nnzb * (1.0 + g_sell_padding)) off.clear();
+  return off;
+}
+}  // namespace deco
+
 const deco_bsr* deco_csr::block32(int R, int C) const {
   using namespace deco;
   if (b32 && b32->R == R && b32->C == C) return b32;
@@ -768,6 +856,27 @@ const deco_bsr* deco_csr::block32(int R, int C) const {
   }
   const int64_t nnzb = scan_to_ptr(cnt.p, B->br.p, nb);
   B->nnzb = (int)nnzb;
+  const std::vector<int> doff = m >= C ? dia_offsets(this, R, C, nb, nnzb) : std::vector<int>();
+  if (!doff.empty()) {   // uniform-offset sliced layout
+    const int ns = (nb + kSellRows - 1) / kSellRows, no = (int)doff.size();
+    std::vector<int> hoff(ns + 1), hwid(ns, no);
+    for (int sl = 0; sl <= ns; ++sl) hoff[sl] = sl * kSellRows * no;
+    B->sell = B->dia = true;
+    B->noff = no;
+    B->doff.upload(doff.data(), no);
+    B->nslices = ns;
+    B->stot = (size_t)ns * kSellRows * no;
+    B->soff.upload(hoff.data(), ns + 1);
+    B->swid.upload(hwid.data(), ns);
+    B->bv.alloc(B->stot * R * C);
+    DECO_HIP(hipMemsetAsync(B->bv.p, 0, sizeof(float) * B->stot * R * C, stream()));
+    hipLaunchKernelGGL(k_dia_fill<float>, dim3(grid_for(nb, 256)), dim3(256), 0, stream(), nb, R, C, no,
+                       (const int*)B->doff.p, r.p, c.p, v.p, (const int*)B->soff.p, B->stot, B->bv.p);
+    DECO_CHECK_LAUNCH();
+    sync();   // the host arrays go out of scope here
+    b32 = B;
+    return B;
+  }
   if (g_sell_padding >= 0.0 && nb > 0 && m >= C) {   // sliced layout when its padding is small
     const int ns = (nb + kSellRows - 1) / kSellRows;
     DBuf<int> ent(ns + 1);
@@ -864,6 +973,7 @@ bool deco_csr::sell_scalar() const {
       DECO_CHECK_LAUNCH();
       const int64_t T = scan_to_ptr(ent.p, off.p, ns);
       sell_ok = (T <= INT_MAX && (double)T <= (double)nnz * (1.0 + g_sell_padding)) ? 1 : 0;
+      if (!sell_ok && !dia_offsets(this, 1, 1, n, nnz).empty()) sell_ok = 1;
     }
   }
   return sell_ok == 1;
@@ -874,6 +984,16 @@ const deco_bsr* deco_csr::block64(int R, int C) const {
   const deco_bsr* B0 = block32(R, C);
   if (!B0 || B0->has64) return B0;
   deco_bsr* B = b32;
+  if (B->dia) {
+    const size_t nv = B->stot * R * C;
+    B->bv64.alloc(nv);
+    DECO_HIP(hipMemsetAsync(B->bv64.p, 0, sizeof(double) * nv, stream()));
+    hipLaunchKernelGGL(k_dia_fill<double>, dim3(grid_for(B->nb, 256)), dim3(256), 0, stream(), B->nb, R, C, B->noff,
+                       (const int*)B->doff.p, r.p, c.p, v.p, (const int*)B->soff.p, B->stot, B->bv64.p);
+    DECO_CHECK_LAUNCH();
+    B->has64 = true;
+    return B;
+  }
   if (B->sell) {
     const size_t nv = std::max<size_t>(B->stot, 1) * R * C;
     B->bv64.alloc(nv);
@@ -1307,7 +1427,7 @@ int deco_block_info(deco_csr* A, int R, int C, int* sliced, int* nnzb, long long
     if (!A) fail(DECO_ERR_ARG, "null CSR");
     const deco_bsr* B = A->block32(R, C);
     if (!B) fail(DECO_ERR_ARG, "%d x %d blocks do not tile a %d x %d matrix", R, C, A->n, A->m);
-    if (sliced) *sliced = B->sell ? (B->d16 ? 2 : 1) : 0;
+    if (sliced) *sliced = B->dia ? 3 : B->sell ? (B->d16 ? 2 : 1) : 0;
     if (nnzb) *nnzb = B->nnzb;
     if (entries) *entries = B->sell ? (long long)B->stot : (long long)B->nnzb;
   });
@@ -1322,6 +1442,10 @@ int deco_set_sell_padding(double max_fraction) {
 
 int deco_set_sell_offsets16(int enable) {
   return guard([&] { g_sell_offsets16 = enable != 0; });
+}
+
+int deco_set_sell_dia(int enable) {
+  return guard([&] { g_sell_dia = enable != 0; });
 }
 
 int deco_read_bandwidth(size_t bytes, int iters, double* gbps) {

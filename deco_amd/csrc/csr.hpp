@@ -31,6 +31,14 @@ struct deco_bsr {
   // rows do not outnumber the block columns (so an offset 0 stays inside x)
   deco::DBuf<short> bcd;
   bool d16 = false;
+  // uniform-offset form of the sliced layout (stencil matrices): the k-th entry of every block row
+  // I is the block at block column I + doff[k] (noff sorted offsets shared by all block rows; zero
+  // blocks where a row has none, read at a column clamped into x), so no column is stored and
+  // every slice has width noff. Chosen before the plain sliced layout when the matrix has at most
+  // kDiaMax distinct offsets and the zero blocks stay within the padding threshold.
+  bool dia = false;
+  int noff = 0;
+  deco::DBuf<int> doff;
 };
 
 struct deco_csr {
@@ -596,6 +604,10 @@ __device__ __forceinline__ BsrVals<VT, R * C> sell_load(const VT* __restrict__ b
   return o;
 }
 
+// column tag of the uniform-offset sliced layout (deco_bsr::dia): no stored columns
+struct DiaCols {};
+constexpr int kDiaMax = 16;   // largest offset set of the uniform-offset layout
+
 // block column of sliced entry e: stored (int) or as an offset from the lane's block row base
 // (short; base 0 for the lanes past the last block row, whose entries hold offset 0)
 template <class CT>
@@ -613,11 +625,14 @@ __device__ __forceinline__ int sell_col(const CT* __restrict__ bc, size_t e, int
 // re-reads the row's last entry with zero values, so a slice of width w takes ceil(w/KU)
 // dependent rounds. FP64 batches of 4 keep a remainder loop of single entries (measured: clamped
 // or larger batches cost those products more in registers than they gain).
+// CT DiaCols: the uniform-offset layout, entry k of block row I at block column I + doff[k]
+// (clamped into the mb block columns; its values are zero where the row has no such block).
 template <class VT, int R, int C, class XT, class E, class CT, int KU>
 __global__ __launch_bounds__(256) void k_sell_stream(int nb, int nslices, const int* __restrict__ soff,
                                                     const int* __restrict__ swid, const CT* __restrict__ bc,
                                                     const VT* __restrict__ bv, size_t T, const XT* __restrict__ x,
-                                                    E epi) {
+                                                    E epi, const int* __restrict__ doff, int mb) {
+  constexpr bool DIA = std::is_same<CT, DiaCols>::value;
   using AT = acc_t<VT, XT>;
   const int sl = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
   if (sl >= nslices) return;
@@ -642,8 +657,10 @@ __global__ __launch_bounds__(256) void k_sell_stream(int nb, int nslices, const 
     XT xv[KU][C];
 #pragma unroll
     for (int u = 0; u < KU; ++u) {
-      const size_t e = (size_t)o + (size_t)min(k + u, w - 1) * kSellRows + lane;
-      cc[u] = sell_col(bc, e, base);
+      const int kk = min(k + u, w - 1);
+      const size_t e = (size_t)o + (size_t)kk * kSellRows + lane;
+      if constexpr (DIA) cc[u] = min(max(base + doff[kk], 0), mb - 1);
+      else cc[u] = sell_col(bc, e, base);
       vv[u] = sell_load<VT, R, C>(bv, e, T);
       if (clamped && k + u >= w) {
 #pragma unroll
@@ -666,7 +683,10 @@ __global__ __launch_bounds__(256) void k_sell_stream(int nb, int nslices, const 
     const size_t e = (size_t)o + (size_t)k * kSellRows + lane;
     const BsrVals<VT, R * C> vv = sell_load<VT, R, C>(bv, e, T);
     XT xv[C];
-    bsr_gather<C>(x, sell_col(bc, e, base), xv);
+    int col;
+    if constexpr (DIA) col = min(max(base + doff[k], 0), mb - 1);
+    else col = sell_col(bc, e, base);
+    bsr_gather<C>(x, col, xv);
 #pragma unroll
     for (int rr = 0; rr < R; ++rr) {
       AT p = (AT)vv.a[rr * C] * (AT)xv[0];
@@ -690,12 +710,18 @@ void bsr_stream_v(const deco_bsr* B, const VT* bv, const XT* x, E epi) {
     const bool narrow = B->stot <= (size_t)4 * kSellRows * B->nslices;  // at most 4 on average
 #define DECO_SELL_LAUNCH_KU(RR, CC, KU)                                                                          \
   do {                                                                                                           \
-    if (B->d16)                                                                                                  \
+    if (B->dia)                                                                                                  \
+      hipLaunchKernelGGL((k_sell_stream<VT, RR, CC, XT, E, DiaCols, KU>), dim3(grid), dim3(256), 0, stream(),     \
+                         B->nb, B->nslices, B->soff.p, B->swid.p, (const DiaCols*)nullptr, bv, B->stot, x, epi,   \
+                         (const int*)B->doff.p, B->mb);                                                          \
+    else if (B->d16)                                                                                             \
       hipLaunchKernelGGL((k_sell_stream<VT, RR, CC, XT, E, short, KU>), dim3(grid), dim3(256), 0, stream(), B->nb, \
-                         B->nslices, B->soff.p, B->swid.p, (const short*)B->bcd.p, bv, B->stot, x, epi);          \
+                         B->nslices, B->soff.p, B->swid.p, (const short*)B->bcd.p, bv, B->stot, x, epi,           \
+                         (const int*)nullptr, B->mb);                                                            \
     else                                                                                                         \
       hipLaunchKernelGGL((k_sell_stream<VT, RR, CC, XT, E, int, KU>), dim3(grid), dim3(256), 0, stream(), B->nb,   \
-                         B->nslices, B->soff.p, B->swid.p, (const int*)B->bc.p, bv, B->stot, x, epi);             \
+                         B->nslices, B->soff.p, B->swid.p, (const int*)B->bc.p, bv, B->stot, x, epi,              \
+                         (const int*)nullptr, B->mb);                                                            \
   } while (0)
 #define DECO_SELL_LAUNCH(RR, CC)                                                        \
   do {                                                                                  \
@@ -717,7 +743,7 @@ void bsr_stream_v(const deco_bsr* B, const VT* bv, const XT* x, E epi) {
 #undef DECO_SELL_LAUNCH
 #undef DECO_SELL_LAUNCH_KU
     DECO_CHECK_LAUNCH();
-    count_bytes((double)B->nnzb * (B->R * B->C * sizeof(VT) + (B->d16 ? 2 : 4)) + 8.0 * B->nslices +
+    count_bytes((double)B->nnzb * (B->R * B->C * sizeof(VT) + (B->dia ? 0 : B->d16 ? 2 : 4)) + 8.0 * B->nslices +
                 (double)sizeof(XT) * B->mb * B->C + (double)B->nb * B->R * epi.row_bytes());
     return;
   }

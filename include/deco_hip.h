@@ -226,9 +226,14 @@ int deco_set_sell_padding(double max_fraction);
  * AMG levels read their columns as 16-bit offsets from each row chunk's smallest column (taken
  * when every chunk spans at most 65535 columns). No reference counterpart. */
 int deco_set_sell_offsets16(int enable);
+/* Whether block shadows made from now on take the uniform-offset form of the sliced layout when it
+ * applies (default 1): at most 16 distinct block-column offsets (block column - block row, as on
+ * stencil grids) and zero blocks within the padding threshold; entry k of every block row is the
+ * block at its offset k, so no column is stored. No reference counterpart. */
+int deco_set_sell_dia(int enable);
 /* The R x C block shadow of A (created if needed): its layout (0 chunked, 1 sliced, 2 sliced with
- * 16-bit block-column offsets), its number of blocks, and the blocks it stores (padding
- * included). No reference counterpart. */
+ * 16-bit block-column offsets, 3 sliced with uniform offsets), its number of blocks, and the
+ * blocks it stores (padding included). No reference counterpart. */
 int deco_block_info(deco_csr* A, int R, int C, int* sliced, int* nnzb, long long* entries);
 /* deco_spmv_time for the block-shadow product (precision as there); nnzb (may be NULL)
  * receives the number of R x C blocks. */

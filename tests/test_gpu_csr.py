@@ -144,23 +144,28 @@ def test_block_reorder_bitexact(gpu, cfg, bs, ro, co):
         A.block_reorder(bs, "1" * bs, co)
 
 
-@pytest.mark.parametrize("pad,off16", [(10.0, True), (10.0, False), (-1.0, True)])
+@pytest.mark.parametrize("pad,off16,dia", [(10.0, True, False), (10.0, False, False), (-1.0, True, False),
+                                           (0.10, True, True)])
 @pytest.mark.parametrize("cfg,shape", [(1, (2, 2)), (2, (3, 3)), (1, "prow"), (1, "tril"), (2, "tril")])
-def test_block_f32_product(gpu, cfg, shape, pad, off16):
+def test_block_f32_product(gpu, cfg, shape, pad, off16, dia):
     """Block-shadow products against the CSR product: full cell blocks, the pressure rows
     (1 x bs blocks), and lower-triangular patterns whose diagonal blocks are partly empty (the
     FSAI factor shape); in the sliced layout (pad 10: chosen whatever its padding) with 16-bit
-    block-column offsets and with 32-bit columns, and in the chunked one (pad -1)."""
+    block-column offsets and with 32-bit columns, in the chunked one (pad -1), and with the
+    uniform-offset form allowed (taken by the full 32x32x16 grid's stencil shapes, not by the
+    black-oil system, whose fault links give more than 16 offsets)."""
     deco_amd.set_sell_padding(pad)
     deco_amd.set_sell_offsets16(off16)
+    deco_amd.set_sell_dia(dia)
     try:
-        _block_products(cfg, shape, 0 if pad < 0 else (2 if off16 else 1))
+        _block_products(cfg, shape, None if dia else (0 if pad < 0 else (2 if off16 else 1)))
     finally:
         deco_amd.set_sell_padding(0.10)
         deco_amd.set_sell_offsets16(True)
+        deco_amd.set_sell_dia(True)
 
 
-def _block_products(cfg, shape, layout):
+def _block_shape(cfg, shape):
     g = gen_system(cfg)
     bs = g["bs"]
     S = to_mat(g).to_scipy()
@@ -173,12 +178,34 @@ def _block_products(cfg, shape, layout):
     else:
         R, C = shape
     S.sort_indices()
+    return S, R, C
+
+
+def _uniform_offsets_apply(S, R, C, pad=0.10):
+    """Whether the uniform-offset layout applies: at most 16 distinct block-column offsets and
+    64-row slices of that width storing at most (1 + pad) times the blocks."""
+    rows = np.repeat(np.arange(S.shape[0]), np.diff(S.indptr)) // R
+    cols = S.indices // C
+    offs = np.unique(cols - rows)
+    nnzb = len(np.unique(rows.astype(np.int64) * (S.shape[1] // C + 1) + cols))
+    nb = S.shape[0] // R
+    return len(offs) <= 16 and ((nb + 63) // 64) * 64 * len(offs) <= nnzb * (1 + pad)
+
+
+def _block_products(cfg, shape, layout):
+    S, R, C = _block_shape(cfg, shape)
     A = deco_amd.CSR.from_scipy(S)
-    if layout == 2:   # 16-bit offsets only where every block column lies within reach
-        rows = np.repeat(np.arange(S.shape[0]), np.diff(S.indptr)) // R
-        fits = np.abs(S.indices // C - rows).max() <= 32767 and S.shape[0] // R <= S.shape[1] // C
-        layout = 2 if fits else 1
-    assert A.block_info(R, C)[0] == layout
+    if layout is None:
+        if _uniform_offsets_apply(S, R, C):
+            assert A.block_info(R, C)[0] == 3
+        else:
+            assert A.block_info(R, C)[0] != 3
+    else:
+        if layout == 2:   # 16-bit offsets only where every block column lies within reach
+            rows = np.repeat(np.arange(S.shape[0]), np.diff(S.indptr)) // R
+            fits = np.abs(S.indices // C - rows).max() <= 32767 and S.shape[0] // R <= S.shape[1] // C
+            layout = 2 if fits else 1
+        assert A.block_info(R, C)[0] == layout
     x = np.random.default_rng(3).uniform(-1, 1, S.shape[1]).astype(np.float32)
     y = A.apply_f32(x, block=(R, C)).numpy().astype(np.float64)
     ref = S @ x.astype(np.float64)
@@ -194,6 +221,28 @@ def _block_products(cfg, shape, layout):
     assert rel(A.apply(xd, block=(R, C)).numpy(), yo) <= 1e-14
     with pytest.raises(deco_amd._lib.DecoError):
         A.apply(xd, block=(R, 5))
+
+
+@pytest.mark.parametrize("cfg,shape", [(1, (2, 2)), (1, "prow"), (1, "tril"), (11, (2, 2))])
+def test_uniform_offset_layout_equals_sliced(gpu, cfg, shape):
+    """The uniform-offset layout sums each block row's blocks in the same column order as the
+    sliced one, its zero blocks adding nothing: FP64 and FP32 products equal the sliced layout's
+    bit for bit; the FP64 one agrees with the oracle to 1e-14."""
+    S, R, C = _block_shape(cfg, shape)
+    assert _uniform_offsets_apply(S, R, C)
+    rng = np.random.default_rng(23)
+    x = rng.uniform(-1, 1, S.shape[1])
+    ys = {}
+    for dia in (True, False):
+        deco_amd.set_sell_dia(dia)
+        try:
+            A = deco_amd.CSR.from_scipy(S)
+            assert A.block_info(R, C)[0] == (3 if dia else 2)
+            ys[dia] = (A.apply(x, block=(R, C)).numpy(), A.apply_f32(x.astype(np.float32), block=(R, C)).numpy())
+        finally:
+            deco_amd.set_sell_dia(True)
+    assert np.array_equal(ys[True][0], ys[False][0]) and np.array_equal(ys[True][1], ys[False][1])
+    assert rel(ys[True][0], oracle.spmv(oracle.Mat.from_scipy(S), x)) <= 1e-14
 
 
 def test_spmv_wide_column_offsets(gpu):
@@ -238,7 +287,13 @@ def test_sliced_offsets_fall_back_to_32bit_columns(gpu):
     near.eliminate_zeros()
     near = (near + sp.csr_matrix(([1.0], ([n - 1], [n - 1])), shape=(n, n))).tocsr()
     near.sort_indices()
-    assert deco_amd.CSR.from_scipy(near).block_info(1, 1)[0] == 2
+    # two offsets (0, 1): the uniform-offset form, or with it disabled 16-bit offsets
+    assert deco_amd.CSR.from_scipy(near).block_info(1, 1)[0] == 3
+    deco_amd.set_sell_dia(False)
+    try:
+        assert deco_amd.CSR.from_scipy(near).block_info(1, 1)[0] == 2
+    finally:
+        deco_amd.set_sell_dia(True)
     # 600 block rows over 300 block columns (sliced whatever its padding)
     W = (sp.random(600, 300, density=0.02, random_state=rng, format="csr") + sp.eye(600, 300, format="csr")).tocsr()
     W.sort_indices()
