@@ -649,7 +649,7 @@ __global__ __launch_bounds__(256) void k_sell_stream(int nb, int nslices, const 
   AT acc[R];
 #pragma unroll
   for (int rr = 0; rr < R; ++rr) acc[rr] = AT(0);
-  constexpr bool clamped = (sizeof(VT) == 4) || KU == 7;
+  constexpr bool clamped = (sizeof(VT) == 4) || KU == 7 || DIA;
   int k = 0;
   for (; clamped ? k < w : k + KU <= w; k += KU) {
     int cc[KU];
