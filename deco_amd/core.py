@@ -601,6 +601,12 @@ def set_sell_offsets16(enable):
     check(lib().deco_set_sell_offsets16(1 if enable else 0))
 
 
+def set_stream_pieces(enable):
+    """Stream products of matrices without a block shadow made from now on use the row-piece
+    sliced form where it fits (default), else the CSR-stream kernel."""
+    check(lib().deco_set_stream_pieces(1 if enable else 0))
+
+
 def set_sell_dia(enable):
     """Block shadows made from now on take the uniform-offset sliced layout (no stored columns)
     where the matrix has at most 16 distinct block-column offsets within the padding threshold

@@ -610,6 +610,7 @@ void to_f32(const double* a, float* b, int64_t n) {
 deco_csr::~deco_csr() {
   delete At;
   delete b32;
+  delete pc;
 }
 
 void deco_csr::invalidate() {
@@ -619,6 +620,9 @@ void deco_csr::invalidate() {
   c16_ok = -1;
   delete b32;
   b32 = nullptr;
+  delete pc;
+  pc = nullptr;
+  pc_ok = -1;
 }
 
 namespace deco {
@@ -1088,6 +1092,168 @@ __global__ __launch_bounds__(256) void k_chunk_c16(const int2* __restrict__ desc
 }
 }  // namespace deco
 
+namespace deco {
+static bool g_stream_pieces = true;   // the row-piece form for the stream products where it fits
+// Largest matrix (entries) that takes the row-piece form. Its products have a shorter dependency
+// chain than the CSR-stream kernel's (no LDS staging or barrier, no row-range load before the
+// sums), which pays where a product is latency-bound: measured per AMG operator of the
+// 168x100x120 hierarchy inside the solves, 1-5 us faster per launch up to 2.7M entries, equal or
+// slower above (level 1 and 2 operators, the level-0 interpolation).
+constexpr int kPieceMaxNnz = 2800000;
+
+// per row: its first and last column into the smallest / largest column of its slice (rows sorted)
+__global__ void k_piece_span(int n, const int* __restrict__ r, const int* __restrict__ c,
+                             const int* __restrict__ place, int* __restrict__ smin, int* __restrict__ smax) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || r[i] == r[i + 1]) return;
+  const int sl = place[i] >> 6;
+  atomicMin(&smin[sl], c[r[i]]);
+  atomicMax(&smax[sl], c[r[i + 1] - 1]);
+}
+
+// per row: its pieces at lanes place & 63 .. of slice place >> 6 (structure: the lanes' row
+// offsets and the slots' columns too, padding slots at the row's last column, or the slice base)
+template <class VT>
+__global__ void k_piece_fill(int n, int W, const int* __restrict__ r, const int* __restrict__ c,
+                             const double* __restrict__ v, const int* __restrict__ place,
+                             const int* __restrict__ srow, const int* __restrict__ cbase,
+                             unsigned char* __restrict__ pdel, unsigned short* __restrict__ c16,
+                             int* __restrict__ c32, VT* __restrict__ vals, bool structure) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = r[i], e = r[i + 1];
+  const int np = max(1, (e - b + W - 1) / W);
+  const int sl = place[i] >> 6, l0 = place[i] & 63;
+  const int last = e > b ? c[e - 1] : (cbase ? cbase[sl] : 0);
+  for (int p = 0; p < np; ++p) {
+    if (structure) pdel[(size_t)sl * 64 + l0 + p] = (unsigned char)(i - srow[sl]);
+    for (int t = 0; t < W; ++t) {
+      const int j = b + p * W + t;
+      const size_t slot = ((size_t)sl * W + t) * 64 + l0 + p;
+      if (structure) {
+        const int cj = j < e ? c[j] : last;
+        if (cbase) c16[slot] = (unsigned short)(cj - cbase[sl]);
+        else c32[slot] = cj;
+      }
+      vals[slot] = j < e ? (VT)v[j] : VT(0);
+    }
+  }
+}
+}  // namespace deco
+
+const deco_pieces* deco_csr::pieces(bool fp64) const {
+  using namespace deco;
+  if (pc_ok < 0) {
+    pc_ok = 0;
+    if (g_stream_pieces && n > 0 && nnz > 0 && nnz <= kPieceMaxNnz) {
+      std::vector<int> hr(n + 1);
+      copy_d2h(hr.data(), r.p, n + 1);
+      int maxlen = 0;
+      for (int i = 0; i < n; ++i) maxlen = std::max(maxlen, hr[i + 1] - hr[i]);
+      // the piece width with the fewest bytes (values and 2-byte columns per slot, a row byte per
+      // lane) among those that keep every row within one slice
+      int bestW = 0;
+      int64_t best_ns = 0;
+      double best_cost = 0.0;
+      for (int W : {1, 2, 4, 8}) {
+        if ((maxlen + W - 1) / W > 64) continue;
+        int64_t ns = 0;
+        int lane = 64;
+        for (int i = 0; i < n; ++i) {
+          const int np = std::max(1, (hr[i + 1] - hr[i] + W - 1) / W);
+          if (lane + np > 64) { ++ns; lane = 0; }
+          lane += np;
+        }
+        const double cost = (double)ns * 64 * W * 6.0 + (double)ns * 64;
+        if (!bestW || cost < best_cost) { bestW = W; best_ns = ns; best_cost = cost; }
+      }
+      if (bestW && (double)best_ns * 64 * bestW <= 1.3 * nnz && best_ns * 64 * bestW <= INT_MAX) {
+        auto* P = new deco_pieces();
+        const int W = bestW, ns = (int)best_ns;
+        P->W = W;
+        P->nslices = ns;
+        P->nslot = (size_t)ns * 64 * W;
+        std::vector<int> place(n), srow(ns);
+        int sl = -1, lane = 64;
+        for (int i = 0; i < n; ++i) {
+          const int np = std::max(1, (hr[i + 1] - hr[i] + W - 1) / W);
+          if (lane + np > 64) { ++sl; lane = 0; srow[sl] = i; }
+          place[i] = sl * 64 + lane;
+          lane += np;
+        }
+        DBuf<int> dplace;
+        dplace.upload(place.data(), n);
+        P->srow.upload(srow.data(), ns);
+        std::vector<int> smin(ns, INT_MAX), smax(ns, INT_MIN);
+        DBuf<int> dmin, dmax;
+        dmin.upload(smin.data(), ns);
+        dmax.upload(smax.data(), ns);
+        hipLaunchKernelGGL(k_piece_span, dim3(grid_for(n, 256)), dim3(256), 0, stream(), n, r.p, c.p,
+                           (const int*)dplace.p, dmin.p, dmax.p);
+        DECO_CHECK_LAUNCH();
+        copy_d2h(smin.data(), dmin.p, ns);
+        copy_d2h(smax.data(), dmax.p, ns);
+        bool fits = g_sell_offsets16;
+        for (int q = 0; q < ns && fits; ++q) {
+          if (smin[q] == INT_MAX) smin[q] = 0;
+          else if ((int64_t)smax[q] - smin[q] > 65535) fits = false;
+        }
+        for (int q = 0; q < ns; ++q)
+          if (smin[q] == INT_MAX) smin[q] = 0;
+        P->d16 = fits;
+        P->pdel.alloc((size_t)ns * 64);
+        DECO_HIP(hipMemsetAsync(P->pdel.p, 0xFF, (size_t)ns * 64, stream()));
+        // padding lanes keep offset / column 0 (a valid column) and value 0
+        if (fits) {
+          P->cbase.upload(smin.data(), ns);
+          P->c16.alloc(P->nslot);
+          DECO_HIP(hipMemsetAsync(P->c16.p, 0, sizeof(unsigned short) * P->nslot, stream()));
+        } else {
+          P->c32.alloc(P->nslot);
+          DECO_HIP(hipMemsetAsync(P->c32.p, 0, sizeof(int) * P->nslot, stream()));
+        }
+        P->vf.alloc(P->nslot);
+        DECO_HIP(hipMemsetAsync(P->vf.p, 0, sizeof(float) * P->nslot, stream()));
+        hipLaunchKernelGGL(k_piece_fill<float>, dim3(grid_for(n, 256)), dim3(256), 0, stream(), n, W, r.p, c.p,
+                           v.p, (const int*)dplace.p, (const int*)P->srow.p,
+                           fits ? (const int*)P->cbase.p : (const int*)nullptr, P->pdel.p, P->c16.p, P->c32.p,
+                           P->vf.p, true);
+        DECO_CHECK_LAUNCH();
+        sync();   // the host arrays go out of scope here
+        pc = P;
+        pc_ok = 1;
+      }
+    }
+  }
+  if (pc_ok != 1) return nullptr;
+  if (fp64 && !pc->has64) {
+    // the FP64 values in the same slots: the placement is recomputed from the row pointers
+    std::vector<int> hr(n + 1);
+    copy_d2h(hr.data(), r.p, n + 1);
+    const int W = pc->W;
+    std::vector<int> place(n);
+    int sl = -1, lane = 64;
+    for (int i = 0; i < n; ++i) {
+      const int np = std::max(1, (hr[i + 1] - hr[i] + W - 1) / W);
+      if (lane + np > 64) { ++sl; lane = 0; }
+      place[i] = sl * 64 + lane;
+      lane += np;
+    }
+    DBuf<int> dplace;
+    dplace.upload(place.data(), n);
+    pc->v64.alloc(pc->nslot);
+    DECO_HIP(hipMemsetAsync(pc->v64.p, 0, sizeof(double) * pc->nslot, stream()));
+    hipLaunchKernelGGL(k_piece_fill<double>, dim3(grid_for(n, 256)), dim3(256), 0, stream(), n, W, r.p, c.p, v.p,
+                       (const int*)dplace.p, (const int*)pc->srow.p,
+                       pc->d16 ? (const int*)pc->cbase.p : (const int*)nullptr, pc->pdel.p, pc->c16.p, pc->c32.p,
+                       pc->v64.p, false);
+    DECO_CHECK_LAUNCH();
+    sync();
+    pc->has64 = true;
+  }
+  return pc;
+}
+
 bool deco_csr::cols16() const {
   using namespace deco;
   if (c16_ok < 0) {
@@ -1446,6 +1612,10 @@ int deco_set_sell_offsets16(int enable) {
 
 int deco_set_sell_dia(int enable) {
   return guard([&] { g_sell_dia = enable != 0; });
+}
+
+int deco_set_stream_pieces(int enable) {
+  return guard([&] { g_stream_pieces = enable != 0; });
 }
 
 int deco_read_bandwidth(size_t bytes, int iters, double* gbps) {

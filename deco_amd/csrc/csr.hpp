@@ -41,6 +41,27 @@ struct deco_bsr {
   deco::DBuf<int> doff;
 };
 
+// Row-piece sliced form of a CSR matrix for the stream products of the AMG levels: every row is
+// cut into pieces of W entries (the last one padded with zero values; an empty row gets one
+// padding piece), the pieces of consecutive rows fill slices of 64 (a row's pieces never cross a
+// slice: the slice is closed with padding lanes first), and slot t of lane l of slice s holds
+// entry t of that lane's piece at (s * W + t) * 64 + l. A wavefront takes a slice: each lane sums
+// its piece, the lanes of one row add their sums in a segmented shuffle tree, and the row's first
+// lane applies the epilogue. No LDS, no barrier, one lane per W entries.
+struct deco_pieces {
+  int W = 1;                         // entries per piece: 1, 2, 4 or 8
+  int nslices = 0;
+  size_t nslot = 0;                  // nslices * 64 * W
+  deco::DBuf<int> srow;              // per slice: the row of its first piece
+  deco::DBuf<unsigned char> pdel;    // per lane: its row - srow (255: padding lane)
+  deco::DBuf<int> cbase;             // per slice: its smallest column (16-bit columns)
+  deco::DBuf<unsigned short> c16;    // per slot: column - cbase
+  deco::DBuf<int> c32;               // per slot: column, when a slice spans more than 65535 columns
+  deco::DBuf<float> vf;              // per slot: FP32 value
+  deco::DBuf<double> v64;            // per slot: FP64 value (filled on first FP64 use)
+  bool d16 = false, has64 = false;
+};
+
 struct deco_csr {
   int n = 0, m = 0, nnz = 0;
   deco::DBuf<int> r, c;
@@ -77,6 +98,11 @@ struct deco_csr {
   mutable deco::DBuf<unsigned short> c16;
   mutable deco::DBuf<int> cbase;
   mutable signed char c16_ok = -1;
+  // the row-piece sliced form (deco_pieces), built on first call when its slots stay within 1.3
+  // times the entries for some W, else null; fp64: with its FP64 values. Dropped by invalidate().
+  const deco_pieces* pieces(bool fp64) const;
+  mutable deco_pieces* pc = nullptr;
+  mutable signed char pc_ok = -1;
   void invalidate();
 };
 
@@ -363,6 +389,84 @@ __global__ __launch_bounds__(stream_block<VT>()) void k_spmv_stream(const int2* 
   }
 }
 
+// Row-piece product (deco_pieces): one wavefront per slice of 64 pieces, W entries per lane with
+// all their loads in flight before the gathers; a row's sum is its pieces' sums in a fixed
+// segmented tree (lane l adds lane l + 2^k's partial while both hold the same row), so it is
+// deterministic; a row of one piece sums its entries in column order.
+template <class VT, class XG, class E, class CT, int W>
+__global__ __launch_bounds__(256) void k_spmv_piece(int nslices, const int* __restrict__ srow,
+                                                    const unsigned char* __restrict__ pdel,
+                                                    const int* __restrict__ cbase, const CT* __restrict__ col,
+                                                    const VT* __restrict__ val, XG x, E epi) {
+  using XT = typename XG::type;
+  using AT = acc_t<VT, XT>;
+  const int s = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+  if (s >= nslices) return;
+  const int lane = threadIdx.x & 63;
+  const int rd = pdel[(size_t)s * 64 + lane];
+  const int r0 = srow[s];
+  int cb = 0;
+  if constexpr (!std::is_same<CT, int>::value) cb = cbase[s];
+  int cc[W];
+  VT vv[W];
+#pragma unroll
+  for (int t = 0; t < W; ++t) {
+    const size_t e = ((size_t)s * W + t) * 64 + lane;
+    cc[t] = cb + (int)__builtin_nontemporal_load(col + e);
+    vv[t] = __builtin_nontemporal_load(val + e);
+  }
+  const int prev = __shfl_up(rd, 1);
+  const bool live = rd != 255;
+  const bool lead = live && (lane == 0 || prev != rd);
+  const int row = r0 + rd;
+  typename E::Pre pre{};
+  if (lead) pre = epi.pre(row);
+  XT xx[W];
+#pragma unroll
+  for (int t = 0; t < W; ++t) xx[t] = x(cc[t]);
+  AT acc = AT(0);
+#pragma unroll
+  for (int t = 0; t < W; ++t) acc += (AT)vv[t] * (AT)xx[t];
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const AT o = __shfl_down(acc, off);
+    const int od = __shfl_down(rd, off);
+    if (lane + off < 64 && od == rd) acc += o;
+  }
+  if (lead) epi(row, acc, pre);
+}
+
+template <class VT, class XG, class E, class CT>
+void spmv_piece_w(const deco_pieces* P, const CT* col, const VT* val, XG x, E epi) {
+  const dim3 grid((P->nslices + 3) / 4), blk(256);
+  switch (P->W) {
+    case 1: hipLaunchKernelGGL((k_spmv_piece<VT, XG, E, CT, 1>), grid, blk, 0, stream(), P->nslices, P->srow.p,
+                               P->pdel.p, P->cbase.p, col, val, x, epi); break;
+    case 2: hipLaunchKernelGGL((k_spmv_piece<VT, XG, E, CT, 2>), grid, blk, 0, stream(), P->nslices, P->srow.p,
+                               P->pdel.p, P->cbase.p, col, val, x, epi); break;
+    case 4: hipLaunchKernelGGL((k_spmv_piece<VT, XG, E, CT, 4>), grid, blk, 0, stream(), P->nslices, P->srow.p,
+                               P->pdel.p, P->cbase.p, col, val, x, epi); break;
+    default: hipLaunchKernelGGL((k_spmv_piece<VT, XG, E, CT, 8>), grid, blk, 0, stream(), P->nslices, P->srow.p,
+                                P->pdel.p, P->cbase.p, col, val, x, epi); break;
+  }
+  DECO_CHECK_LAUNCH();
+}
+
+// the product on A's row-piece form when it has one (values of A in VT), true if launched
+template <class VT, class XG, class E>
+bool spmv_piece(const deco_csr* A, XG x, E epi) {
+  const deco_pieces* P = A->pieces(sizeof(VT) == 8);
+  if (!P) return false;
+  const VT* val;
+  if constexpr (sizeof(VT) == 8) val = P->v64.p;
+  else val = P->vf.p;
+  if (P->d16) spmv_piece_w(P, (const unsigned short*)P->c16.p, val, x, epi);
+  else spmv_piece_w(P, (const int*)P->c32.p, val, x, epi);
+  count_bytes((double)A->nnz * (sizeof(VT) + (P->d16 ? 2 : 4)) + 65.0 * P->nslices + 4.0 * P->nslices +
+              (double)sizeof(typename XG::type) * A->m + (double)A->n * epi.row_bytes());
+  return true;
+}
+
 template <class VT, class XG, class E>
 void spmv_stream_g(const deco_csr* A, const VT* v, XG x, E epi) {
   if (A->n <= 0) return;
@@ -378,6 +482,8 @@ void spmv_stream_g(const deco_csr* A, const VT* v, XG x, E epi) {
 template <class VT, class XG, class E>
 void spmv_stream_c16(const deco_csr* A, const VT* v, XG x, E epi) {
   if (A->n <= 0) return;
+  // A's own values: its row-piece form where it has one
+  if ((v == (const VT*)A->v.p || (A->vf_valid && v == (const VT*)A->vf.p)) && spmv_piece<VT>(A, x, epi)) return;
   if (!A->cols16()) {
     spmv_stream_g(A, v, x, epi);
     return;

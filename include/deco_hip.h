@@ -231,6 +231,13 @@ int deco_set_sell_offsets16(int enable);
  * stencil grids) and zero blocks within the padding threshold; entry k of every block row is the
  * block at its offset k, so no column is stored. No reference counterpart. */
 int deco_set_sell_dia(int enable);
+/* Whether the stream products of matrices without a block shadow (the AMG levels below the
+ * finest, their restriction and interpolation operators) made from now on use the row-piece
+ * sliced form (default 1): rows cut into pieces of 1, 2, 4 or 8 entries, 64 pieces per
+ * wavefront, a row's pieces summed in a segmented shuffle tree; taken for matrices of at most
+ * 2.8M entries (latency-bound products) when its slots stay within 1.3 times the entries, else
+ * the CSR-stream kernel. No reference counterpart. */
+int deco_set_stream_pieces(int enable);
 /* The R x C block shadow of A (created if needed): its layout (0 chunked, 1 sliced, 2 sliced with
  * 16-bit block-column offsets, 3 sliced with uniform offsets), its number of blocks, and the
  * blocks it stores (padding included). No reference counterpart. */

@@ -326,6 +326,34 @@ def test_amg_chunk_columns16_bit_identical(gpu):
     assert np.array_equal(out[True][0], out[False][0]) and np.array_equal(out[True][1], out[False][1])
 
 
+def test_amg_row_piece_products_match_stream(gpu):
+    """The AMG operators' row-piece products (rows cut into pieces of 1-8 entries, 64 per
+    wavefront, segmented sums) against the CSR-stream kernel on the same hierarchy: the V-cycle
+    agrees to 1e-13 in FP64 and to FP32 rounding in FP32, forward and transposed, and the
+    oracle's V-cycle to 1e-12."""
+    g = gen_system(1)
+    tree = {"amg_SoC_alpha": "0.5", "amg_coarse_type": "10", "amg_relax_type": "-1",
+            "amg_prol_trunc_fac": "1e-3", "amg_A_drop_tol": "1e-4", "amg_max_coarse": "50"}
+    x = np.random.default_rng(16).uniform(-1, 1, g["n"] // 2)
+    out = {}
+    for pieces in (True, False):
+        deco_amd.set_stream_pieces(pieces)
+        try:
+            Ap = dev_csr(g).local_matrix(2, 0)
+            G = deco_amd.AMG(tree, Ap)
+            y64, yt = G.apply(x).numpy(), G.apply_t(x).numpy()
+            G.set_precision(deco_amd.PREC_FP32)
+            out[pieces] = (y64, yt, G.apply(x).numpy())
+            if pieces:
+                G.set_precision(deco_amd.PREC_FP64)
+                Go, _, _ = oracle_amg(G, oracle.local_matrix(to_mat(g), 2, 0), tree)
+                assert rel(y64, Go.apply(x)) <= 1e-12 and rel(yt, Go.apply_t(x)) <= 1e-12
+        finally:
+            deco_amd.set_stream_pieces(True)
+    assert rel(out[True][0], out[False][0]) <= 1e-13 and rel(out[True][1], out[False][1]) <= 1e-13
+    assert rel(out[True][2], out[False][2]) <= 1e-5
+
+
 def test_amg_wide_chunk_span_falls_back(gpu):
     """Couplings 80,000 rows apart make the restriction's row chunks span more than 65535
     columns: those products keep 32-bit columns, and the V-cycle still matches the oracle's."""
