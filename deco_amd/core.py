@@ -607,6 +607,13 @@ def set_stream_pieces(enable):
     check(lib().deco_set_stream_pieces(1 if enable else 0))
 
 
+def set_stream_sorted(mode):
+    """When CSR-stream products of matrices without a block shadow read the column-sorted chunk
+    form (the same sums bit for bit, gathers of one wavefront on a few cache lines): 1 or True
+    by the library's rule (default), 2 wherever it applies, 0 or False never."""
+    check(lib().deco_set_stream_sorted(int(mode)))
+
+
 def set_sell_dia(enable):
     """Block shadows made from now on take the uniform-offset sliced layout (no stored columns)
     where the matrix has at most 16 distinct block-column offsets within the padding threshold

@@ -611,6 +611,7 @@ deco_csr::~deco_csr() {
   delete At;
   delete b32;
   delete pc;
+  delete cs;
 }
 
 void deco_csr::invalidate() {
@@ -623,6 +624,10 @@ void deco_csr::invalidate() {
   delete pc;
   pc = nullptr;
   pc_ok = -1;
+  delete cs;
+  cs = nullptr;
+  cs_ok = -1;
+  gather_seg = -1.0;
 }
 
 namespace deco {
@@ -1028,6 +1033,7 @@ const int2* deco_csr::row_chunks() const {
     // would let a chunk overflow the tile (rows up to half the tile); the window a chunk's range
     // can span is cap + longest - 1
     int cap = deco::kStreamCap, win = deco::kStreamTile;
+    chunks_fit = false;
     if (n > 0 && nnz > 0) {
       deco::DBuf<int> mx(1);
       mx.zero();
@@ -1038,6 +1044,7 @@ const int2* deco_csr::row_chunks() const {
       if (longest <= deco::kStreamTile / 2) {
         if (longest > deco::kStreamTile - deco::kStreamCap + 1) cap = deco::kStreamTile - longest + 1;
         win = cap + std::max(longest, 1) - 1;
+        chunks_fit = true;   // a chunk's last row starts below (k+1)*cap, so it ends within the window
       }
     }
     chunk_cap = cap;
@@ -1053,10 +1060,10 @@ const int2* deco_csr::row_chunks() const {
 }
 
 namespace deco {
-// per stream chunk: its smallest column (the base of its 16-bit offsets), and ovf set when its
-// columns span more than 65535
+// per stream chunk: its smallest column (the base of its column offsets), and ovf set when its
+// columns span more than maxspan
 __global__ __launch_bounds__(256) void k_chunk_span(const int2* __restrict__ desc, const int* __restrict__ c,
-                                                    int* __restrict__ cbase, int* __restrict__ ovf) {
+                                                    int* __restrict__ cbase, int* __restrict__ ovf, int maxspan) {
   __shared__ int slo[4], shi[4];
   const int j0 = desc[blockIdx.x].y, j1 = desc[blockIdx.x + 1].y;
   int lo = INT_MAX, hi = INT_MIN;
@@ -1081,7 +1088,7 @@ __global__ __launch_bounds__(256) void k_chunk_span(const int2* __restrict__ des
     lo = min(lo, slo[0]);
     hi = max(hi, shi[0]);
     cbase[blockIdx.x] = j1 > j0 ? lo : 0;
-    if (j1 > j0 && (int64_t)hi - lo > 65535) atomicOr(ovf, 1);
+    if (j1 > j0 && (int64_t)hi - lo > maxspan) atomicOr(ovf, 1);
   }
 }
 
@@ -1263,7 +1270,8 @@ bool deco_csr::cols16() const {
       cbase.ensure(nchunks);
       DBuf<int> ovf(1);
       ovf.zero();
-      hipLaunchKernelGGL(k_chunk_span, dim3(nchunks), dim3(256), 0, stream(), desc, (const int*)c.p, cbase.p, ovf.p);
+      hipLaunchKernelGGL(k_chunk_span, dim3(nchunks), dim3(256), 0, stream(), desc, (const int*)c.p, cbase.p, ovf.p,
+                         65535);
       DECO_CHECK_LAUNCH();
       int o = 0;
       copy_d2h(&o, ovf.p, 1);
@@ -1279,6 +1287,149 @@ bool deco_csr::cols16() const {
     }
   }
   return c16_ok == 1;
+}
+
+namespace deco {
+// the column-sorted chunk form for the stream products: 0 never, 1 by the rule of
+// deco_csr::sorted (default), 2 wherever it applies
+static int g_stream_sorted = 1;
+// FP32 products take the sorted form when the row-order gathers of a wavefront touch at least
+// this many 64-byte segments of x on average (measured on the 168x100x120 AMG operators: the
+// level-0 interpolation, 7.2, was 0.7 us slower sorted; the level-0 restriction, 10.8, 1.4 us
+// faster; the level matrices, 19-33, 1-5 us faster)
+constexpr double kSortMinSegments = 9.0;
+
+// per 64-entry group of the column array (the stream kernel's wavefront loads): the number of
+// distinct 64-byte segments of an FP32 x its gathers touch, summed into *total
+__global__ __launch_bounds__(256) void k_gather_segments(int nnz, const int* __restrict__ c,
+                                                         unsigned long long* __restrict__ total) {
+  const int j = blockIdx.x * 256 + (int)threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int sg = j < nnz ? (c[j] >> 4) : -1;
+  bool first = j < nnz;
+  for (int k = 0; k < 64; ++k) {
+    const int o = __shfl(sg, k);
+    if (k < lane && o == sg) first = false;
+  }
+  const unsigned long long b = __ballot(first);
+  if (lane == 0) atomicAdd(total, (unsigned long long)__popcll(b));
+}
+
+// Chunk ch's own entries [j0, j1) sorted by column in LDS (a bitonic sort of 4096 keys, column
+// offset << 12 | slot, ties in row order), written as column offset | slot << 20. Every chunk fits
+// its window (at most kStreamTile < 4096 entries from its start), so its slots stay below 4096.
+__global__ __launch_bounds__(256) void k_csort_build(const int2* __restrict__ desc, const int* __restrict__ c,
+                                                     int cap, const int* __restrict__ cbase,
+                                                     unsigned int* __restrict__ p) {
+  constexpr int N = 4096;
+  __shared__ unsigned int key[N];
+  const int ch = blockIdx.x;
+  const int j0 = desc[ch].y, j1 = desc[ch + 1].y, w0 = ch * cap, cb = cbase[ch];
+  const int cnt = j1 - j0;
+  for (int i = threadIdx.x; i < N; i += 256)
+    key[i] = i < cnt ? ((unsigned int)(c[j0 + i] - cb) << 12) | (unsigned int)(j0 + i - w0) : 0xFFFFFFFFu;
+  __syncthreads();
+  for (int k = 2; k <= N; k <<= 1) {
+    for (int h = k >> 1; h > 0; h >>= 1) {
+      for (int i = threadIdx.x; i < N; i += 256) {
+        const int o = i ^ h;
+        if (o > i) {
+          const unsigned int a = key[i], b = key[o];
+          if (((i & k) == 0) ? (a > b) : (a < b)) {
+            key[i] = b;
+            key[o] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = threadIdx.x; i < cnt; i += 256) {
+    const unsigned int q = key[i];
+    p[j0 + i] = (q >> 12) | ((q & 0xFFFu) << kSortColBits);
+  }
+}
+
+// values in the sorted order: dst[j] = src[window start + slot of entry j]
+template <class VT>
+__global__ __launch_bounds__(256) void k_csort_vals(const int2* __restrict__ desc, int cap,
+                                                    const unsigned int* __restrict__ p, const VT* __restrict__ src,
+                                                    VT* __restrict__ dst) {
+  const int ch = blockIdx.x;
+  const int j0 = desc[ch].y, j1 = desc[ch + 1].y, w0 = ch * cap;
+  for (int j = j0 + (int)threadIdx.x; j < j1; j += 256) dst[j] = src[w0 + (int)(p[j] >> kSortColBits)];
+}
+}  // namespace deco
+
+double deco_csr::gather_segments() const {
+  using namespace deco;
+  if (gather_seg < 0.0) {
+    gather_seg = 0.0;
+    if (nnz > 0) {
+      DBuf<unsigned long long> tot(1);
+      tot.zero();
+      hipLaunchKernelGGL(k_gather_segments, dim3(grid_for(nnz, 256)), dim3(256), 0, stream(), nnz, (const int*)c.p,
+                         tot.p);
+      DECO_CHECK_LAUNCH();
+      unsigned long long t = 0;
+      copy_d2h(&t, tot.p, 1);
+      gather_seg = (double)t / (double)((nnz + 63) / 64);
+    }
+  }
+  return gather_seg;
+}
+
+const deco_csort* deco_csr::sorted(bool fp64) const {
+  using namespace deco;
+  static_assert(kStreamTile < 4096, "slots of the sorted form are 12 bits");
+  if (g_stream_sorted == 0 || n <= 0 || nnz <= 0) return nullptr;
+  // the rule: FP64 values only in place of 32-bit columns (with 16-bit columns the two extra bytes
+  // per entry cost more than the gathers save); FP32 values where the row-order gathers scatter
+  if (g_stream_sorted == 1 && (fp64 ? cols16() : gather_segments() < kSortMinSegments)) return nullptr;
+  if (cs_ok < 0) {
+    cs_ok = 0;
+    const int2* desc = row_chunks();
+    if (chunks_fit) {
+      auto* S = new deco_csort();
+      S->cbase.alloc(nchunks);
+      DBuf<int> ovf(1);
+      ovf.zero();
+      hipLaunchKernelGGL(k_chunk_span, dim3(nchunks), dim3(256), 0, stream(), desc, (const int*)c.p, S->cbase.p,
+                         ovf.p, (int)kSortColMask);
+      DECO_CHECK_LAUNCH();
+      int o = 0;
+      copy_d2h(&o, ovf.p, 1);
+      if (o == 0) {
+        S->p.alloc(nnz);
+        hipLaunchKernelGGL(k_csort_build, dim3(nchunks), dim3(256), 0, stream(), desc, (const int*)c.p, chunk_cap,
+                           (const int*)S->cbase.p, S->p.p);
+        DECO_CHECK_LAUNCH();
+        cs = S;
+        cs_ok = 1;
+      } else {
+        delete S;
+      }
+    }
+  }
+  if (cs_ok != 1) return nullptr;
+  const int2* desc = row_chunks();
+  if (fp64 && !cs->has64) {
+    cs->v64.alloc(nnz);
+    hipLaunchKernelGGL(k_csort_vals<double>, dim3(nchunks), dim3(256), 0, stream(), desc, chunk_cap,
+                       (const unsigned int*)cs->p.p, (const double*)v.p, cs->v64.p);
+    DECO_CHECK_LAUNCH();
+    cs->has64 = true;
+  }
+  if (!fp64 && !cs->has32) {
+    // the FP32 values are the f32 shadow's, so the products equal those on the shadow bit for bit
+    const float* f = const_cast<deco_csr*>(this)->values_f32();
+    cs->vf.alloc(nnz);
+    hipLaunchKernelGGL(k_csort_vals<float>, dim3(nchunks), dim3(256), 0, stream(), desc, chunk_cap,
+                       (const unsigned int*)cs->p.p, f, cs->vf.p);
+    DECO_CHECK_LAUNCH();
+    cs->has32 = true;
+  }
+  return cs;
 }
 
 const float* deco_csr::values_f32() {
@@ -1616,6 +1767,13 @@ int deco_set_sell_dia(int enable) {
 
 int deco_set_stream_pieces(int enable) {
   return guard([&] { g_stream_pieces = enable != 0; });
+}
+
+int deco_set_stream_sorted(int mode) {
+  return guard([&] {
+    if (mode < 0 || mode > 2) fail(DECO_ERR_ARG, "stream sorted mode %d (0, 1 or 2)", mode);
+    g_stream_sorted = mode;
+  });
 }
 
 int deco_read_bandwidth(size_t bytes, int iters, double* gbps) {

@@ -62,6 +62,21 @@ struct deco_pieces {
   bool d16 = false, has64 = false;
 };
 
+// Column-sorted form of the CSR-stream chunks: the entries of each chunk's own rows, [j0, j1),
+// stored by ascending column, each with the slot of its row-order position in the chunk's window
+// (j - k*cap, below 4096), so that the stream kernel's x gathers of one wavefront instruction touch
+// a few adjacent cache lines while the products still land in row order in LDS (the same sums, bit
+// for bit). Packed per entry: (column - cbase[chunk]) in the low 20 bits, the slot in the high 12.
+struct deco_csort {
+  deco::DBuf<int> cbase;             // per chunk: its smallest column
+  deco::DBuf<unsigned int> p;        // per entry: column offset | slot << 20
+  deco::DBuf<float> vf;              // per entry: FP32 value in the same order
+  deco::DBuf<double> v64;            // per entry: FP64 value (filled on first FP64 use)
+  bool has32 = false, has64 = false;
+};
+constexpr int kSortColBits = 20;
+constexpr unsigned int kSortColMask = (1u << kSortColBits) - 1u;
+
 struct deco_csr {
   int n = 0, m = 0, nnz = 0;
   deco::DBuf<int> r, c;
@@ -75,6 +90,7 @@ struct deco_csr {
   mutable int nchunks = 0;
   mutable int chunk_cap = 0;
   mutable int chunk_win = 0;   // entries a chunk's range can span: cap + longest row - 1 (at most kStreamTile)
+  mutable bool chunks_fit = false;   // every chunk's range lies in its window (no row above half the tile)
   mutable bool rb_valid = false;
   ~deco_csr();
   const float* values_f32();  // creates / refreshes the shadow
@@ -103,6 +119,17 @@ struct deco_csr {
   const deco_pieces* pieces(bool fp64) const;
   mutable deco_pieces* pc = nullptr;
   mutable signed char pc_ok = -1;
+  // the column-sorted chunk form (deco_csort) for products with FP64 (fp64) or FP32 values (from
+  // the f32 shadow, created if needed), built on first call when every chunk fits its window and
+  // spans fewer than 2^20 columns; null where it does not apply or, by default, where the
+  // row-order form is expected to be faster (FP64 values with 16-bit columns; FP32 values with
+  // row-order gathers on few segments of x). Dropped by invalidate().
+  const deco_csort* sorted(bool fp64) const;
+  mutable deco_csort* cs = nullptr;
+  mutable signed char cs_ok = -1;
+  // distinct 64-byte segments of an FP32 x per 64 row-order gathers, on average (computed once)
+  double gather_segments() const;
+  mutable double gather_seg = -1.0;
   void invalidate();
 };
 
@@ -303,7 +330,10 @@ __device__ __forceinline__ AT stream_row_sum(const AT* prod, int k, int e, int G
 }
 
 // CT int: c holds the columns. CT unsigned short: c holds 16-bit offsets from cbase[chunk]
-// (deco_csr::cols16).
+// (deco_csr::cols16). CT unsigned int: the column-sorted form (deco_csort; every chunk fits its
+// window): c holds column offset | slot << 20 and v the values in the same order; an entry of the
+// chunk's own range writes its product to its slot, the window's other entries to a slot no row
+// reads.
 template <class VT, class XG, class E, class CT = int>
 __global__ __launch_bounds__(stream_block<VT>()) void k_spmv_stream(const int2* __restrict__ desc,
                                                                    const int* __restrict__ r,
@@ -316,6 +346,8 @@ __global__ __launch_bounds__(stream_block<VT>()) void k_spmv_stream(const int2* 
   constexpr int BLK = stream_block<VT>();
   constexpr int U = (kStreamTile + BLK - 1) / BLK;
   constexpr int WIN = U * BLK;
+  constexpr bool SORTED = std::is_same<CT, unsigned int>::value;
+  static_assert(WIN - 1 >= kStreamTile && WIN <= (1 << (32 - kSortColBits)), "slot range");
   __shared__ AT prod[WIN];
   const int ch = xcd_block(blockIdx.x, gridDim.x);
   const int w0 = ch * cap;   // window start
@@ -324,12 +356,19 @@ __global__ __launch_bounds__(stream_block<VT>()) void k_spmv_stream(const int2* 
   if constexpr (!std::is_same<CT, int>::value) cb = cbase[ch];
   int cc[U];
   VT vv[U];
+  int slot[SORTED ? U : 1];
   const int jl = min(w0 + win, nnz) - 1;   // last entry of the window
   if (nnz > 0) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int j = min(w0 + tid + u * BLK, jl);
-      cc[u] = min(cb + (int)__builtin_nontemporal_load(c + j), m - 1);
+      if constexpr (SORTED) {
+        const unsigned int p = __builtin_nontemporal_load(c + j);
+        cc[u] = min(cb + (int)(p & kSortColMask), m - 1);
+        slot[u] = (int)(p >> kSortColBits);
+      } else {
+        cc[u] = min(cb + (int)__builtin_nontemporal_load(c + j), m - 1);
+      }
       vv[u] = __builtin_nontemporal_load(v + j);
     }
   }
@@ -360,7 +399,14 @@ __global__ __launch_bounds__(stream_block<VT>()) void k_spmv_stream(const int2* 
 #pragma unroll
     for (int u = 0; u < U; ++u) xx[u] = x(cc[u]);
 #pragma unroll
-    for (int u = 0; u < U; ++u) prod[tid + u * BLK] = (AT)vv[u] * (AT)xx[u];
+    for (int u = 0; u < U; ++u) {
+      if constexpr (SORTED) {
+        const int j = w0 + tid + u * BLK;
+        prod[(j >= j0 && j < j1) ? slot[u] : WIN - 1] = (AT)vv[u] * (AT)xx[u];
+      } else {
+        prod[tid + u * BLK] = (AT)vv[u] * (AT)xx[u];
+      }
+    }
   }
   __syncthreads();
   if (fits) {
@@ -377,7 +423,7 @@ __global__ __launch_bounds__(stream_block<VT>()) void k_spmv_stream(const int2* 
         if (sub == 0) epi(rw, a2, epi.pre(rw));
       }
     }
-  } else {
+  } else if constexpr (!SORTED) {   // (the sorted form exists only when every chunk fits)
     __shared__ AT red[BLK / 64];
     for (int rw = r0; rw < r1; ++rw) {
       const int b = r[rw], e = r[rw + 1];
@@ -467,6 +513,24 @@ bool spmv_piece(const deco_csr* A, XG x, E epi) {
   return true;
 }
 
+// the product on A's column-sorted chunk form when it has one (values of A in VT), true if launched
+template <class VT, class XG, class E>
+bool spmv_sorted(const deco_csr* A, XG x, E epi) {
+  const deco_csort* S = A->sorted(sizeof(VT) == 8);
+  if (!S) return false;
+  const VT* val;
+  if constexpr (sizeof(VT) == 8) val = S->v64.p;
+  else val = S->vf.p;
+  const int2* desc = A->row_chunks();
+  hipLaunchKernelGGL((k_spmv_stream<VT, XG, E, unsigned int>), dim3(A->nchunks), dim3(stream_block<VT>()), 0,
+                     stream(), desc, A->r.p, (const unsigned int*)S->p.p, val, x, epi, (const int*)S->cbase.p,
+                     A->chunk_cap, A->chunk_win, A->nnz, A->m);
+  DECO_CHECK_LAUNCH();
+  count_bytes((double)A->nnz * (sizeof(VT) + 4) + 4.0 * (A->n + 1) + 12.0 * A->nchunks + 8.0 +
+              (double)sizeof(typename XG::type) * A->m + (double)A->n * epi.row_bytes());
+  return true;
+}
+
 template <class VT, class XG, class E>
 void spmv_stream_g(const deco_csr* A, const VT* v, XG x, E epi) {
   if (A->n <= 0) return;
@@ -482,8 +546,10 @@ void spmv_stream_g(const deco_csr* A, const VT* v, XG x, E epi) {
 template <class VT, class XG, class E>
 void spmv_stream_c16(const deco_csr* A, const VT* v, XG x, E epi) {
   if (A->n <= 0) return;
-  // A's own values: its row-piece form where it has one
-  if ((v == (const VT*)A->v.p || (A->vf_valid && v == (const VT*)A->vf.p)) && spmv_piece<VT>(A, x, epi)) return;
+  // A's own values: its row-piece form where it has one, else its column-sorted chunk form
+  if ((v == (const VT*)A->v.p || (A->vf_valid && v == (const VT*)A->vf.p)) &&
+      (spmv_piece<VT>(A, x, epi) || spmv_sorted<VT>(A, x, epi)))
+    return;
   if (!A->cols16()) {
     spmv_stream_g(A, v, x, epi);
     return;

@@ -354,9 +354,45 @@ def test_amg_row_piece_products_match_stream(gpu):
     assert rel(out[True][2], out[False][2]) <= 1e-5
 
 
+def _sorted_vs_row_order(G_factory, x):
+    """V-cycles (FP64 forward and transposed, FP32 forward) with the CSR-stream products on the
+    column-sorted chunk form wherever it applies (mode 2), by the library's rule (1), and on the
+    row-order arrays (0), the row-piece form off."""
+    out = {}
+    deco_amd.set_stream_pieces(False)
+    try:
+        for mode in (2, 1, 0):
+            deco_amd.set_stream_sorted(mode)
+            G = G_factory()
+            y64, yt = G.apply(x).numpy(), G.apply_t(x).numpy()
+            G.set_precision(deco_amd.PREC_FP32)
+            out[mode] = (y64, yt, G.apply(x).numpy())
+    finally:
+        deco_amd.set_stream_sorted(1)
+        deco_amd.set_stream_pieces(True)
+    return out
+
+
+@pytest.mark.parametrize("relax", [18, -1])
+def test_amg_sorted_chunks_bit_identical(gpu, relax):
+    """The stream products on the column-sorted chunk form (entries by column, products written
+    to their row-order slots) give the V-cycle of the row-order arrays bit for bit, in both
+    precisions and transposed, with the Jacobi-type up-sweep operators (18) and the two-stage
+    smoother's fused restriction operators (-1)."""
+    g = gen_system(1)
+    tree = {"amg_SoC_alpha": "0.5", "amg_coarse_type": "10", "amg_relax_type": str(relax),
+            "amg_prol_trunc_fac": "1e-3", "amg_A_drop_tol": "1e-4", "amg_max_coarse": "50"}
+    x = np.random.default_rng(17).uniform(-1, 1, g["n"] // 2)
+    out = _sorted_vs_row_order(lambda: deco_amd.AMG(tree, dev_csr(g).local_matrix(2, 0)), x)
+    for mode in (2, 1):
+        for a, b in zip(out[mode], out[0]):
+            assert np.array_equal(a, b)
+
+
 def test_amg_wide_chunk_span_falls_back(gpu):
     """Couplings 80,000 rows apart make the restriction's row chunks span more than 65535
-    columns: those products keep 32-bit columns, and the V-cycle still matches the oracle's."""
+    columns: those products keep 32-bit columns (20-bit offsets in the column-sorted form), and
+    the V-cycle still matches the oracle's."""
     import scipy.sparse as sp
     m = 300
     T = sp.diags([-1.0, 2.0, -1.0], [-1, 0, 1], shape=(m, m))
@@ -374,6 +410,11 @@ def test_amg_wide_chunk_span_falls_back(gpu):
     assert len(levels) >= 2
     x = np.random.default_rng(15).uniform(-1, 1, n)
     assert rel(G.apply(x).numpy(), Go.apply(x)) <= 1e-12
+    # the column-sorted form takes these chunks too (20-bit offsets), with the same sums
+    out = _sorted_vs_row_order(lambda: deco_amd.AMG(tree, deco_amd.CSR.from_scipy(S)), x)
+    for mode in (2, 1):
+        for a, b in zip(out[mode], out[0]):
+            assert np.array_equal(a, b)
 
 
 @pytest.mark.parametrize("relax,cover", [(18, 1), (7, 2), (-1, 1), (-1, 2), (12, 2), (14, 1)])
