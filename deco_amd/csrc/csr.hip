@@ -1293,10 +1293,10 @@ namespace deco {
 // the column-sorted chunk form for the stream products: 0 never, 1 by the rule of
 // deco_csr::sorted (default), 2 wherever it applies
 static int g_stream_sorted = 1;
-// FP32 products take the sorted form when the row-order gathers of a wavefront touch at least
-// this many 64-byte segments of x on average (measured on the 168x100x120 AMG operators: the
-// level-0 interpolation, 7.2, was 0.7 us slower sorted; the level-0 restriction, 10.8, 1.4 us
-// faster; the level matrices, 19-33, 1-5 us faster)
+// FP32 products of matrices with 16-bit chunk columns take the sorted form when the row-order
+// gathers of a wavefront touch at least this many 64-byte segments of x on average (measured on
+// the 168x100x120 AMG operators: the level-0 interpolation, 7.2, was 0.7 us slower sorted; the
+// level-0 restriction, 10.8, 1.4 us faster; the level matrices, 19-33, 1-5 us faster)
 constexpr double kSortMinSegments = 9.0;
 
 // per 64-entry group of the column array (the stream kernel's wavefront loads): the number of
@@ -1383,9 +1383,10 @@ const deco_csort* deco_csr::sorted(bool fp64) const {
   using namespace deco;
   static_assert(kStreamTile < 4096, "slots of the sorted form are 12 bits");
   if (g_stream_sorted == 0 || n <= 0 || nnz <= 0) return nullptr;
-  // the rule: FP64 values only in place of 32-bit columns (with 16-bit columns the two extra bytes
-  // per entry cost more than the gathers save); FP32 values where the row-order gathers scatter
-  if (g_stream_sorted == 1 && (fp64 ? cols16() : gather_segments() < kSortMinSegments)) return nullptr;
+  // the rule: in place of 32-bit columns always (the same bytes: measured on A of the 168x100x120
+  // system, FP32 108 -> 80 us, FP64 142 -> 141 us); in place of 16-bit columns (two bytes more per
+  // entry) only for FP32 values whose row-order gathers scatter
+  if (g_stream_sorted == 1 && cols16() && (fp64 || gather_segments() < kSortMinSegments)) return nullptr;
   if (cs_ok < 0) {
     cs_ok = 0;
     const int2* desc = row_chunks();

@@ -122,7 +122,7 @@ struct deco_csr {
   // the column-sorted chunk form (deco_csort) for products with FP64 (fp64) or FP32 values (from
   // the f32 shadow, created if needed), built on first call when every chunk fits its window and
   // spans fewer than 2^20 columns; null where it does not apply or, by default, where the
-  // row-order form is expected to be faster (FP64 values with 16-bit columns; FP32 values with
+  // row-order form has 16-bit columns and is expected to be faster (FP64 values; FP32 values with
   // row-order gathers on few segments of x). Dropped by invalidate().
   const deco_csort* sorted(bool fp64) const;
   mutable deco_csort* cs = nullptr;
@@ -531,8 +531,9 @@ bool spmv_sorted(const deco_csr* A, XG x, E epi) {
   return true;
 }
 
+// the CSR-stream product on A's row-order arrays with 32-bit columns
 template <class VT, class XG, class E>
-void spmv_stream_g(const deco_csr* A, const VT* v, XG x, E epi) {
+void spmv_stream_i32(const deco_csr* A, const VT* v, XG x, E epi) {
   if (A->n <= 0) return;
   const int2* desc = A->row_chunks();
   hipLaunchKernelGGL((k_spmv_stream<VT, XG, E>), dim3(A->nchunks), dim3(stream_block<VT>()), 0, stream(), desc,
@@ -542,16 +543,30 @@ void spmv_stream_g(const deco_csr* A, const VT* v, XG x, E epi) {
               (double)sizeof(typename XG::type) * A->m + (double)A->n * epi.row_bytes());
 }
 
-// the same on the 16-bit chunk columns when A has them (deco_csr::cols16), else as above
+// whether v is A's own values (FP64, or the current f32 shadow), which its other forms copy
+template <class VT>
+bool own_values(const deco_csr* A, const VT* v) {
+  return v == (const VT*)A->v.p || (A->vf_valid && v == (const VT*)A->vf.p);
+}
+
+// the CSR-stream product: on A's column-sorted chunk form where deco_csr::sorted takes it (A's
+// own values), else on the row-order arrays with 32-bit columns
+template <class VT, class XG, class E>
+void spmv_stream_g(const deco_csr* A, const VT* v, XG x, E epi) {
+  if (A->n <= 0) return;
+  if (own_values(A, v) && spmv_sorted<VT>(A, x, epi)) return;
+  spmv_stream_i32(A, v, x, epi);
+}
+
+// the product of the AMG operators: A's row-piece form where it has one, else its column-sorted
+// chunk form (both for A's own values), else the row-order arrays with 16-bit chunk columns when
+// A has them (deco_csr::cols16), else with 32-bit columns
 template <class VT, class XG, class E>
 void spmv_stream_c16(const deco_csr* A, const VT* v, XG x, E epi) {
   if (A->n <= 0) return;
-  // A's own values: its row-piece form where it has one, else its column-sorted chunk form
-  if ((v == (const VT*)A->v.p || (A->vf_valid && v == (const VT*)A->vf.p)) &&
-      (spmv_piece<VT>(A, x, epi) || spmv_sorted<VT>(A, x, epi)))
-    return;
+  if (own_values(A, v) && (spmv_piece<VT>(A, x, epi) || spmv_sorted<VT>(A, x, epi))) return;
   if (!A->cols16()) {
-    spmv_stream_g(A, v, x, epi);
+    spmv_stream_i32(A, v, x, epi);
     return;
   }
   const int2* desc = A->row_chunks();

@@ -238,14 +238,15 @@ int deco_set_sell_dia(int enable);
  * 2.8M entries (latency-bound products) when its slots stay within 1.3 times the entries, else
  * the CSR-stream kernel. No reference counterpart. */
 int deco_set_stream_pieces(int enable);
-/* When the CSR-stream products of those matrices read the column-sorted chunk form: each stream
+/* When the CSR-stream products (those of the matrices above, and the plain CSR products with a
+ * matrix's own values, such as deco_spmv) read the column-sorted chunk form: each stream
  * chunk's entries stored by ascending column with the slot of their row-order position, so that
  * the x gathers of one wavefront fall on a few cache lines and the products still sum in row order
  * (the same results bit for bit). The form applies when every chunk's columns span fewer than 2^20
- * and no row exceeds half the chunk tile. mode 1 (default): products with FP32 values whose
- * row-order gathers touch at least 9 distinct 64-byte segments of x per wavefront on average, and
- * products with FP64 values where the row-order form would store 32-bit columns; mode 2: wherever it
- * applies; mode 0: never. The row-piece form goes first where it applies. No reference
+ * and no row exceeds half the chunk tile. mode 1 (default): wherever the row-order form would
+ * store 32-bit columns (the same bytes), and in place of 16-bit columns for products with FP32
+ * values whose row-order gathers touch at least 9 distinct 64-byte segments of x per wavefront on
+ * average; mode 2: wherever it applies; mode 0: never. The row-piece form goes first where it applies. No reference
  * counterpart. */
 int deco_set_stream_sorted(int mode);
 /* The R x C block shadow of A (created if needed): its layout (0 chunked, 1 sliced, 2 sliced with
