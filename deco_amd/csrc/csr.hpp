@@ -223,6 +223,7 @@ void launch_spmv(int G, int nout, int rs, const int* r, const int* c, const VT* 
 // reduction per row.
 constexpr int kStreamCap = 3072;                 // target entries per chunk
 constexpr int kStreamTile = kStreamCap + 128;    // window entries a chunk's range must fit
+constexpr int kStreamPre = 3;                    // a lane's further rows whose operands load together
 template <class VT>
 constexpr int stream_block() {
   return sizeof(VT) == 8 ? 256 : 512;
@@ -411,12 +412,39 @@ __global__ __launch_bounds__(stream_block<VT>()) void k_spmv_stream(const int2* 
   __syncthreads();
   if (fits) {
     if (row < r1) {   // a lane group is active or inactive as a whole
-      const int G = 1 << lg;
+      const int G = 1 << lg, S = BLK >> lg;
       const bool inorder = cnt < 8 * nr;   // short rows: column order
+      // A chunk of more rows than lane groups gives a lane further rows, row + S, row + 2S, ...:
+      // the bounds and epilogue operands of the next kStreamPre of them load together here, after
+      // the window's registers are free, so their memory rounds overlap instead of following one
+      // another in the loop below.
+      int pb[kStreamPre], pe[kStreamPre];
+      typename E::Pre pp[kStreamPre];
+#pragma unroll
+      for (int k = 0; k < kStreamPre; ++k) {
+        const int rw = row + (k + 1) * S;
+        pb[k] = 0;
+        pe[k] = 0;
+        pp[k] = typename E::Pre{};
+        if (rw < r1) {
+          pb[k] = r[rw];
+          pe[k] = r[rw + 1];
+          pp[k] = epi.pre(rw);
+        }
+      }
       AT acc = stream_row_sum<AT>(prod, rbeg - w0 + sub, rend - w0, G, inorder);
       for (int off = G >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
       if (sub == 0) epi(row, acc, pre);
-      for (int rw = row + (BLK >> lg); rw < r1; rw += BLK >> lg) {
+#pragma unroll
+      for (int k = 0; k < kStreamPre; ++k) {
+        const int rw = row + (k + 1) * S;
+        if (rw < r1) {
+          AT a2 = stream_row_sum<AT>(prod, pb[k] - w0 + sub, pe[k] - w0, G, inorder);
+          for (int off = G >> 1; off > 0; off >>= 1) a2 += __shfl_xor(a2, off);
+          if (sub == 0) epi(rw, a2, pp[k]);
+        }
+      }
+      for (int rw = row + (kStreamPre + 1) * S; rw < r1; rw += S) {
         const int b = r[rw] - w0, e = r[rw + 1] - w0;
         AT a2 = stream_row_sum<AT>(prod, b + sub, e, G, inorder);
         for (int off = G >> 1; off > 0; off >>= 1) a2 += __shfl_xor(a2, off);
