@@ -363,33 +363,36 @@ __global__ __launch_bounds__(256) void k_dense(int n, const MT* __restrict__ C, 
 }
 
 // x = Z b for the dense sub-cycle operator Z, n rows of ld >= n entries (ld a multiple of 4, the
-// columns past n zero, so every row starts on a 16-byte boundary): two rows per wavefront, each lane
-// loads 16-byte vectors of both rows and the b entries they pair with, kDenseU vectors per row in
-// flight (one scalar load per lane and row kept about 4 MB in flight over the chip, a third of what
-// the HBM needs); FP64 arithmetic, fixed-order butterfly sums; with add_to set,
-// add_to[i*stride] += x_i instead of storing x
+// columns past n zero, so every row starts on a 16-byte boundary): a workgroup takes two rows and
+// each of its four wavefronts a quarter of their columns, so one round of loads covers a row pair
+// (the whole operator, 76 MB in FP32 at 4365 rows, is in flight at once instead of in three rounds
+// of two-row wavefronts); each lane loads 16-byte vectors of both rows and the b entries they pair
+// with, kDenseU vectors per row in flight; FP64 arithmetic, fixed-order butterfly sums, the four
+// quarters added in order; with add_to set, add_to[i*stride] += x_i instead of storing x
 constexpr int kDenseU = 6;
 template <class T, class MT>
 __global__ __launch_bounds__(256) void k_dense_tail(int n, int ld, const MT* __restrict__ Z, const T* __restrict__ b,
                                                     T* __restrict__ x, T* __restrict__ add_to, int stride) {
   constexpr int VM = 16 / sizeof(MT);   // entries per 16-byte vector
   typedef MT mv __attribute__((ext_vector_type(VM)));
-  const int lane = threadIdx.x & 63;
-  const int row0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * 2;
-  if (row0 >= n) return;
+  __shared__ double part[4][2];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int row0 = blockIdx.x * 2;   // the grid covers the rows, so row0 < n
   const bool two = row0 + 1 < n;
   const mv* z0 = reinterpret_cast<const mv*>(Z + (size_t)row0 * ld);
   const mv* z1 = two ? z0 + ld / VM : z0;
   const bool bvec = ((uintptr_t)b & 15) == 0 && sizeof(T) == sizeof(MT);
   const int nv = ld / VM;
+  const int q = (nv + 3) / 4;
+  const int vb = w * q, ve = min(nv, vb + q);   // this wavefront's quarter of the vectors
   double s0 = 0.0, s1 = 0.0;
-  for (int v0 = 0; v0 < nv; v0 += 64 * kDenseU) {
+  for (int v0 = vb; v0 < ve; v0 += 64 * kDenseU) {
     mv a0[kDenseU], a1[kDenseU];
     T bb[kDenseU][VM];
 #pragma unroll
     for (int u = 0; u < kDenseU; ++u) {
       const int v = v0 + lane + 64 * u;
-      if (v < nv) {
+      if (v < ve) {
         a0[u] = __builtin_nontemporal_load(z0 + v);
         a1[u] = __builtin_nontemporal_load(z1 + v);
       } else {
@@ -399,33 +402,41 @@ __global__ __launch_bounds__(256) void k_dense_tail(int n, int ld, const MT* __r
     }
 #pragma unroll
     for (int u = 0; u < kDenseU; ++u) {
-      const int e = (v0 + lane + 64 * u) * VM;
-      if (bvec && e + VM <= n) {
+      const int v = v0 + lane + 64 * u;
+      const int e = v * VM;
+      if (v < ve && bvec && e + VM <= n) {
         const mv t = *reinterpret_cast<const mv*>(b + e);
 #pragma unroll
-        for (int q = 0; q < VM; ++q) bb[u][q] = (T)t[q];
+        for (int qq = 0; qq < VM; ++qq) bb[u][qq] = (T)t[qq];
       } else {
 #pragma unroll
-        for (int q = 0; q < VM; ++q) bb[u][q] = e + q < n ? b[e + q] : T(0);
+        for (int qq = 0; qq < VM; ++qq) bb[u][qq] = (v < ve && e + qq < n) ? b[e + qq] : T(0);
       }
     }
 #pragma unroll
     for (int u = 0; u < kDenseU; ++u)
 #pragma unroll
-      for (int q = 0; q < VM; ++q) {
-        s0 += (double)a0[u][q] * (double)bb[u][q];
-        s1 += (double)a1[u][q] * (double)bb[u][q];
+      for (int qq = 0; qq < VM; ++qq) {
+        s0 += (double)a0[u][qq] * (double)bb[u][qq];
+        s1 += (double)a1[u][qq] * (double)bb[u][qq];
       }
   }
   s0 = wave_sum(s0);
   s1 = wave_sum(s1);
   if (lane == 0) {
+    part[w][0] = s0;
+    part[w][1] = s1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double t0 = ((part[0][0] + part[1][0]) + part[2][0]) + part[3][0];
+    const double t1 = ((part[0][1] + part[1][1]) + part[2][1]) + part[3][1];
     if (add_to) {
-      add_to[(size_t)row0 * stride] += (T)s0;
-      if (two) add_to[(size_t)(row0 + 1) * stride] += (T)s1;
+      add_to[(size_t)row0 * stride] += (T)t0;
+      if (two) add_to[(size_t)(row0 + 1) * stride] += (T)t1;
     } else {
-      x[row0] = (T)s0;
-      if (two) x[row0 + 1] = (T)s1;
+      x[row0] = (T)t0;
+      if (two) x[row0 + 1] = (T)t1;
     }
   }
 }
@@ -866,7 +877,7 @@ struct AMG : deco_prec {
   template <class T, class MT>
   void dense_tail(const MT* Zop, const T* b, T* x, T* add_to, int stride) {
     const int m = tail;
-    hipLaunchKernelGGL((k_dense_tail<T, MT>), dim3((m + 7) / 8), dim3(256), 0, stream(), m, tail_ld, Zop, b, x, add_to,
+    hipLaunchKernelGGL((k_dense_tail<T, MT>), dim3((m + 1) / 2), dim3(256), 0, stream(), m, tail_ld, Zop, b, x, add_to,
                        stride);
     DECO_CHECK_LAUNCH();
     count_bytes((double)m * m * sizeof(MT) + (double)m * sizeof(T) * (add_to ? 3 : 2));
