@@ -365,3 +365,48 @@ def test_stream_product_row_lengths(gpu):
     assert np.all(np.abs(y32 - ref) <= 1e-5 * (absAx + 1e-30) + 1e-6)
     E = deco_amd.CSR(5000, 100, np.zeros(5001, np.int32), np.zeros(0, np.int32), np.zeros(0))
     assert np.array_equal(E.apply(np.ones(100)).numpy(), np.zeros(5000))
+
+
+@pytest.mark.parametrize("shape", ["ragged", "long_rows", "too_long", "wide"])
+def test_column_sorted_chunks_bit_identical(gpu, shape):
+    """CSR products on the column-sorted chunk form, forced wherever it applies, against the
+    row-order arrays: the same values bit for bit in FP64 and FP32 (product and residual) for
+    ragged rows with empty ones, rows up to half the chunk tile (every chunk fits its window), a
+    row longer than that (the form does not apply) and chunks spanning more than 2^20 columns
+    (the form does not apply); the FP64 product against scipy to the CSR-stream bound."""
+    rng = np.random.default_rng({"ragged": 21, "long_rows": 22, "too_long": 23, "wide": 24}[shape])
+    n = 30_000
+    m = 3_000_000 if shape == "wide" else n
+    if shape == "ragged":
+        lens = rng.integers(0, 40, n)
+        lens[rng.choice(n, 500, replace=False)] = 0
+    else:
+        lens = rng.integers(1, 12, n)
+        if shape == "long_rows":
+            lens[rng.choice(n, 20, replace=False)] = rng.integers(200, 1600, 20)
+        elif shape == "too_long":
+            lens[n // 2] = 2500
+    cols = []
+    for i, k in enumerate(lens):
+        lo, hi = (0, m) if shape == "wide" else (max(0, i - 6000), min(m, i + 6000))
+        cols.append(np.unique(lo + rng.integers(0, hi - lo, k)))   # sorted, duplicates dropped
+    lens = np.array([len(c_) for c_ in cols])
+    r = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    c = np.concatenate(cols).astype(np.int32)
+    v = rng.uniform(-1, 1, len(c))
+    x = rng.uniform(-1, 1, m)
+    z = rng.uniform(-1, 1, n)
+    out = {}
+    try:
+        for mode in (2, 0):
+            deco_amd.set_stream_sorted(mode)
+            A = deco_amd.CSR(n, m, r, c, v)
+            out[mode] = (A.apply(x).numpy(), A.residual(x, z).numpy(),
+                         A.apply_f32(x.astype(np.float32)).numpy())
+    finally:
+        deco_amd.set_stream_sorted(1)
+    for a, b in zip(out[2], out[0]):
+        assert np.array_equal(a, b)
+    S = sp.csr_matrix((v, c, r), shape=(n, m))
+    absAx = abs(S) @ np.abs(x)
+    assert np.all(np.abs(out[2][0] - S @ x) <= 2 * max(lens.max(), 1) * 1.2e-16 * absAx)
