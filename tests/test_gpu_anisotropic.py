@@ -110,7 +110,7 @@ def test_fullsize_anisotropic_solves_and_adjoint(big5):
           f"(rtol 1e-8), {out['fp64', '1e-11'][0]} / {out['fp32', '1e-11'][0]} (1e-11); FP32 vs FP64 "
           f"{d8:.2e} at 1e-8, {d11:.2e} at 1e-11; FP64 at 1e-8 vs 1e-11 {e8:.2e}")
     # at rtol 1e-8 the two precisions differ by less than either differs from the tighter solve;
-    # at 1e-11 they agree to the north-star bar
+    # at 1e-11 they agree to 1e-6
     assert d8 <= e8
     assert d11 <= 1e-6
     # adjoint: (W A)^T mu = w, lambda = W^T mu solves A^T lambda = w; dL/dA = -lambda_i x_j
