@@ -144,9 +144,10 @@ def solve_only(deco_amd, g, A, key, log):
 
 
 def adjoint_small(deco_amd, args, log):
-    """Forward CPR-GMRES(30) solve of b = A x* and the adjoint of L = w.x (w ~ N(0,1)): the
-    transposed solve, lambda = W^T mu and dL/dA on A's pattern, to rtol 1e-8 on the 32x32x16
-    two-unknown system, both preconditioner precisions; medians of args.steps runs."""
+    """Forward CPR-GMRES(30) solve of b = A x* and the adjoint of L = w.x (w ~ N(0,1), and the
+    manufactured w = A^T y* whose solution is y*): the transposed solve, lambda = W^T mu and dL/dA
+    on A's pattern, to rtol 1e-8 on the 32x32x16 two-unknown system, both preconditioner
+    precisions; medians of args.steps runs."""
     g = deco_amd.generate(1)
     n = g["n"]
     A = deco_amd.CSR(n, n, g["r"], g["c"], g["v"])
@@ -154,6 +155,10 @@ def adjoint_small(deco_amd, args, log):
     M = deco_amd.CPR(deco_amd.PTree(dict(PREC_TREE, block_size=str(g["bs"]))), A, b)
     sol = {"type": "rgmres", "reltol": "1e-8", "restart": "30", "itmax": "500", "verbosity": "0"}
     w = deco_amd.DeviceArray.from_numpy(np.random.default_rng(1004).standard_normal(n))
+    # a manufactured adjoint right-hand side w = A^T y* (A before decoupling): lambda = y*
+    import scipy.sparse as sps
+    ystar = np.random.default_rng(1006).uniform(-1.0, 1.0, n)
+    wm = deco_amd.DeviceArray.from_numpy(sps.csr_matrix((g["v"], g["c"], g["r"]), shape=(n, n)).T @ ystar)
     out = {"system": f"two-phase 32x32x16, n={n}, nnz={g['nnz']}"}
 
     def med(fn):
@@ -181,9 +186,19 @@ def adjoint_small(deco_amd, args, log):
             deco_amd.grad_A_pattern(A, lam, Sf.x, out=dA)
             return St
         adj_ms, St = med(adjoint)
+
+        def adjoint_man():
+            St = deco_amd.SOL(sol, A, M, wm, mu, transpose=True)
+            M.transform_transpose(St.x, out=lam)
+            deco_amd.grad_A_pattern(A, lam, Sf.x, out=dA)
+            return St
+        adjm_ms, Sm = med(adjoint_man)
+        err_m = float(np.linalg.norm(lam.numpy() - ystar) / np.linalg.norm(ystar))
         out[key] = {"forward_ms": fw_ms, "forward_iter": Sf.iter, "adjoint_ms": adj_ms, "adjoint_iter": St.iter,
                     "ratio_to_forward": adj_ms / fw_ms, "forward_same_rhs_ms": fwv_ms,
-                    "forward_same_rhs_iter": Sv.iter, "ratio_to_forward_same_rhs": adj_ms / fwv_ms}
+                    "forward_same_rhs_iter": Sv.iter, "ratio_to_forward_same_rhs": adj_ms / fwv_ms,
+                    "adjoint_manufactured_ms": adjm_ms, "adjoint_manufactured_iter": Sm.iter,
+                    "adjoint_manufactured_err_vs_ystar": err_m, "ratio_manufactured_to_forward": adjm_ms / fw_ms}
     M.set_precision(deco_amd.PREC_FP64)
     log(f"adjoint (32x32x16): {out}")
     return out
@@ -532,6 +547,10 @@ def main():
                                "speedup_fp32_vs_fp64": j64["wall_ms"] / j32["wall_ms"]}}
         log(f"smoothers: {result['solve']['smoothers']}")
         del M2, A2, b2
+    # a manufactured adjoint right-hand side, as b = A x* is for the forward solve: w = A^T y*
+    # (the system before decoupling), so the adjoint solution is lambda = y*
+    ystar = np.random.default_rng(1006).uniform(-1.0, 1.0, n)
+    w_man = A0.T @ ystar
     del A0
 
     # ---------------------------------------------------------------- adjoint
@@ -570,6 +589,29 @@ def main():
                 "ratio_to_forward_same_rhs": adj_ms / fw_ms,
                 "per_iteration_ratio": (adj_ms / max(St.iter, 1)) / (fw_ms / max(Sf.iter, 1))}
             log(f"adjoint {key}: {result['adjoint'][key]}")
+        # the manufactured right-hand side w = A^T y*: the like-for-like counterpart of the forward
+        # solve of b = A x*, and lambda = y* checks the adjoint solution
+        wm = deco_amd.DeviceArray.from_numpy(w_man)
+        result["adjoint_manufactured"] = {"rhs": "w = A^T y*, y* ~ U[-1,1] (seed 1006), A before decoupling"}
+        for prec, key in ((deco_amd.PREC_FP64, "fp64"), (deco_amd.PREC_FP32, "fp32_precond")):
+            M.set_precision(prec)
+            St = deco_amd.SOL(sol, A, M, wm, mu, transpose=True)
+            deco_amd.lib().deco_sync()
+            ts = []
+            for _ in range(args.steps):
+                t1 = time.perf_counter()
+                St = deco_amd.SOL(sol, A, M, wm, mu, transpose=True)
+                M.transform_transpose(St.x, out=lam)
+                deco_amd.grad_A_pattern(A, lam, S64.x, out=dA)
+                deco_amd.lib().deco_sync()
+                ts.append(time.perf_counter() - t1)
+            adj_ms = float(np.median(ts)) * 1e3
+            lam_h = lam.numpy()
+            result["adjoint_manufactured"][key] = {
+                "wall_ms": adj_ms, "iter": St.iter, "flag": St.flag, "res_rel": St.res_rel,
+                "err_vs_ystar": float(np.linalg.norm(lam_h - ystar) / np.linalg.norm(ystar)),
+                "ratio_to_forward_b": adj_ms / fwd[prec]["wall_ms"], "forward_b_iter": fwd[prec]["iter"]}
+            log(f"adjoint, manufactured rhs, {key}: {result['adjoint_manufactured'][key]}")
         M.set_precision(deco_amd.PREC_FP64)
 
     # adjoint-to-forward time on the 32x32x16 two-unknown system (the adjoint case's timing size)
