@@ -395,12 +395,28 @@ __global__ void k_add3(int n, const double* __restrict__ a, const double* __rest
   const int i = blockIdx.x * blockDim.x + threadIdx.x;  // x += a + e
   if (i < n) x[i] += a[i] + e[i];
 }
+// e = a + P ep: ep (one value per cell) at the pressure unknowns bs*i, 0 elsewhere
+__global__ void k_add_p(int n, int bs, const float* __restrict__ a, const float* __restrict__ ep, float* __restrict__ e) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) e[i] = a[i] + (i % bs == 0 ? ep[i / bs] : 0.f);
+}
+// x += a + P ep
+__global__ void k_add3_p(int n, int bs, const double* __restrict__ a, const double* __restrict__ ep,
+                         double* __restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] += a[i] + (i % bs == 0 ? ep[i / bs] : 0.0);
+}
 
 struct CPR : deco_prec {
   deco_cpr_params p{};
   deco_csr* A = nullptr;                 // caller's matrix, decoupled in place
   deco_csr* Ap = nullptr;
   deco_csr* Apr = nullptr;               // pressure rows (bs*i) of the decoupled A, copied at setup
+  // pressure columns (bs*j) of the decoupled A^T, n x sn, made on the first transposed application
+  // and remade when A changes (atp_gen = A->gen when made): A^T P e for a pressure correction e
+  // reads only them
+  deco_csr* Atp = nullptr;
+  unsigned long long atp_gen = 0;
   deco_prec* glo = nullptr;
   deco_prec* loc = nullptr;
   deco_prec* jac = nullptr;
@@ -431,9 +447,9 @@ struct CPR : deco_prec {
   }
 
   void release() {
-    delete glo; delete loc; delete jac; delete Ap; delete Apr;
+    delete glo; delete loc; delete jac; delete Ap; delete Apr; delete Atp;
     glo = loc = jac = nullptr;
-    Ap = Apr = nullptr;
+    Ap = Apr = Atp = nullptr;
   }
   ~CPR() override { release(); }
 
@@ -595,6 +611,30 @@ struct CPR : deco_prec {
     else spmv_stream_f(M, const_cast<deco_csr*>(M)->values_f32(), x, e);
   }
 
+  // A^T (P ep) for a vector ep on the cells, through the pressure columns of A^T (bs x 1 blocks)
+  const deco_csr* pressure_cols_t() {
+    if (!A->At) A->At = csr_transpose(A);
+    if (!Atp || atp_gen != A->gen) {
+      delete Atp;
+      Atp = nullptr;
+      Atp = csr_col_subset(A->At, bs, 0);
+      atp_gen = A->gen;
+    }
+    return Atp;
+  }
+  template <class E>
+  void atp_product(const double* ep, E e) {
+    const deco_csr* M = pressure_cols_t();
+    if (const deco_bsr* B = blocks64() ? M->block64(bs, 1) : nullptr) bsr_stream_d(B, ep, e);
+    else spmv_stream_f(M, M->v.p, ep, e);
+  }
+  template <class E>
+  void atp_product(const float* ep, E e) {
+    const deco_csr* M = pressure_cols_t();
+    if (const deco_bsr* B = blocks32() ? M->block32(bs, 1) : nullptr) bsr_stream_f(B, ep, e);
+    else spmv_stream_f(M, const_cast<deco_csr*>(M)->values_f32(), ep, e);
+  }
+
   // strided pressure residual rt[i] = r[bs i] - (A x)[bs i]   (CPR.hpp:241-246)
   template <class T>
   void pressure_residual(const T* x, const double* r, T* out) {
@@ -732,26 +772,20 @@ struct CPR : deco_prec {
     if (!F) fail(DECO_ERR_UNSUPPORTED, "the FP32 transposed CPR needs the FSAI global stage");
     s32.ensure(n);
     e32.ensure(n);
+    pressure_cols_t();
     float* a = t32.p;
     fsai_apply32_t(F, u, a);                                           // a = G^T u
     {
       ProfPhase ph("CPR_rest");
-      residual32_t(a, u, s32.p);                                       // s = u - A^T a
-      hipLaunchKernelGGL((k_gather_pressure<float, float>), dim3(gs()), dim3(256), 0, stream(), sn, bs,
-                         (const float*)s32.p, rt32.p);
-      count_bytes((double)sn * 12);
-      DECO_CHECK_LAUNCH();
+      // s = u - A^T a, and its pressure entries rt = s[bs i] from the same pass
+      a_product(A->At, (const float*)a, EpiResP<float, double>{s32.p, u, rt32.p, bs});
     }
-    loc->apply32_t(rt32.p, xp32.p);                                    // e = P L^T R s
+    loc->apply32_t(rt32.p, xp32.p);                                    // e = P ep, ep = L^T R s
     ProfPhase ph_rest("CPR_rest");
-    zero_async(e32.p, sizeof(float) * n);
-    hipLaunchKernelGGL((k_set_pressure<float, float>), dim3(gs()), dim3(256), 0, stream(), sn, bs, e32.p,
-                       (const float*)xp32.p);
-    count_bytes((double)sn * 12);
-    DECO_CHECK_LAUNCH();
-    residual32_t(e32.p, (const float*)s32.p, x32.p);                    // s - A^T e
-    hipLaunchKernelGGL(k_add_f, dim3(gn()), dim3(256), 0, stream(), n, (const float*)a, e32.p);
-    count_bytes(12.0 * n);   // e = a + e
+    atp_product((const float*)xp32.p, EpiStd<EPI_RES, float, float>{x32.p, s32.p});   // s - A^T e
+    hipLaunchKernelGGL(k_add_p, dim3(gn()), dim3(256), 0, stream(), n, bs, (const float*)a, (const float*)xp32.p,
+                       e32.p);
+    count_bytes(8.0 * n + 4.0 * sn);   // a + e
     DECO_CHECK_LAUNCH();
     ph_rest.end();
     fsai_apply32_t_addz(F, x32.p, e32.p, xo);                          // out = a + e + G^T (s - A^T e)
@@ -763,22 +797,16 @@ struct CPR : deco_prec {
     if (!A->At) A->At = csr_transpose(A);
     if (precision == DECO_PREC_FP32) { apply32_t_seq(u, x); return; }
     deco_csr* At = A->At;
+    pressure_cols_t();
     double* a = tmp.p;
     double* s = tmp2.p;
     glo->apply_t(u, a);
-    a_product(At, a, EpiStd<EPI_RES, double, double>{s, u});
-    hipLaunchKernelGGL((k_gather_pressure<double, double>), dim3(gs()), dim3(256), 0, stream(), sn, bs,
-                       (const double*)s, rt.p);
-    count_bytes((double)sn * 12);
-    loc->apply_t(rt.p, xp.p);
-    zero_async(x2.p, sizeof(double) * n);
-    hipLaunchKernelGGL((k_set_pressure<double, double>), dim3(gs()), dim3(256), 0, stream(), sn, bs, x2.p,
-                       (const double*)xp.p);
-    count_bytes((double)sn * 12);
-    a_product(At, x2.p, EpiStd<EPI_RES, double, double>{rt.p, s});   // rt (length n) = s - A^T e
+    a_product(At, (const double*)a, EpiResP<double, double>{s, u, rt.p, bs});   // s = u - A^T a, rt = s[bs i]
+    loc->apply_t(rt.p, xp.p);                                          // e = P ep
+    atp_product((const double*)xp.p, EpiStd<EPI_RES, double, double>{rt.p, s});   // rt (length n) = s - A^T e
     glo->apply_t(rt.p, x);
-    hipLaunchKernelGGL(k_add3, dim3(gn()), dim3(256), 0, stream(), n, (const double*)a, (const double*)x2.p, x);
-    count_bytes(32.0 * n);
+    hipLaunchKernelGGL(k_add3_p, dim3(gn()), dim3(256), 0, stream(), n, bs, (const double*)a, (const double*)xp.p, x);
+    count_bytes(24.0 * n + 8.0 * sn);
     DECO_CHECK_LAUNCH();
   }
 

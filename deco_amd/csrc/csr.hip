@@ -4,6 +4,7 @@
 #include "csr.hpp"
 #include <algorithm>
 #include <climits>
+#include <memory>
 #include <cstring>
 #include <mutex>
 
@@ -492,6 +493,49 @@ deco_csr* csr_row_subset(const deco_csr* A, int stride, int off) {
   return S;
 }
 
+// per row: its entries in the columns stride*j + off
+__global__ void k_colsub_count(int n, int stride, int off, const int* __restrict__ r, const int* __restrict__ c,
+                               int* __restrict__ cnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int k = 0;
+  for (int j = r[i]; j < r[i + 1]; ++j) k += (c[j] % stride == off);
+  cnt[i] = k;
+}
+// one thread per row: those entries with column (c - off) / stride, in order
+__global__ void k_colsub_fill(int n, int stride, int off, const int* __restrict__ r, const int* __restrict__ c,
+                              const double* __restrict__ v, const int* __restrict__ r2, int* __restrict__ c2,
+                              double* __restrict__ v2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int o = r2[i];
+  for (int j = r[i]; j < r[i + 1]; ++j)
+    if (c[j] % stride == off) {
+      c2[o] = (c[j] - off) / stride;
+      v2[o] = v[j];
+      ++o;
+    }
+}
+
+deco_csr* csr_col_subset(const deco_csr* A, int stride, int off) {
+  if (stride <= 0 || off < 0 || off >= stride || A->m % stride != 0)
+    fail(DECO_ERR_ARG, "column subset stride %d offset %d does not fit m=%d", stride, off, A->m);
+  std::unique_ptr<deco_csr> S(csr_new(A->n, A->m / stride));
+  DBuf<int> cnt(A->n + 1);
+  if (A->n > 0) {
+    hipLaunchKernelGGL(k_colsub_count, dim3(grid_for(A->n, 256)), dim3(256), 0, stream(), A->n, stride, off, A->r.p,
+                       A->c.p, cnt.p);
+    DECO_CHECK_LAUNCH();
+  }
+  build_row_ptr(S.get(), cnt.p, A->n);
+  if (A->n > 0) {
+    hipLaunchKernelGGL(k_colsub_fill, dim3(grid_for(A->n, 256)), dim3(256), 0, stream(), A->n, stride, off, A->r.p,
+                       A->c.p, A->v.p, S->r.p, S->c.p, S->v.p);
+    DECO_CHECK_LAUNCH();
+  }
+  return S.release();
+}
+
 // ------------------------------------------------------------------ in-cell reorder ---
 // Unknown k of a cell moves to position reord[k] (rows: reord_row, columns: reord_col);
 // new row i holds old row (i/bs)*bs + ord_row[i%bs] (adapter/csr2coo.cpp:31-37,56-67,82-86).
@@ -615,6 +659,7 @@ deco_csr::~deco_csr() {
 }
 
 void deco_csr::invalidate() {
+  ++gen;
   vf_valid = false;
   rb_valid = false;
   sell_ok = -1;

@@ -130,6 +130,8 @@ struct deco_csr {
   // distinct 64-byte segments of an FP32 x per 64 row-order gathers, on average (computed once)
   double gather_segments() const;
   mutable double gather_seg = -1.0;
+  // count of invalidate() calls: matrices derived from this one compare it to see a change
+  unsigned long long gen = 0;
   void invalidate();
 };
 
@@ -255,6 +257,25 @@ struct EpiStd {
     if (EPI == EPI_ACC) return 2 * sizeof(YT);
     return sizeof(ZT) + sizeof(YT);
   }
+};
+
+// y[o] = z[o] - sum, and for every bs-th row (the pressure rows of a system of bs unknowns per cell)
+// also yp[o / bs] = y[o]: a residual and its pressure restriction in one pass
+template <class YT, class ZT>
+struct EpiResP {
+  YT* __restrict__ y;
+  const ZT* __restrict__ z;
+  YT* __restrict__ yp;
+  int bs;
+  struct Pre { ZT a; };
+  __device__ __forceinline__ Pre pre(int o) const { return Pre{z[o]}; }
+  template <class AT>
+  __device__ __forceinline__ void operator()(int o, AT acc, const Pre& p) const {
+    const YT r = (YT)((AT)p.a - acc);
+    y[o] = r;
+    if (o % bs == 0) yp[o / bs] = r;
+  }
+  size_t row_bytes() const { return sizeof(ZT) + sizeof(YT) + sizeof(YT) / bs; }
 };
 
 // y[o] = sum for the Krylov solvers' product with the system matrix: its own kernel
@@ -967,6 +988,8 @@ void bsr_stream_v(const deco_bsr* B, const VT* bv, const XT* x, E epi) {
     else if (B->R == 1 && B->C == 2) DECO_SELL_LAUNCH(1, 2);
     else if (B->R == 3 && B->C == 3) DECO_SELL_LAUNCH(3, 3);
     else if (B->R == 1 && B->C == 3) DECO_SELL_LAUNCH(1, 3);
+    else if (B->R == 2 && B->C == 1) DECO_SELL_LAUNCH(2, 1);
+    else if (B->R == 3 && B->C == 1) DECO_SELL_LAUNCH(3, 1);
     else fail(DECO_ERR_UNSUPPORTED, "block shape %d x %d", B->R, B->C);
 #undef DECO_SELL_LAUNCH
 #undef DECO_SELL_LAUNCH_KU
@@ -983,6 +1006,8 @@ void bsr_stream_v(const deco_bsr* B, const VT* bv, const XT* x, E epi) {
   else if (B->R == 1 && B->C == 2) DECO_BSR_LAUNCH(1, 2);
   else if (B->R == 3 && B->C == 3) DECO_BSR_LAUNCH(3, 3);
   else if (B->R == 1 && B->C == 3) DECO_BSR_LAUNCH(1, 3);
+  else if (B->R == 2 && B->C == 1) DECO_BSR_LAUNCH(2, 1);
+  else if (B->R == 3 && B->C == 1) DECO_BSR_LAUNCH(3, 1);
   else fail(DECO_ERR_UNSUPPORTED, "block shape %d x %d", B->R, B->C);
 #undef DECO_BSR_LAUNCH
   DECO_CHECK_LAUNCH();
@@ -1053,6 +1078,8 @@ int max_row_length(const deco_csr* A);
 deco_csr* csr_transpose(const deco_csr* A);
 // rows stride*i + off of A (all columns), as an (n/stride) x m CSR
 deco_csr* csr_row_subset(const deco_csr* A, int stride, int off);
+// columns stride*j + off of A (all rows), as an n x (m/stride) CSR with column j
+deco_csr* csr_col_subset(const deco_csr* A, int stride, int off);
 deco_csr* csr_clone(const deco_csr* A);
 deco_csr* csr_new(int n, int m);
 
