@@ -129,6 +129,14 @@ class CSR:
         check(lib().deco_csr_download(self.h, _vp(r), _vp(c), _vp(v)))
         return r, c, v
 
+    def update_values(self, v):
+        """New values on the same pattern (host array of nz entries); drops what was derived from
+        the old ones (shadows, cached transpose)."""
+        v = np.ascontiguousarray(v, np.float64)
+        if v.size != self.nz:
+            raise ValueError(f"{v.size} values for a pattern of {self.nz} entries")
+        check(lib().deco_csr_update_values(self.h, _vp(v)))
+
     def to_scipy(self):
         import scipy.sparse as sp
         r, c, v = self.download()

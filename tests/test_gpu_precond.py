@@ -474,3 +474,38 @@ def test_cpr_dense_tail_is_the_whole_pressure_cycle(gpu, seq):
     M.set_precision(deco_amd.PREC_FP32)
     assert rel(M.apply(x).numpy(), Mo.apply(x)) <= 2e-4
     M.set_precision(deco_amd.PREC_FP64)
+
+
+@pytest.mark.parametrize("cfg", [4, 2])
+def test_transposed_cpr_sequence_and_value_update(gpu, cfg):
+    """The transposed CPR application (seq 3) against its definition assembled on the host from
+    the device's FSAI and AMG transposes and scipy products with the decoupled A^T:
+    M^T u = a + P e_p + G^T (s - A^T P e_p), a = G^T u, s = u - A^T a, e_p = AMG^T(s[bs i]).
+    Then A's values change under the preconditioner (here to 1.5 A): the transposed application's
+    products with A^T, its pressure columns included, follow the new values."""
+    g = gen_system(cfg)
+    bs = g["bs"]
+    A = dev_csr(g)
+    M = deco_amd.CPR(small_params(bs), A, None)
+    F, G = M.fsai(), M.amg()
+    u = np.random.default_rng(21).uniform(-1, 1, g["n"])
+
+    def expected(At):
+        a = F.apply_t(u).numpy()
+        s = u - At @ a
+        e = np.zeros_like(u)
+        e[::bs] = G.apply_t(s[::bs]).numpy()
+        return a + e + F.apply_t(s - At @ e).numpy()
+
+    _, _, v = A.download()
+    At = A.to_scipy().T.tocsr()
+    t0 = M.apply_t(u).numpy()
+    assert rel(t0, expected(At)) <= 1e-12
+    A.update_values(v)                                    # the same values: the same operator
+    assert rel(M.apply_t(u).numpy(), t0) <= 1e-14
+    A.update_values(1.5 * v)
+    t1 = M.apply_t(u).numpy()
+    assert rel(t1, expected(1.5 * At)) <= 1e-12 and rel(t1, t0) > 1e-3
+    M.set_precision(deco_amd.PREC_FP32)
+    assert rel(M.apply_t(u).numpy(), t1) <= 2e-4
+    M.set_precision(deco_amd.PREC_FP64)
