@@ -107,6 +107,7 @@ SIGNATURES = {
     "deco_set_sell_offsets16": (i, [i]),
     "deco_set_sell_dia": (i, [i]),
     "deco_set_stream_pieces": (i, [i]),
+    "deco_set_sell_pairs": (i, [i]),
     "deco_set_stream_sorted": (i, [i]),
     "deco_spmv_block_time": (i, [vp, i, i, i, i, dp, ip]),
     "deco_spgemm": (i, [vp, vp, pp]),

@@ -615,6 +615,13 @@ def set_stream_pieces(enable):
     check(lib().deco_set_stream_pieces(1 if enable else 0))
 
 
+def set_sell_pairs(mode):
+    """FP32 products on uniform-offset sliced block shadows with two slices per wavefront: 0
+    never, 1 when a batch holds at most 16 values per lane (default), 2 always; the same sums
+    bit for bit."""
+    check(lib().deco_set_sell_pairs(int(mode)))
+
+
 def set_stream_sorted(mode):
     """When CSR-stream products of matrices without a block shadow read the column-sorted chunk
     form (the same sums bit for bit, gathers of one wavefront on a few cache lines): 1 or True

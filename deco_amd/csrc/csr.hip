@@ -726,6 +726,7 @@ __global__ void k_bsr_fill(int nb, int R, int C, const int* __restrict__ r, cons
 static double g_sell_padding = 0.10;
 static bool g_sell_offsets16 = true;
 static bool g_sell_dia = true;   // the uniform-offset form where it applies
+int g_sell_pairs = 1;
 
 // 16-bit offsets of the sliced layout's block columns from their block rows (bcd zeroed
 // beforehand: padding entries and the lanes past the last block row keep 0); ovf is set when an
@@ -1809,6 +1810,13 @@ int deco_set_sell_offsets16(int enable) {
 
 int deco_set_sell_dia(int enable) {
   return guard([&] { g_sell_dia = enable != 0; });
+}
+
+int deco_set_sell_pairs(int mode) {
+  return guard([&] {
+    if (mode < 0 || mode > 2) fail(DECO_ERR_ARG, "sell pair mode %d (0, 1 or 2)", mode);
+    g_sell_pairs = mode;
+  });
 }
 
 int deco_set_stream_pieces(int enable) {

@@ -876,79 +876,112 @@ __device__ __forceinline__ int sell_col(const CT* __restrict__ bc, size_t e, int
 // or larger batches cost those products more in registers than they gain).
 // CT DiaCols: the uniform-offset layout, entry k of block row I at block column I + doff[k]
 // (clamped into the mb block columns; its values are zero where the row has no such block).
-template <class VT, int R, int C, class XT, class E, class CT, int KU>
+// SPW slices per wavefront (uniform-offset layout only): the wave issues the loads of all SPW
+// slices' batches together, SPW times the bytes in flight per wave for the short FP32 rows. A
+// slice past the end, or one of width 0, reads the first slice's first entries with zero values.
+// Each row's sum takes the same terms in the same order as with one slice per wave.
+template <class VT, int R, int C, class XT, class E, class CT, int KU, int SPW = 1>
 __global__ __launch_bounds__(256) void k_sell_stream(int nb, int nslices, const int* __restrict__ soff,
                                                     const int* __restrict__ swid, const CT* __restrict__ bc,
                                                     const VT* __restrict__ bv, size_t T, const XT* __restrict__ x,
                                                     E epi, const int* __restrict__ doff, int mb) {
   constexpr bool DIA = std::is_same<CT, DiaCols>::value;
+  static_assert(SPW == 1 || DIA, "several slices per wave need the uniform-offset layout");
   using AT = acc_t<VT, XT>;
-  const int sl = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
-  if (sl >= nslices) return;
+  const int sl0 = (xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6)) * SPW;
+  if (sl0 >= nslices) return;
   const int lane = threadIdx.x & 63;
-  const int I = sl * kSellRows + lane;
-  const bool live = I < nb;
-  const int base = live ? I : 0;
-  typename E::Pre pre[R];
-  if (live) {
+  int I[SPW], base[SPW], o[SPW], w[SPW];
+  bool live[SPW];
+  typename E::Pre pre[SPW][R];
+  AT acc[SPW][R];
+  int wmax = 0;
 #pragma unroll
-    for (int rr = 0; rr < R; ++rr) pre[rr] = epi.pre(I * R + rr);
-  }
-  const int o = soff[sl], w = swid[sl];
-  AT acc[R];
+  for (int s = 0; s < SPW; ++s) {
+    const int sl = sl0 + s;
+    const bool in = sl < nslices;
+    I[s] = sl * kSellRows + lane;
+    live[s] = in && I[s] < nb;
+    base[s] = live[s] ? I[s] : 0;
+    o[s] = in ? soff[sl] : 0;
+    w[s] = in ? swid[sl] : 0;
+    wmax = max(wmax, w[s]);
+    if (live[s]) {
 #pragma unroll
-  for (int rr = 0; rr < R; ++rr) acc[rr] = AT(0);
-  constexpr bool clamped = (sizeof(VT) == 4) || KU == 7 || DIA;
-  int k = 0;
-  for (; clamped ? k < w : k + KU <= w; k += KU) {
-    int cc[KU];
-    BsrVals<VT, R * C> vv[KU];
-    XT xv[KU][C];
-#pragma unroll
-    for (int u = 0; u < KU; ++u) {
-      const int kk = min(k + u, w - 1);
-      const size_t e = (size_t)o + (size_t)kk * kSellRows + lane;
-      if constexpr (DIA) cc[u] = min(max(base + doff[kk], 0), mb - 1);
-      else cc[u] = sell_col(bc, e, base);
-      vv[u] = sell_load<VT, R, C>(bv, e, T);
-      if (clamped && k + u >= w) {
-#pragma unroll
-        for (int q = 0; q < R * C; ++q) vv[u].a[q] = VT(0);
-      }
+      for (int rr = 0; rr < R; ++rr) pre[s][rr] = epi.pre(I[s] * R + rr);
     }
 #pragma unroll
-    for (int u = 0; u < KU; ++u) bsr_gather<C>(x, cc[u], xv[u]);
+    for (int rr = 0; rr < R; ++rr) acc[s][rr] = AT(0);
+  }
+  if constexpr (SPW > 1) {
+#pragma unroll
+    for (int s = 1; s < SPW; ++s)
+      if (w[s] == 0) o[s] = o[0];     // slice 0 is in range; width 0 only with an all-empty matrix
+  }
+  constexpr bool clamped = (sizeof(VT) == 4) || KU == 7 || DIA;
+  static_assert(SPW == 1 || clamped, "several slices per wave need clamped batches");
+  int k = 0;
+  for (; clamped ? k < wmax : k + KU <= w[0]; k += KU) {
+    int cc[SPW][KU];
+    BsrVals<VT, R * C> vv[SPW][KU];
+    XT xv[SPW][KU][C];
 #pragma unroll
     for (int u = 0; u < KU; ++u)
 #pragma unroll
-      for (int rr = 0; rr < R; ++rr) {
-        AT p = (AT)vv[u].a[rr * C] * (AT)xv[u][0];
+      for (int s = 0; s < SPW; ++s) {
+        const int kk = max(min(k + u, w[s] - 1), 0);
+        const size_t e = (size_t)o[s] + (size_t)kk * kSellRows + lane;
+        if constexpr (DIA) cc[s][u] = min(max(base[s] + doff[kk], 0), mb - 1);
+        else cc[s][u] = sell_col(bc, e, base[s]);
+        vv[s][u] = sell_load<VT, R, C>(bv, e, T);
+        if (clamped && k + u >= w[s]) {
 #pragma unroll
-        for (int q = 1; q < C; ++q) p += (AT)vv[u].a[rr * C + q] * (AT)xv[u][q];
-        acc[rr] += p;
+          for (int q = 0; q < R * C; ++q) vv[s][u].a[q] = VT(0);
+        }
       }
+#pragma unroll
+    for (int u = 0; u < KU; ++u)
+#pragma unroll
+      for (int s = 0; s < SPW; ++s) bsr_gather<C>(x, cc[s][u], xv[s][u]);
+#pragma unroll
+    for (int u = 0; u < KU; ++u)
+#pragma unroll
+      for (int s = 0; s < SPW; ++s)
+#pragma unroll
+        for (int rr = 0; rr < R; ++rr) {
+          AT p = (AT)vv[s][u].a[rr * C] * (AT)xv[s][u][0];
+#pragma unroll
+          for (int q = 1; q < C; ++q) p += (AT)vv[s][u].a[rr * C + q] * (AT)xv[s][u][q];
+          acc[s][rr] += p;
+        }
   }
-  for (; k < w; ++k) {   // remainder of the unclamped batches
-    const size_t e = (size_t)o + (size_t)k * kSellRows + lane;
-    const BsrVals<VT, R * C> vv = sell_load<VT, R, C>(bv, e, T);
-    XT xv[C];
-    int col;
-    if constexpr (DIA) col = min(max(base + doff[k], 0), mb - 1);
-    else col = sell_col(bc, e, base);
-    bsr_gather<C>(x, col, xv);
+  if constexpr (!clamped) {
+    for (; k < w[0]; ++k) {   // remainder of the unclamped batches (one slice per wave)
+      const size_t e = (size_t)o[0] + (size_t)k * kSellRows + lane;
+      const BsrVals<VT, R * C> vv = sell_load<VT, R, C>(bv, e, T);
+      XT xv[C];
+      const int col = sell_col(bc, e, base[0]);
+      bsr_gather<C>(x, col, xv);
 #pragma unroll
-    for (int rr = 0; rr < R; ++rr) {
-      AT p = (AT)vv.a[rr * C] * (AT)xv[0];
+      for (int rr = 0; rr < R; ++rr) {
+        AT p = (AT)vv.a[rr * C] * (AT)xv[0];
 #pragma unroll
-      for (int q = 1; q < C; ++q) p += (AT)vv.a[rr * C + q] * (AT)xv[q];
-      acc[rr] += p;
+        for (int q = 1; q < C; ++q) p += (AT)vv.a[rr * C + q] * (AT)xv[q];
+        acc[0][rr] += p;
+      }
     }
   }
-  if (live) {
 #pragma unroll
-    for (int rr = 0; rr < R; ++rr) epi(I * R + rr, acc[rr], pre[rr]);
-  }
+  for (int s = 0; s < SPW; ++s)
+    if (live[s]) {
+#pragma unroll
+      for (int rr = 0; rr < R; ++rr) epi(I[s] * R + rr, acc[s][rr], pre[s][rr]);
+    }
 }
+
+// FP32 uniform-offset products with two slices per wavefront: 0 never, 1 when a batch holds at
+// most 16 values per lane (default), 2 always
+extern int g_sell_pairs;
 
 template <class VT, class XT, class E>
 void bsr_stream_v(const deco_bsr* B, const VT* bv, const XT* x, E epi) {
@@ -959,7 +992,11 @@ void bsr_stream_v(const deco_bsr* B, const VT* bv, const XT* x, E epi) {
     const bool narrow = B->stot <= (size_t)4 * kSellRows * B->nslices;  // at most 4 on average
 #define DECO_SELL_LAUNCH_KU(RR, CC, KU)                                                                          \
   do {                                                                                                           \
-    if (B->dia)                                                                                                  \
+    if (B->dia && sizeof(VT) == 4 && (g_sell_pairs == 2 || (g_sell_pairs == 1 && KU * RR * CC <= 16)))          \
+      hipLaunchKernelGGL((k_sell_stream<VT, RR, CC, XT, E, DiaCols, KU, 2>), dim3((B->nslices + 7) / 8), dim3(256), \
+                         0, stream(), B->nb, B->nslices, B->soff.p, B->swid.p, (const DiaCols*)nullptr, bv,      \
+                         B->stot, x, epi, (const int*)B->doff.p, B->mb);                                         \
+    else if (B->dia)                                                                                             \
       hipLaunchKernelGGL((k_sell_stream<VT, RR, CC, XT, E, DiaCols, KU>), dim3(grid), dim3(256), 0, stream(),     \
                          B->nb, B->nslices, B->soff.p, B->swid.p, (const DiaCols*)nullptr, bv, B->stot, x, epi,   \
                          (const int*)B->doff.p, B->mb);                                                          \

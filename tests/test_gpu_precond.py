@@ -509,3 +509,30 @@ def test_transposed_cpr_sequence_and_value_update(gpu, cfg):
     M.set_precision(deco_amd.PREC_FP32)
     assert rel(M.apply_t(u).numpy(), t1) <= 2e-4
     M.set_precision(deco_amd.PREC_FP64)
+
+
+@pytest.mark.parametrize("bs", [2, 3])
+def test_sell_pairs_bit_identical(gpu, bs):
+    """FP32 products on the uniform-offset sliced block shadows with two slices per wavefront
+    (modes 1 and 2) against one slice per wavefront (mode 0): the CPR application, its transpose
+    and an FP32-preconditioned solve agree bit for bit. 23 x 19 x 5 cells make 35 slices of 64
+    cell rows, the last one partial, so the last wavefront's second slice lies past the end."""
+    g = gen_system(0, custom=(23, 19, 5, bs))
+    A = dev_csr(g)
+    M = deco_amd.CPR(small_params(bs), A, None)
+    M.set_layout(deco_amd.LAYOUT_BLOCK)
+    M.set_precision(deco_amd.PREC_FP32)
+    u = np.random.default_rng(5).uniform(-1, 1, g["n"])
+    sol = {"type": "rgmres", "reltol": "1e-8", "restart": "30", "itmax": "200", "verbosity": "0"}
+    out = {}
+    try:
+        for mode in (0, 1, 2):
+            deco_amd.set_sell_pairs(mode)
+            S = deco_amd.SOL(sol, A, M, g["b"])
+            out[mode] = (M.apply(u).numpy(), M.apply_t(u).numpy(), S.x.numpy())
+            assert S.flag == 0
+    finally:
+        deco_amd.set_sell_pairs(1)
+    for mode in (1, 2):
+        for a, b in zip(out[mode], out[0]):
+            assert np.array_equal(a, b)
