@@ -617,7 +617,7 @@ def set_stream_pieces(enable):
 
 def set_sell_pairs(mode):
     """FP32 products on uniform-offset sliced block shadows with two slices per wavefront: 0
-    never, 1 when a batch holds at most 16 values per lane (default), 2 always; the same sums
+    never, 1 when a batch holds at most 8 values per lane (default), 2 always; the same sums
     bit for bit."""
     check(lib().deco_set_sell_pairs(int(mode)))
 

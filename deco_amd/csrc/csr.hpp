@@ -980,7 +980,8 @@ __global__ __launch_bounds__(256) void k_sell_stream(int nb, int nslices, const 
 }
 
 // FP32 uniform-offset products with two slices per wavefront: 0 never, 1 when a batch holds at
-// most 16 values per lane (default), 2 always
+// most 8 values per lane (default: the scalar rows; measured per kernel, the 2 x 2 rows of 4 blocks
+// and the 1 x 2 rows of 7 run slower with two slices), 2 always
 extern int g_sell_pairs;
 
 template <class VT, class XT, class E>
@@ -992,7 +993,7 @@ void bsr_stream_v(const deco_bsr* B, const VT* bv, const XT* x, E epi) {
     const bool narrow = B->stot <= (size_t)4 * kSellRows * B->nslices;  // at most 4 on average
 #define DECO_SELL_LAUNCH_KU(RR, CC, KU)                                                                          \
   do {                                                                                                           \
-    if (B->dia && sizeof(VT) == 4 && (g_sell_pairs == 2 || (g_sell_pairs == 1 && KU * RR * CC <= 16)))          \
+    if (B->dia && sizeof(VT) == 4 && (g_sell_pairs == 2 || (g_sell_pairs == 1 && KU * RR * CC <= 8)))          \
       hipLaunchKernelGGL((k_sell_stream<VT, RR, CC, XT, E, DiaCols, KU, 2>), dim3((B->nslices + 7) / 8), dim3(256), \
                          0, stream(), B->nb, B->nslices, B->soff.p, B->swid.p, (const DiaCols*)nullptr, bv,      \
                          B->stot, x, epi, (const int*)B->doff.p, B->mb);                                         \

@@ -239,9 +239,8 @@ int deco_set_sell_dia(int enable);
  * the CSR-stream kernel. No reference counterpart. */
 int deco_set_stream_pieces(int enable);
 /* FP32 products on uniform-offset sliced block shadows with two 64-row slices per wavefront (the
- * loads of both slices' batches issued together): 0 never, 1 when a batch holds at most 16
- * values per lane (default: the FSAI factors' 2 x 2 rows of 4 blocks, the scalar pressure-level
- * rows, the pressure rows' 1 x 2 blocks), 2 always. The sums are the same bit for bit. No
+ * loads of both slices' batches issued together): 0 never, 1 when a batch holds at most 8
+ * values per lane (default: the scalar rows of the pressure level), 2 always. The sums are the same bit for bit. No
  * reference counterpart. */
 int deco_set_sell_pairs(int mode);
 /* When the CSR-stream products (those of the matrices above, and the plain CSR products with a
